@@ -1,0 +1,167 @@
+/*
+ * rcurves.h — C ABI of the MI355X Sobolev-step library (librcurves.so).
+ *
+ * Drop-in boundary for the reference's src/core energy / gradient /
+ * Sobolev-step API and its worker-solver messages. Plain pointers and sizes
+ * only; every entry point is synchronous on the calling thread and returns an
+ * rc_status. All device memory is owned by an rc_ctx (one HIP stream each);
+ * host buffers passed in are read/written only during the call.
+ *
+ * Layouts: vertices are AoS double[V][3] (the reference's Vec3[]), edges are
+ * int32[E][2] (Edge[]), flattened fields use the coordinate-block layout
+ * [x0..x{V-1}, y0.., z0..] (src/core/sobolev/layout.ts:23-58).
+ */
+#ifndef RCURVES_H
+#define RCURVES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RC_OK = 0,
+    RC_SINGULAR = 1,     /* linsolve.ts /singular/ throw contract */
+    RC_NONFINITE = 2,    /* linsolve.ts "treating as singular" (non-finite input) */
+    RC_BAD_ARG = 3,
+    RC_HIP_ERROR = 4,
+    RC_UNSUPPORTED = 5,
+} rc_status;
+
+typedef struct rc_ctx rc_ctx;
+
+/* Tangent-point exponents and regulariser (optimizer.ts:18 DEFAULTS). */
+typedef struct {
+    double alpha;
+    double beta;
+    double epsilon;
+} rc_params;
+
+/* One ConstraintBlock as POD (constraintSet.ts:39-52); blocks stack in array
+ * order exactly like a ConstraintSet. */
+enum { RC_BLOCK_BARYCENTER = 0, RC_BLOCK_TOTAL_LENGTH = 1, RC_BLOCK_EDGE_LENGTHS = 2, RC_BLOCK_POINT = 3 };
+typedef struct {
+    int32_t kind;
+    int32_t vertex;       /* RC_BLOCK_POINT: pinned vertex index */
+    double x0[3];         /* RC_BLOCK_BARYCENTER: frozen target */
+    double L0;            /* RC_BLOCK_TOTAL_LENGTH: frozen target */
+    double target[3];     /* RC_BLOCK_POINT: frozen pin target */
+    const double* ell0;   /* RC_BLOCK_EDGE_LENGTHS: frozen per-edge targets (host) */
+    int32_t ell0_len;
+    int32_t _pad;
+} rc_block;
+
+/* PenaltyConfig (penalties.ts:33-49): zero weight = off. */
+typedef struct {
+    double w_length;
+    double w_diff;
+    double w_field;
+    double X[3];
+} rc_penalties;
+
+enum { RC_PRECISION_FP64 = 0, RC_PRECISION_MIXED = 1 };
+
+/* SobolevStepOptions (optimizer.ts:95-175) + LineSearchOptions
+ * (lineSearch.ts:333-371) + ProjectConstraintSetOptions (:120-145). */
+typedef struct {
+    rc_params params;
+    int32_t has_energy_before;
+    int32_t precision;          /* RC_PRECISION_FP64 (plain) | RC_PRECISION_MIXED */
+    double energy_before;
+    int32_t projection_frozen;  /* must be 1: 'frozen' (store.ts:359 app default) */
+    int32_t max_iter;           /* projection maxIter, default 8 */
+    double tol_abs, tol_rel;    /* projection tolerances, default 1e-4 */
+    double c1, shrink, tau0, tau_min; /* Armijo: 1e-4, 0.5, 1, 1e-12 */
+    double refine_tol;          /* mixed form: PCG relative residual target, default 1e-12 */
+    int32_t refine_max_iter;    /* mixed form: PCG iteration cap, default 30 */
+    int32_t collect_timings;    /* fill rc_timings (hipEvent per phase) */
+} rc_step_opts;
+
+/* SobolevStepStats (optimizer.ts:196-202) + line-search diagnostics. */
+enum {
+    RC_REASON_NONE = 0,
+    RC_REASON_ZERO_OR_NONFINITE_GRADIENT_NORM = 1,
+    RC_REASON_NOT_A_DESCENT_DIRECTION = 2,
+    RC_REASON_ARMIJO_FAILED = 3,
+    RC_REASON_PROJECTION_FAILED = 4,
+    RC_REASON_TAU_BELOW_MIN = 5,
+    RC_REASON_SINGULAR_SYSTEM = 6,
+};
+typedef struct {
+    double energy;
+    double energy_before;
+    double tau;
+    double residual;          /* ||K z - rhs|| / max(1, ||rhs||), FP64, structured */
+    double gradient_l2_norm;
+    double slope;
+    double phi_norm;
+    double refine_residual;   /* final PCG relative residual (mixed form) */
+    int32_t accepted;
+    int32_t converged;
+    int32_t projection_iterations; /* -1 = null */
+    int32_t reason;
+    int32_t trials;
+    int32_t refine_iters;
+} rc_step_stats;
+
+/* Per-phase milliseconds, keys of phaseTimings.ts:9-20 (sub-phases overlap). */
+typedef struct {
+    double dE, energy, assembleA, saddle, factor, projection, lineSearch, step;
+} rc_timings;
+
+/* Fills defaults (optimizer.ts:18, lineSearch.ts:207-209,435-438). */
+void rc_default_step_opts(rc_step_opts* o);
+
+/* Topology message / calculateDisjointPairs (solverWorker.ts:37-43,
+ * tangentPointEnergy.ts:33-47): builds the device topology once per
+ * graphVersion. Disjointness is derived on the device from endpoints. */
+rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, rc_ctx** out);
+void rc_ctx_destroy(rc_ctx* ctx);
+/* Message of the last failing call on this context (never NULL). */
+const char* rc_last_error(const rc_ctx* ctx);
+
+/* calculateEnergy (tangentPointEnergy.ts:50-111). */
+rc_status rc_energy(rc_ctx* ctx, const double* xyz, const rc_params* p, double* out);
+/* gradientAnalytical (tangentPointEnergy.ts:139-392): adjoint of rc_energy,
+ * AoS double[V][3] out, ascent orientation. Optionally also the energy. */
+rc_status rc_gradient(rc_ctx* ctx, const double* xyz, const rc_params* p, double* grad,
+                      double* energy_out);
+/* assembleAFlat (innerProduct.ts:325-350): V x V row-major A = B + B0. */
+rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A);
+
+/* penaltyEnergy / penaltyGradient (penalties.ts:139-241). grad nullable. */
+rc_status rc_penalties_eval(rc_ctx* ctx, const double* xyz, const rc_penalties* pen,
+                            double* energy, double* grad);
+
+/* solveConstrainedGradientSetFrozen (gradient.ts:84-115): g~ and lambda for
+ * a given dE (AoS). lambda has rc_constraint_rows() entries. */
+rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xyz, const double* dE,
+                                        const rc_block* blocks, int32_t nblocks,
+                                        const rc_step_opts* o, double* gtilde, double* lambda,
+                                        double* residual);
+int32_t rc_constraint_rows(const rc_ctx* ctx, const rc_block* blocks, int32_t nblocks);
+
+/* sobolevStepSet (optimizer.ts:231-440), mode 'analytical',
+ * projectionMode 'frozen'. Positions in/out are host AoS; gtilde_out
+ * (collectField -> descentField) and timings are nullable. Never fails on a
+ * singular saddle: that folds into stats->reason like the reference. */
+rc_status rc_sobolev_step(rc_ctx* ctx, const double* xyz_in, const rc_block* blocks,
+                          int32_t nblocks, const rc_penalties* pen, const rc_step_opts* o,
+                          double* xyz_out, rc_step_stats* stats, double* gtilde_out,
+                          rc_timings* timings);
+
+/* Device-resident form for callers that keep the curve in HBM across steps:
+ * rc_set_positions uploads once, rc_sobolev_step_resident advances the
+ * context's device positions in place (an accepted step replaces them) with
+ * one host round trip (the stats), rc_get_positions downloads. */
+rc_status rc_set_positions(rc_ctx* ctx, const double* xyz);
+rc_status rc_get_positions(rc_ctx* ctx, double* xyz);
+rc_status rc_sobolev_step_resident(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
+                                   const rc_penalties* pen, const rc_step_opts* o,
+                                   rc_step_stats* stats, rc_timings* timings);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,799 @@
+// C ABI of librcurves: context lifecycle, topology tables and the step
+// orchestration (one HIP stream per context, synchronous entry points).
+#include <algorithm>
+#include <cstddef>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "rc_internal.h"
+#include "rc_step.h"
+
+rc_status rc_factor_fp64(rc_ctx* ctx);
+rc_status rc_solve_fp64(rc_ctx* ctx, double* d_B, int nrhs);
+rc_status rc_factor_mixed(rc_ctx* ctx);
+rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, double tol,
+                         int max_iter, double* d_refine_out);
+int rc_trsm_max_coresident(void);
+rc_status rc_launch_constraint_jacobian(rc_ctx* ctx, const DevSet* d_set, int k);
+rc_status rc_launch_penalty_grad(rc_ctx* ctx, const DevPen& pen, const double* gsrc, int Vsrc,
+                                 double* gdst, int Vdst);
+rc_status rc_launch_penalty_energy(rc_ctx* ctx, const DevPen& pen, double* d_out);
+rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B);
+rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k, double* gt,
+                          double* d_dots, double* d_bsum, double* d_sol, double* d_Minv);
+rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
+                             const double* d_sol, double* d_Ax, double* d_Cg, LSCtl* d_ctl);
+rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
+                             const double* e0tpe, const double* e0pen, int has_e0,
+                             double e0_given, double tau0, LSCtl* d_ctl);
+rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
+                             const rc_params* p, const double* pos0, const double* dir,
+                             double* cur, const double* Y, const double* Minv,
+                             double* d_epart_ls);
+rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
+                                int update, double* d_stats);
+
+namespace {
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+}
+
+__global__ void k_copy_refine(LSCtl* c, const double* r) {
+    c->refine_res = r[0];
+    c->refine_iters = (int)r[1];
+}
+
+__global__ void k_pick_output(const double* __restrict__ pos, const double* __restrict__ cur,
+                              const LSCtl* __restrict__ c, int V, double* __restrict__ o) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double* s = c->accepted ? cur : pos;
+    o[3 * v] = s[v];
+    o[3 * v + 1] = s[V + v];
+    o[3 * v + 2] = s[2 * V + v];
+}
+
+}  // namespace
+
+extern "C" void rc_default_step_opts(rc_step_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->params.alpha = 3.0;
+    o->params.beta = 6.0;
+    o->params.epsilon = 1e-10;
+    o->has_energy_before = 0;
+    o->precision = RC_PRECISION_FP64;
+    o->projection_frozen = 1;
+    o->max_iter = 8;
+    o->tol_abs = 1e-4;
+    o->tol_rel = 1e-4;
+    o->c1 = 1e-4;
+    o->shrink = 0.5;
+    o->tau0 = 1.0;
+    o->tau_min = 1e-12;
+    o->refine_tol = 1e-12;
+    o->refine_max_iter = 30;
+    o->collect_timings = 0;
+}
+
+extern "C" const char* rc_last_error(const rc_ctx* ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+static void free_all(rc_ctx* ctx) {
+    void* ptrs[] = {ctx->d_i1, ctx->d_i2, ctx->d_vOff, ctx->d_vEdge, ctx->d_vSlot, ctx->d_vLocal,
+                    ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_interior, ctx->d_pos,
+                    ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
+                    ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_A,
+                    ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv32, ctx->d_dscale,
+                    ctx->d_flags, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
+                    ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
+                    ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
+                    ctx->d_dE, ctx->d_gt, ctx->d_Y};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    void* sp[] = {ctx->d_set, ctx->d_ctl, ctx->d_dots, ctx->d_bsum, ctx->d_ssol, ctx->d_Minv,
+                  ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_refine,
+                  ctx->d_stage};
+    for (void* p : sp)
+        if (p) (void)hipFree(p);
+    if (ctx->d_host_stage) (void)hipHostFree(ctx->d_host_stage);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
+    if (!ctx) return;
+    free_all(ctx);
+    delete ctx;
+}
+
+static rc_status build_topology(rc_ctx* ctx, const int32_t* edges) {
+    const int V = ctx->V, E = ctx->E;
+    std::vector<int32_t> i1(ctx->Epad, -1), i2(ctx->Epad, -2);
+    std::vector<int32_t> deg(V, 0);
+    for (int K = 0; K < E; ++K) {
+        const int a = edges[2 * K], b = edges[2 * K + 1];
+        if (a < 0 || a >= V || b < 0 || b >= V) {
+            ctx->err = "edge " + std::to_string(K) + " has an endpoint outside [0, V)";
+            return RC_BAD_ARG;
+        }
+        if (a == b) {
+            ctx->err = "edge " + std::to_string(K) + " is a self-loop";
+            return RC_BAD_ARG;
+        }
+        i1[K] = a;
+        i2[K] = b;
+        deg[a]++;
+        deg[b]++;
+    }
+    std::vector<int32_t> vOff(V + 1, 0), vEdge(2 * (size_t)E), vSlot(2 * (size_t)E),
+        vLocal(2 * (size_t)E);
+    for (int v = 0; v < V; ++v) vOff[v + 1] = vOff[v] + deg[v];
+    std::vector<int32_t> fill(vOff.begin(), vOff.end() - 1);
+    for (int K = 0; K < E; ++K) {
+        vEdge[fill[i1[K]]] = K;
+        vSlot[fill[i1[K]]++] = 0;
+        vEdge[fill[i2[K]]] = K;
+        vSlot[fill[i2[K]]++] = 1;
+    }
+    int maxdeg = 0;
+    bool iso = false;
+    for (int v = 0; v < V; ++v) {
+        maxdeg = std::max(maxdeg, deg[v]);
+        if (deg[v] == 0) iso = true;
+    }
+    ctx->maxdeg = maxdeg;
+    ctx->has_isolated = iso;
+    // vertex tiles and their incident edge lists
+    const int nvt = ctx->nvt;
+    std::vector<int32_t> rOff(nvt + 1, 0), rEdge;
+    for (int t = 0; t < nvt; ++t) {
+        std::vector<int32_t> lst;
+        for (int v = t * RC_VTILE; v < std::min(V, (t + 1) * RC_VTILE); ++v)
+            for (int q = vOff[v]; q < vOff[v + 1]; ++q) lst.push_back(vEdge[q]);
+        std::sort(lst.begin(), lst.end());
+        lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+        if ((int)lst.size() > RC_RMAX) {
+            ctx->err = "vertex tile " + std::to_string(t) + " touches " +
+                       std::to_string(lst.size()) + " edges (max " + std::to_string(RC_RMAX) +
+                       "); renumber vertices along the curve";
+            return RC_UNSUPPORTED;
+        }
+        for (int v = t * RC_VTILE; v < std::min(V, (t + 1) * RC_VTILE); ++v)
+            for (int q = vOff[v]; q < vOff[v + 1]; ++q)
+                vLocal[q] = (int32_t)(std::lower_bound(lst.begin(), lst.end(), vEdge[q]) -
+                                      lst.begin());
+        rEdge.insert(rEdge.end(), lst.begin(), lst.end());
+        rOff[t + 1] = (int32_t)rEdge.size();
+    }
+    std::vector<int32_t> owner(E);
+    for (int K = 0; K < E; ++K) owner[K] = i1[K] / RC_VTILE;
+
+    RC_HIP(dalloc(&ctx->d_i1, ctx->Epad));
+    RC_HIP(dalloc(&ctx->d_i2, ctx->Epad));
+    RC_HIP(dalloc(&ctx->d_vOff, V + 1));
+    RC_HIP(dalloc(&ctx->d_vEdge, 2 * (size_t)E));
+    RC_HIP(dalloc(&ctx->d_vSlot, 2 * (size_t)E));
+    RC_HIP(dalloc(&ctx->d_vLocal, 2 * (size_t)E));
+    RC_HIP(dalloc(&ctx->d_rOff, nvt + 1));
+    RC_HIP(dalloc(&ctx->d_rEdge, rEdge.size()));
+    RC_HIP(dalloc(&ctx->d_owner, E));
+    RC_HIP(hipMemcpy(ctx->d_i1, i1.data(), sizeof(int32_t) * ctx->Epad, hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_i2, i2.data(), sizeof(int32_t) * ctx->Epad, hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_vOff, vOff.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_vEdge, vEdge.data(), sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_vSlot, vSlot.data(), sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_vLocal, vLocal.data(), sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_rOff, rOff.data(), sizeof(int32_t) * (nvt + 1), hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_rEdge, rEdge.data(), sizeof(int32_t) * rEdge.size(),
+                     hipMemcpyHostToDevice));
+    RC_HIP(hipMemcpy(ctx->d_owner, owner.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    ctx->h_edges.assign(edges, edges + 2 * (size_t)E);
+    return RC_OK;
+}
+
+extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, rc_ctx** out) {
+    if (!out) return RC_BAD_ARG;
+    *out = nullptr;
+    if (!edges || E < 1 || V < 2) return RC_BAD_ARG;
+    rc_ctx* ctx = new (std::nothrow) rc_ctx();
+    if (!ctx) return RC_BAD_ARG;
+    ctx->V = V;
+    ctx->E = E;
+    ctx->Epad = ((E + RC_TILE - 1) / RC_TILE) * RC_TILE;
+    ctx->ntiles = ctx->Epad / RC_TILE;
+    ctx->npairs = ctx->ntiles * (ctx->ntiles + 1) / 2;
+    ctx->nvt = (V + RC_VTILE - 1) / RC_VTILE;
+    ctx->ngram = ctx->nvt * (ctx->nvt + 1) / 2;
+    ctx->nblk = (V + RC_NB - 1) / RC_NB;
+    auto fail = [&](rc_status s) {
+        *out = ctx;  // keep the context so rc_last_error can report; caller destroys
+        return s;
+    };
+    hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        ctx->err = std::string("hipStreamCreate: ") + hipGetErrorString(he);
+        return fail(RC_HIP_ERROR);
+    }
+    rc_status s = build_topology(ctx, edges);
+    if (s != RC_OK) return fail(s);
+    const int Vp = ctx->nblk * RC_NB;
+    const int Epad = ctx->Epad;
+#define ALLOC(ptr, n)                                                        \
+    do {                                                                     \
+        hipError_t _e = dalloc(&(ptr), (n));                                 \
+        if (_e != hipSuccess) {                                              \
+            ctx->err = std::string("hipMalloc(" #ptr "): ") + hipGetErrorString(_e); \
+            return fail(RC_HIP_ERROR);                                       \
+        }                                                                    \
+    } while (0)
+    ALLOC(ctx->d_pos, 3 * (size_t)V);
+    ALLOC(ctx->d_work, 3 * (size_t)V);
+    ALLOC(ctx->d_edge_block, 14 * (size_t)Epad);
+    double* eb = ctx->d_edge_block;
+    double** fields[] = {&ctx->ed.ax, &ctx->ed.ay, &ctx->ed.az, &ctx->ed.bx, &ctx->ed.by,
+                         &ctx->ed.bz, &ctx->ed.ell, &ctx->ed.rlen, &ctx->ed.ux, &ctx->ed.uy,
+                         &ctx->ed.uz, &ctx->ed.lpow, &ctx->ed.dpc};
+    for (int i = 0; i < 13; ++i) *fields[i] = eb + (size_t)i * Epad;
+    ALLOC(ctx->d_edge_iblock, 2 * (size_t)Epad);
+    ctx->ed.i1 = ctx->d_edge_iblock;
+    ctx->ed.i2 = ctx->d_edge_iblock + Epad;
+    ALLOC(ctx->d_epart, ctx->npairs);
+    ALLOC(ctx->d_gpart, (size_t)ctx->ntiles * 6 * Epad);
+    ALLOC(ctx->d_grad, 3 * (size_t)Vp);
+    ALLOC(ctx->d_scal, 16);
+    ALLOC(ctx->d_rsp, (size_t)ctx->nvt * 2 * ctx->E);
+    ALLOC(ctx->d_rs, 2 * (size_t)ctx->E);
+    ALLOC(ctx->d_A, (size_t)V * V);
+    ALLOC(ctx->d_F, (size_t)Vp * Vp);
+    ALLOC(ctx->d_Linv, (size_t)ctx->nblk * RC_NB * RC_NB);
+    ALLOC(ctx->d_flags, ctx->nblk);
+    ALLOC(ctx->d_status, 4);
+    ctx->nrhs_cap = 3 + 3 * RC_KMAX;
+    ALLOC(ctx->d_rhs, (size_t)ctx->nrhs_cap * Vp);
+    ALLOC(ctx->d_sol, (size_t)ctx->nrhs_cap * Vp);
+    ALLOC(ctx->d_tmp, 3 * (size_t)Vp);
+    ALLOC(ctx->d_C, (size_t)RC_KMAX * 3 * Vp);
+    ALLOC(ctx->d_ell0, ctx->E);
+    ALLOC(ctx->d_dir, 3 * (size_t)Vp);
+    ALLOC(ctx->d_dE, 3 * (size_t)Vp);
+    ALLOC(ctx->d_gt, 3 * (size_t)Vp);
+    ALLOC(ctx->d_set, 1);
+    ALLOC(ctx->d_ctl, 1);
+    ALLOC(ctx->d_dots, (size_t)RC_KMAX * (RC_KMAX + 1));
+    ALLOC(ctx->d_bsum, 3 * (RC_KMAX + 1));
+    ALLOC(ctx->d_ssol, RC_KMAX + 3);
+    ALLOC(ctx->d_Minv, (size_t)(RC_KMAX + 3) * (RC_KMAX + 3));
+    ALLOC(ctx->d_Ax, 3 * (size_t)Vp);
+    ALLOC(ctx->d_Cg, RC_KMAX);
+    ALLOC(ctx->d_stats, 16);
+    ALLOC(ctx->d_epart_ls, 1024);
+    ALLOC(ctx->d_refine, 8);
+    ALLOC(ctx->d_stage, 3 * (size_t)V);
+#undef ALLOC
+    he = hipHostMalloc((void**)&ctx->d_host_stage, 4096, hipHostMallocDefault);
+    if (he != hipSuccess) {
+        ctx->err = std::string("hipHostMalloc: ") + hipGetErrorString(he);
+        return fail(RC_HIP_ERROR);
+    }
+    ctx->host_stage_bytes = 4096;
+    he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk);
+    if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
+    if (he != hipSuccess) {
+        ctx->err = std::string("hipMemset: ") + hipGetErrorString(he);
+        return fail(RC_HIP_ERROR);
+    }
+    // edge endpoint tables for the pair kernels
+    he = hipMemcpy(ctx->ed.i1, ctx->d_i1, sizeof(int32_t) * Epad, hipMemcpyDeviceToDevice);
+    if (he == hipSuccess)
+        he = hipMemcpy(ctx->ed.i2, ctx->d_i2, sizeof(int32_t) * Epad, hipMemcpyDeviceToDevice);
+    if (he != hipSuccess) {
+        ctx->err = std::string("hipMemcpy: ") + hipGetErrorString(he);
+        return fail(RC_HIP_ERROR);
+    }
+    const int cores = rc_trsm_max_coresident();
+    if (cores < ctx->nblk) {
+        ctx->err = "triangular solve needs " + std::to_string(ctx->nblk) +
+                   " co-resident workgroups, device holds " + std::to_string(cores);
+        return fail(RC_UNSUPPORTED);
+    }
+    *out = ctx;
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+static bool params_ok(const rc_params* p) {
+    return p && std::isfinite(p->alpha) && std::isfinite(p->beta) && std::isfinite(p->epsilon) &&
+           p->alpha != 0.0;
+}
+
+extern "C" rc_status rc_set_positions(rc_ctx* ctx, const double* xyz) {
+    if (!ctx || !xyz) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s != RC_OK) return s;
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_get_positions(rc_ctx* ctx, double* xyz) {
+    if (!ctx || !xyz) return RC_BAD_ARG;
+    return rc_download_positions_aos(ctx, ctx->d_pos, xyz);
+}
+
+extern "C" rc_status rc_energy(rc_ctx* ctx, const double* xyz, const rc_params* p, double* out) {
+    if (!ctx || !xyz || !out || !params_ok(p)) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s == RC_OK) s = rc_launch_energy(ctx, p, ctx->d_scal);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_scal, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->d_host_stage[0];
+    return RC_OK;
+}
+
+extern "C" rc_status rc_gradient(rc_ctx* ctx, const double* xyz, const rc_params* p, double* grad,
+                                 double* energy_out) {
+    if (!ctx || !xyz || !grad || !params_ok(p)) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s == RC_OK && energy_out) s = rc_launch_energy(ctx, p, ctx->d_scal);
+    if (s == RC_OK) s = rc_launch_gradient(ctx, p, ctx->d_dE);
+    if (s != RC_OK) return s;
+    if (energy_out) {
+        RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_scal, sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    s = rc_download_positions_aos(ctx, ctx->d_dE, grad);
+    if (s != RC_OK) return s;
+    if (energy_out) *energy_out = ctx->d_host_stage[0];
+    return RC_OK;
+}
+
+extern "C" rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A) {
+    if (!ctx || !xyz || !A || !params_ok(p)) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s == RC_OK) s = rc_launch_gram(ctx, p);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemcpyAsync(A, ctx->d_A, sizeof(double) * (size_t)ctx->V * ctx->V,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    return RC_OK;
+}
+
+static DevPen resolve_pen(const rc_penalties* pen) {
+    DevPen d{};
+    if (!pen) return d;
+    d.wL = pen->w_length;
+    d.wD = pen->w_diff;
+    d.wF = pen->w_field;
+    if (d.wF != 0.0) {
+        const double n = std::sqrt(pen->X[0] * pen->X[0] + pen->X[1] * pen->X[1] + pen->X[2] * pen->X[2]);
+        if (n < 1e-14) {
+            d.wF = 0.0;
+        } else {
+            d.X[0] = pen->X[0] / n;
+            d.X[1] = pen->X[1] / n;
+            d.X[2] = pen->X[2] / n;
+        }
+    }
+    d.active = (d.wL != 0.0 || d.wD != 0.0 || d.wF != 0.0) ? 1 : 0;
+    return d;
+}
+
+extern "C" rc_status rc_penalties_eval(rc_ctx* ctx, const double* xyz, const rc_penalties* pen,
+                                       double* energy, double* grad) {
+    if (!ctx || !xyz || !energy) return RC_BAD_ARG;
+    rc_params p{3.0, 6.0, 1e-10};
+    const DevPen dp = resolve_pen(pen);
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, &p);
+    if (s == RC_OK) s = rc_launch_penalty_energy(ctx, dp, ctx->d_scal + 4);
+    if (s != RC_OK) return s;
+    if (grad) {
+        const int Vp = ctx->nblk * RC_NB;
+        RC_HIP(hipMemsetAsync(ctx->d_tmp, 0, sizeof(double) * 3 * Vp, ctx->stream));
+        s = rc_launch_penalty_grad(ctx, dp, ctx->d_tmp, Vp, ctx->d_dE, ctx->V);
+        if (s == RC_OK) s = rc_download_positions_aos(ctx, ctx->d_dE, grad);
+        if (s != RC_OK) return s;
+    }
+    RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_scal + 4, sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    *energy = ctx->d_host_stage[0];
+    return RC_OK;
+}
+
+// Translate the POD block list into the device descriptor (set order kept).
+static rc_status make_set(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks, DevSet& ds) {
+    std::memset(&ds, 0, sizeof(ds));
+    if (nblocks < 0 || nblocks > RC_MAXPIN + 2 || (nblocks > 0 && !blocks)) {
+        ctx->err = "constraint block list: bad count";
+        return RC_BAD_ARG;
+    }
+    int row = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const rc_block& B = blocks[b];
+        ds.kind[b] = B.kind;
+        ds.row0[b] = row;
+        switch (B.kind) {
+            case RC_BLOCK_BARYCENTER:
+                if (ds.has_bary) {
+                    ctx->err = "more than one barycenter block";
+                    return RC_UNSUPPORTED;
+                }
+                ds.has_bary = 1;
+                std::memcpy(ds.x0, B.x0, sizeof(ds.x0));
+                ds.rows[b] = 3;
+                break;
+            case RC_BLOCK_TOTAL_LENGTH:
+                if (ds.has_len) {
+                    ctx->err = "more than one length block";
+                    return RC_UNSUPPORTED;
+                }
+                ds.has_len = 1;
+                ds.L0 = B.L0;
+                ds.len_row0 = row;
+                ds.rows[b] = 1;
+                break;
+            case RC_BLOCK_EDGE_LENGTHS: {
+                if (ds.has_len) {
+                    ctx->err = "more than one length block";
+                    return RC_UNSUPPORTED;
+                }
+                ds.has_len = 1;
+                ds.len_per_edge = 1;
+                ds.len_row0 = row;
+                ds.rows[b] = ctx->E;
+                std::vector<double> e0(ctx->E, NAN);
+                for (int i = 0; i < ctx->E && B.ell0 && i < B.ell0_len; ++i) e0[i] = B.ell0[i];
+                RC_HIP(hipMemcpy(ctx->d_ell0, e0.data(), sizeof(double) * ctx->E,
+                                 hipMemcpyHostToDevice));
+                break;
+            }
+            case RC_BLOCK_POINT:
+                if (ds.npin >= RC_MAXPIN) {
+                    ctx->err = "too many point blocks";
+                    return RC_UNSUPPORTED;
+                }
+                ds.pin_of_block[b] = ds.npin;
+                ds.pin_vertex[ds.npin] = B.vertex;
+                std::memcpy(ds.pin_target[ds.npin], B.target, sizeof(double) * 3);
+                ds.npin++;
+                ds.rows[b] = 3;
+                break;
+            default:
+                ctx->err = "unknown constraint block kind";
+                return RC_BAD_ARG;
+        }
+        row += ds.rows[b];
+    }
+    ds.nblocks = nblocks;
+    ds.k = row;
+    if (row > RC_KMAX) {
+        ctx->err = "constraint rows " + std::to_string(row) + " exceed the device limit " +
+                   std::to_string(RC_KMAX);
+        return RC_UNSUPPORTED;
+    }
+    return RC_OK;
+}
+
+extern "C" int32_t rc_constraint_rows(const rc_ctx* ctx, const rc_block* blocks, int32_t nblocks) {
+    if (!ctx) return -1;
+    int k = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        switch (blocks[b].kind) {
+            case RC_BLOCK_BARYCENTER: k += 3; break;
+            case RC_BLOCK_TOTAL_LENGTH: k += 1; break;
+            case RC_BLOCK_EDGE_LENGTHS: k += ctx->E; break;
+            case RC_BLOCK_POINT: k += 3; break;
+            default: break;
+        }
+    }
+    return k;
+}
+
+__global__ void k_aos_to_blocks(const double* __restrict__ aos, int V, int S,
+                                double* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= S) return;
+    for (int c = 0; c < 3; ++c) out[(size_t)c * S + v] = (v < V) ? aos[3 * v + c] : 0.0;
+}
+
+__global__ void k_blocks_to_aos(const double* __restrict__ in, int V, int S,
+                                double* __restrict__ aos) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (int c = 0; c < 3; ++c) aos[3 * v + c] = in[(size_t)c * S + v];
+}
+
+// Gradient-solve phase shared by rc_solve_constrained_gradient and the step:
+// assembly, constraints, factor, solves, Schur system, g~ and residual.
+// dE (stride Vp) must be resident in ctx->d_dE; edge table at gamma0.
+static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, int precision,
+                             const rc_step_opts* o, hipEvent_t* ev_asm0, hipEvent_t* ev_asm1,
+                             hipEvent_t* ev_fac1) {
+    const int k = ds.k;
+    RC_HIP(hipMemcpyAsync(ctx->d_set, &ds, sizeof(DevSet), hipMemcpyHostToDevice, ctx->stream));
+    if (ev_asm0) RC_HIP(hipEventRecord(*ev_asm0, ctx->stream));
+    rc_status s = rc_launch_gram(ctx, p);
+    if (s != RC_OK) return s;
+    if (ev_asm1) RC_HIP(hipEventRecord(*ev_asm1, ctx->stream));
+    if (k > 0) {
+        s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
+        if (s != RC_OK) return s;
+    }
+    s = (precision == RC_PRECISION_MIXED) ? rc_factor_mixed(ctx) : rc_factor_fp64(ctx);
+    if (s != RC_OK) return s;
+    if (ctx->has_isolated) {
+        // isolated vertex: zero rows of A and C -> singular saddle (optimizer-sobolev.test.ts:83-105)
+        int32_t one = 1;
+        RC_HIP(hipMemcpyAsync(ctx->d_status, &one, sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+    if (ev_fac1) RC_HIP(hipEventRecord(*ev_fac1, ctx->stream));
+    const int nrhs = 3 + 3 * k;
+    s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs);
+    if (s != RC_OK) return s;
+    if (precision == RC_PRECISION_MIXED) {
+        s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, o ? o->refine_tol : 1e-13,
+                           o ? o->refine_max_iter : 30, ctx->d_refine);
+        if (s != RC_OK) return s;
+    } else {
+        const int Vp = ctx->nblk * RC_NB;
+        RC_HIP(hipMemcpyAsync(ctx->d_sol, ctx->d_rhs, sizeof(double) * (size_t)nrhs * Vp,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        s = rc_solve_fp64(ctx, ctx->d_sol, nrhs);
+        if (s != RC_OK) return s;
+        RC_HIP(hipMemsetAsync(ctx->d_refine, 0, sizeof(double) * 2, ctx->stream));
+    }
+    s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_gt, ctx->d_dots, ctx->d_bsum, ctx->d_ssol,
+                        ctx->d_Minv);
+    if (s != RC_OK) return s;
+    s = rc_launch_residual(ctx, ctx->d_gt, ctx->d_dE, k, ctx->d_ssol, ctx->d_Ax, ctx->d_Cg, ctx->d_ctl);
+    return s;
+}
+
+extern "C" rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xyz,
+                                                   const double* dE, const rc_block* blocks,
+                                                   int32_t nblocks, const rc_step_opts* o,
+                                                   double* gtilde, double* lambda,
+                                                   double* residual) {
+    if (!ctx || !xyz || !dE || !gtilde) return RC_BAD_ARG;
+    rc_step_opts def;
+    if (!o) {
+        rc_default_step_opts(&def);
+        o = &def;
+    }
+    if (!params_ok(&o->params)) return RC_BAD_ARG;
+    DevSet ds;
+    rc_status s = make_set(ctx, blocks, nblocks, ds);
+    if (s != RC_OK) return s;
+    const int Vp = ctx->nblk * RC_NB;
+    s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, &o->params);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemcpyAsync(ctx->d_work, dE, sizeof(double) * 3 * ctx->V, hipMemcpyHostToDevice,
+                          ctx->stream));
+    hipLaunchKernelGGL(k_aos_to_blocks, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_work, ctx->V, Vp, ctx->d_dE);
+    RC_CHECK_LAUNCH();
+    s = solve_phase(ctx, &o->params, ds, o->precision, o, nullptr, nullptr, nullptr);
+    if (s != RC_OK) return s;
+    hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_gt, ctx->V, Vp, ctx->d_work);
+    RC_CHECK_LAUNCH();
+    RC_HIP(hipMemcpyAsync(gtilde, ctx->d_work, sizeof(double) * 3 * ctx->V, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    std::vector<double> sol(ds.k + 3);
+    RC_HIP(hipMemcpyAsync(sol.data(), ctx->d_ssol, sizeof(double) * (ds.k + 3), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    LSCtl ctl;
+    RC_HIP(hipMemcpyAsync(&ctl, ctx->d_ctl, sizeof(LSCtl), hipMemcpyDeviceToHost, ctx->stream));
+    int32_t st[4];
+    RC_HIP(hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    if (st[0] & 2) {
+        ctx->err = "triangular solve dependency wait timed out";
+        return RC_HIP_ERROR;
+    }
+    if (lambda)
+        for (int i = 0; i < ds.k; ++i) lambda[i] = sol[i];
+    if (residual) *residual = ctl.residual;
+    if (st[0] & (1 | 4)) {
+        ctx->err = "saddle system is singular";
+        return RC_SINGULAR;
+    }
+    return RC_OK;
+}
+
+static int trial_count(double tau0, double shrink, double tau_min) {
+    int n = 0;
+    double t = tau0;
+    while (t >= tau_min && n < 4096) {
+        ++n;
+        t *= shrink;
+    }
+    return n;
+}
+
+// Device-resident step on ctx->d_pos. update: replace d_pos with the accepted
+// curve. Stats (and g~ / positions when requested) are the single D2H copy.
+static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
+                           const rc_penalties* pen, const rc_step_opts* o, rc_step_stats* stats,
+                           rc_timings* timings, double* gtilde_out, int update,
+                           double* xyz_out) {
+    if (!ctx || !o || !stats || !params_ok(&o->params)) return RC_BAD_ARG;
+    if (!o->projection_frozen) {
+        ctx->err = "projectionMode 'reassemble' is not available on the device";
+        return RC_UNSUPPORTED;
+    }
+    if (!(o->shrink > 0.0 && o->shrink < 1.0) || !(o->tau0 > 0.0) || o->max_iter < 0) {
+        ctx->err = "line-search options out of range";
+        return RC_BAD_ARG;
+    }
+    DevSet ds;
+    rc_status s = make_set(ctx, blocks, nblocks, ds);
+    if (s != RC_OK) return s;
+    const DevPen dp = resolve_pen(pen);
+    const rc_params* p = &o->params;
+    const int Vp = ctx->nblk * RC_NB;
+    const bool tm = o->collect_timings && timings;
+    hipEvent_t ev[10] = {};
+    if (tm)
+        for (auto& e : ev) RC_HIP(hipEventCreate(&e));
+    if (tm) RC_HIP(hipEventRecord(ev[0], ctx->stream));
+
+    s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s != RC_OK) return s;
+    // E0 = objective at gamma0 unless the caller supplies it (optimizer.ts:117-132)
+    if (!o->has_energy_before) {
+        s = rc_launch_energy(ctx, p, ctx->d_scal + 2);
+        if (s == RC_OK) s = rc_launch_penalty_energy(ctx, dp, ctx->d_scal + 3);
+        if (s != RC_OK) return s;
+    }
+    if (tm) RC_HIP(hipEventRecord(ev[1], ctx->stream));
+    // dE = gradient + penalty gradient (optimizer.ts:278-294), stride Vp
+    s = rc_launch_gradient(ctx, p, ctx->d_grad);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemsetAsync(ctx->d_dE, 0, sizeof(double) * 3 * Vp, ctx->stream));
+    s = rc_launch_penalty_grad(ctx, dp, ctx->d_grad, ctx->V, ctx->d_dE, Vp);
+    if (s != RC_OK) return s;
+    if (tm) RC_HIP(hipEventRecord(ev[2], ctx->stream));
+
+    // control block
+    LSCtl hc;
+    std::memset(&hc, 0, sizeof(hc));
+    hc.c1 = o->c1;
+    hc.shrink = o->shrink;
+    hc.tau_min = o->tau_min;
+    hc.tol_abs = o->tol_abs;
+    hc.tol_rel = o->tol_rel;
+    hc.max_iter = o->max_iter;
+    RC_HIP(hipMemcpyAsync(ctx->d_ctl, &hc, sizeof(LSCtl), hipMemcpyHostToDevice, ctx->stream));
+
+    s = solve_phase(ctx, p, ds, o->precision, o, tm ? &ev[3] : nullptr, tm ? &ev[4] : nullptr,
+                    tm ? &ev[5] : nullptr);
+    if (s != RC_OK) return s;
+    if (tm) RC_HIP(hipEventRecord(ev[6], ctx->stream));
+    s = rc_launch_ls_setup(ctx, ctx->d_gt, ctx->d_dE, ctx->d_dir, ctx->d_scal + 2,
+                           ctx->d_scal + 3, o->has_energy_before, o->energy_before, o->tau0,
+                           ctx->d_ctl);
+    if (s != RC_OK) return s;
+    // refine diagnostics into the control block
+    hipLaunchKernelGGL(k_copy_refine, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctl,
+                       (const double*)ctx->d_refine);
+    RC_CHECK_LAUNCH();
+
+    const int ntr = trial_count(o->tau0, o->shrink, o->tau_min);
+    const int batches[3] = {4, 12, 1 << 20};
+    int done_trials = 0;
+    for (int bi = 0; bi < 3 && done_trials < ntr; ++bi) {
+        const int nb = std::min(batches[bi], ntr - done_trials);
+        for (int i = 0; i < nb; ++i) {
+            s = rc_launch_ls_trial(ctx, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
+                                   ctx->d_work, ctx->d_sol + 3 * (size_t)Vp, ctx->d_Minv,
+                                   ctx->d_epart_ls);
+            if (s != RC_OK) return s;
+        }
+        done_trials += nb;
+        if (done_trials < ntr) {
+            RC_HIP(hipMemcpyAsync(ctx->d_host_stage, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            RC_HIP(hipStreamSynchronize(ctx->stream));
+            if (*(int32_t*)ctx->d_host_stage) break;
+        }
+    }
+    if (tm) RC_HIP(hipEventRecord(ev[7], ctx->stream));
+    s = rc_launch_ls_finalize(ctx, ctx->d_ctl, ctx->d_pos, ctx->d_work, update, ctx->d_stats);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_stats, 16 * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    int32_t* hst = (int32_t*)(ctx->d_host_stage + 16);
+    RC_HIP(hipMemcpyAsync(hst, ctx->d_status, sizeof(int32_t) * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    if (gtilde_out) {
+        hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                           ctx->d_gt, ctx->V, Vp, ctx->d_tmp);
+        RC_CHECK_LAUNCH();
+        RC_HIP(hipMemcpyAsync(gtilde_out, ctx->d_tmp, sizeof(double) * 3 * ctx->V,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (xyz_out) {
+        // accepted -> the projected trial curve, otherwise the input echo
+        hipLaunchKernelGGL(k_pick_output, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const double*)ctx->d_pos, (const double*)ctx->d_work,
+                           (const LSCtl*)ctx->d_ctl, ctx->V, ctx->d_stage);
+        RC_CHECK_LAUNCH();
+        RC_HIP(hipMemcpyAsync(xyz_out, ctx->d_stage, sizeof(double) * 3 * ctx->V,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (tm) RC_HIP(hipEventRecord(ev[8], ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    if (hst[0] & 2) {
+        ctx->err = "triangular solve dependency wait timed out";
+        return RC_HIP_ERROR;
+    }
+    const double* d = ctx->d_host_stage;
+    stats->energy = d[0];
+    stats->energy_before = d[1];
+    stats->tau = d[2];
+    stats->residual = d[3];
+    stats->gradient_l2_norm = d[4];
+    stats->slope = d[5];
+    stats->phi_norm = d[6];
+    stats->refine_residual = d[7];
+    const int32_t* is = (const int32_t*)(d + 8);
+    stats->accepted = is[0];
+    stats->converged = is[1];
+    stats->projection_iterations = is[2];
+    stats->reason = is[3];
+    stats->trials = is[4];
+    stats->refine_iters = is[5];
+    if (tm) {
+        float ms[9] = {};
+        for (int i = 1; i <= 8; ++i) RC_HIP(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
+        std::memset(timings, 0, sizeof(*timings));
+        timings->energy = ms[1];
+        timings->dE = ms[2];
+        // ev[2]->ev[3]: control upload; ev[3]->ev[4] assembly
+        float asm_ms = 0, fac_ms = 0, sad_ms = 0, ls_ms = 0, step_ms = 0;
+        RC_HIP(hipEventElapsedTime(&asm_ms, ev[3], ev[4]));
+        RC_HIP(hipEventElapsedTime(&fac_ms, ev[4], ev[5]));
+        RC_HIP(hipEventElapsedTime(&sad_ms, ev[4], ev[6]));
+        RC_HIP(hipEventElapsedTime(&ls_ms, ev[6], ev[7]));
+        RC_HIP(hipEventElapsedTime(&step_ms, ev[0], ev[8]));
+        timings->assembleA = asm_ms;
+        timings->factor = fac_ms;
+        timings->saddle = sad_ms;
+        timings->lineSearch = ls_ms;
+        timings->projection = 0.0;
+        timings->step = step_ms;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return RC_OK;
+}
+
+extern "C" rc_status rc_sobolev_step_resident(rc_ctx* ctx, const rc_block* blocks,
+                                              int32_t nblocks, const rc_penalties* pen,
+                                              const rc_step_opts* o, rc_step_stats* stats,
+                                              rc_timings* timings) {
+    return step_impl(ctx, blocks, nblocks, pen, o, stats, timings, nullptr, 1, nullptr);
+}
+
+extern "C" rc_status rc_sobolev_step(rc_ctx* ctx, const double* xyz_in, const rc_block* blocks,
+                                     int32_t nblocks, const rc_penalties* pen,
+                                     const rc_step_opts* o, double* xyz_out, rc_step_stats* stats,
+                                     double* gtilde_out, rc_timings* timings) {
+    if (!ctx || !xyz_in || !xyz_out) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz_in, ctx->d_pos);
+    if (s != RC_OK) return s;
+    return step_impl(ctx, blocks, nblocks, pen, o, stats, timings, gtilde_out, 0, xyz_out);
+}

@@ -1,0 +1,271 @@
+// Fractional Sobolev Gram matrix A = B + B0 (V x V, FP64).
+//
+// Reference: assembleBHighFlat (src/core/sobolev/innerProduct.ts:38-140),
+// assembleBLowFlat (:177-285), assembleAFlat (:325-350). The reference scatters
+// 16 + 16 signed updates per ordered disjoint pair and symmetrises. Summing
+// both orders of each unordered pair {I,J} gives, with g_I = (d_i1 - d_i2)/ell_I
+// and h_I = d_i1 + d_i2 (vertex indicator vectors),
+//
+//   A = sum_K [ W_K g_K g_K^T + Q_K h_K h_K^T ]
+//       - sum_{I != J disjoint} [ X_IJ ell_I ell_J g_I g_J^T + Y_IJ h_I h_J^T ]
+//
+//   w_IJ = 1/4 ell_I ell_J sum_ab (|p_a - q_b| + eps)^-(2 sigma + 1)      (symmetric)
+//   X_IJ = 2 w_IJ (T_I . T_J) / (ell_I ell_J),  W_K = 2 sum_J w_KJ
+//   q_IJ = 1/16 ell_I ell_J sum_ab (|T_I x (p_a - q_b)| + eps)^2 (|.|+eps)^-4 (|.|+eps)^-(2s+1)
+//   Y_IJ = q_IJ + q_JI,                         Q_K = sum_J Y_KJ
+//
+// Ownership tiles: a block owns the A tile (U x V') of 32 x 32 vertices
+// (U <= V'), evaluates X, Y, 2w for every pair of the edges incident to U and
+// to V' into LDS, and gathers each A entry from its <= deg^2 incident edge
+// pairs. Every A entry has exactly one writer (no atomics); both orientations
+// are written from one value, so A is exactly symmetric. Row sums W_K, Q_K
+// come from per-tile partials [tile][K] (each slot has one writer) reduced in
+// fixed order, then k_gram_band adds the edge-own terms on the diagonal and
+// the adjacent entries.
+#include "rc_internal.h"
+#include "rc_math.h"
+
+namespace {
+
+struct GEdge {
+    double P[RC_RMAX][6];
+    double u[RC_RMAX][3];
+    double ell[RC_RMAX];
+    int id[RC_RMAX], i1[RC_RMAX], i2[RC_RMAX], own[RC_RMAX];
+};
+
+__device__ __forceinline__ void tri_decode(int p, int& a, int& b) {
+    int bb = (int)((__builtin_sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((bb + 1) * (bb + 2) / 2 <= p) ++bb;
+    while (bb * (bb + 1) / 2 > p) --bb;
+    b = bb;
+    a = p - bb * (bb + 1) / 2;
+}
+
+template <class P>
+__device__ __forceinline__ void pair_weights(const P& pw, double eps, const double* PI,
+                                             const double* uI, double ellI, const double* PJ,
+                                             const double* uJ, double ellJ, double& w2,
+                                             double& X, double& Y) {
+    double sumD = 0.0, sumQI = 0.0, sumQJ = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const double dx = PI[3 * ia] - PJ[3 * jb];
+            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
+            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
+            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+            const double pd = pw.inv_dexp(deps);
+            const double inv = rcm::rcp(deps);
+            const double i2 = inv * inv;
+            const double inv4pd = i2 * i2 * pd;
+            const double cIx = uI[1] * dz - uI[2] * dy;
+            const double cIy = uI[2] * dx - uI[0] * dz;
+            const double cIz = uI[0] * dy - uI[1] * dx;
+            const double cJx = uJ[1] * dz - uJ[2] * dy;
+            const double cJy = uJ[2] * dx - uJ[0] * dz;
+            const double cJz = uJ[0] * dy - uJ[1] * dx;
+            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
+            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
+            sumD += pd;
+            sumQI += nI * nI * inv4pd;
+            sumQJ += nJ * nJ * inv4pd;
+        }
+    }
+    const double ll = ellI * ellJ;
+    const double w = 0.25 * ll * sumD;
+    const double c = uI[0] * uJ[0] + uI[1] * uJ[1] + uI[2] * uJ[2];
+    w2 = 2.0 * w;
+    X = w2 * c / ll;
+    Y = 0.0625 * ll * (sumQI + sumQJ);
+}
+
+__device__ __forceinline__ void load_tile_edges(GEdge& G, const int32_t* __restrict__ rOff,
+                                                const int32_t* __restrict__ rEdge, int t,
+                                                const EdgeSoA& ed,
+                                                const int32_t* __restrict__ owner, int& n) {
+    const int o = rOff[t];
+    n = rOff[t + 1] - o;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int K = rEdge[o + q];
+        G.P[q][0] = ed.ax[K]; G.P[q][1] = ed.ay[K]; G.P[q][2] = ed.az[K];
+        G.P[q][3] = ed.bx[K]; G.P[q][4] = ed.by[K]; G.P[q][5] = ed.bz[K];
+        G.u[q][0] = ed.ux[K]; G.u[q][1] = ed.uy[K]; G.u[q][2] = ed.uz[K];
+        G.ell[q] = ed.ell[K];
+        G.id[q] = K;
+        G.i1[q] = ed.i1[K];
+        G.i2[q] = ed.i2[K];
+        G.own[q] = owner[K];
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps, int V, int E,
+                                                    int nvt, int ngram,
+                                                    const int32_t* __restrict__ rOff,
+                                                    const int32_t* __restrict__ rEdge,
+                                                    const int32_t* __restrict__ owner,
+                                                    const int32_t* __restrict__ vOff,
+                                                    const int32_t* __restrict__ vSlot,
+                                                    const int32_t* __restrict__ vLocal,
+                                                    double* __restrict__ A,
+                                                    double* __restrict__ rsp) {
+    __shared__ GEdge GU, GV;
+    __shared__ double Xt[RC_RMAX][RC_RMAX + 1];
+    __shared__ double Yt[RC_RMAX][RC_RMAX + 1];
+    __shared__ double Wt[RC_RMAX][RC_RMAX + 1];
+    for (int p = blockIdx.x; p < ngram; p += gridDim.x) {
+        int U, W;
+        tri_decode(p, U, W);  // U <= W
+        const bool diag = (U == W);
+        int nU, nV;
+        load_tile_edges(GU, rOff, rEdge, U, ed, owner, nU);
+        load_tile_edges(GV, rOff, rEdge, W, ed, owner, nV);
+        __syncthreads();
+        // pair phase
+        for (int q = threadIdx.x; q < nU * nV; q += blockDim.x) {
+            const int li = q / nV, lj = q - (q / nV) * nV;
+            const int I = GU.id[li], J = GV.id[lj];
+            const bool dis = GU.i1[li] != GV.i1[lj] && GU.i1[li] != GV.i2[lj] &&
+                             GU.i2[li] != GV.i1[lj] && GU.i2[li] != GV.i2[lj];
+            double w2 = 0.0, X = 0.0, Y = 0.0;
+            if (dis) {
+                // canonical role order (lower edge id first) so equal pairs give equal bits
+                if (I <= J)
+                    pair_weights(pw, eps, GU.P[li], GU.u[li], GU.ell[li], GV.P[lj], GV.u[lj],
+                                 GV.ell[lj], w2, X, Y);
+                else
+                    pair_weights(pw, eps, GV.P[lj], GV.u[lj], GV.ell[lj], GU.P[li], GU.u[li],
+                                 GU.ell[li], w2, X, Y);
+            }
+            Xt[li][lj] = X;
+            Yt[li][lj] = Y;
+            Wt[li][lj] = w2;
+        }
+        __syncthreads();
+        // row-sum partials: slot [t][K] for K owned by one side, summed over
+        // the edges owned by the other side (each ordered pair counted once).
+        if ((int)threadIdx.x < nU) {
+            const int li = threadIdx.x;
+            if (GU.own[li] == U) {
+                double sw = 0.0, sy = 0.0;
+                for (int lj = 0; lj < nV; ++lj)
+                    if (GV.own[lj] == W) {
+                        sw += Wt[li][lj];
+                        sy += Yt[li][lj];
+                    }
+                rsp[((size_t)W * 2) * E + GU.id[li]] = sw;
+                rsp[((size_t)W * 2 + 1) * E + GU.id[li]] = sy;
+            }
+        }
+        if (!diag && (int)threadIdx.x >= 64 && (int)threadIdx.x - 64 < nV) {
+            const int lj = threadIdx.x - 64;
+            if (GV.own[lj] == W) {
+                double sw = 0.0, sy = 0.0;
+                for (int li = 0; li < nU; ++li)
+                    if (GU.own[li] == U) {
+                        sw += Wt[li][lj];
+                        sy += Yt[li][lj];
+                    }
+                rsp[((size_t)U * 2) * E + GV.id[lj]] = sw;
+                rsp[((size_t)U * 2 + 1) * E + GV.id[lj]] = sy;
+            }
+        }
+        // gather phase: val(u, v), u in U row tile, v in W column tile,
+        // outer loop over incidences of the lower vertex index.
+        for (int pass = 0; pass < (diag ? 1 : 2); ++pass) {
+            for (int q = threadIdx.x; q < RC_VTILE * RC_VTILE; q += blockDim.x) {
+                int r, c;
+                if (pass == 0) {  // consecutive threads -> consecutive columns v
+                    r = q / RC_VTILE;
+                    c = q % RC_VTILE;
+                } else {          // consecutive threads -> consecutive rows u (for A[v][u])
+                    c = q / RC_VTILE;
+                    r = q % RC_VTILE;
+                }
+                int u = U * RC_VTILE + r, v = W * RC_VTILE + c;
+                if (u >= V || v >= V) continue;
+                bool swap = diag && v < u;   // canonical (min, max) order
+                const int lo = swap ? v : u, hi = swap ? u : v;
+                double val = 0.0;
+                for (int q1 = vOff[lo]; q1 < vOff[lo + 1]; ++q1) {
+                    const int li = vLocal[q1];
+                    const double si = vSlot[q1] ? -1.0 : 1.0;
+                    for (int q2 = vOff[hi]; q2 < vOff[hi + 1]; ++q2) {
+                        const int lj = vLocal[q2];
+                        const double sj = vSlot[q2] ? -1.0 : 1.0;
+                        // lo always indexes the U-side edge list in off-diagonal tiles
+                        val -= si * sj * Xt[li][lj] + Yt[li][lj];
+                    }
+                }
+                if (pass == 0)
+                    A[(size_t)u * V + v] = val;
+                else
+                    A[(size_t)v * V + u] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_rowsum_reduce(const double* __restrict__ rsp, int nvt, int E,
+                                double* __restrict__ rs) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= E) return;
+    double sw = 0.0, sy = 0.0;
+    for (int t = 0; t < nvt; ++t) {
+        sw += rsp[((size_t)t * 2) * E + K];
+        sy += rsp[((size_t)t * 2 + 1) * E + K];
+    }
+    rs[K] = sw;
+    rs[E + K] = sy;
+}
+
+// Edge-own terms: sum_K W_K g_K g_K^T + Q_K h_K h_K^T touches (u,u) and the
+// adjacent (u,v). One thread per vertex; each off-diagonal entry receives the
+// same value from both of its vertices' threads (exact symmetry).
+__global__ void k_gram_band(EdgeSoA ed, const double* __restrict__ rs, int V, int E,
+                            const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
+                            double* __restrict__ A) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= V) return;
+    double dsum = 0.0;
+    for (int q = vOff[u]; q < vOff[u + 1]; ++q) {
+        const int K = vEdge[q];
+        const double l = ed.ell[K];
+        const double wk = rs[K] / (l * l);
+        const double qk = rs[E + K];
+        dsum += wk + qk;
+        const int v = (ed.i1[K] == u) ? ed.i2[K] : ed.i1[K];
+        A[(size_t)u * V + v] += qk - wk;
+    }
+    A[(size_t)u * V + u] += dsum;
+}
+
+}  // namespace
+
+rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p) {
+    const int grid = ctx->ngram < 8192 ? ctx->ngram : 8192;
+    if (p->alpha == 3.0 && p->beta == 6.0) {
+        hipLaunchKernelGGL(k_gram_tiles<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           Pow36{}, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram, ctx->d_rOff,
+                           ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot, ctx->d_vLocal,
+                           ctx->d_A, ctx->d_rsp);
+    } else {
+        const double s = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+        hipLaunchKernelGGL(k_gram_tiles<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream,
+                           ctx->ed, pw, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram,
+                           ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot,
+                           ctx->d_vLocal, ctx->d_A, ctx->d_rsp);
+    }
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_rsp, ctx->nvt, ctx->E, ctx->d_rs);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_gram_band, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
+                       ctx->d_rs, ctx->V, ctx->E, ctx->d_vOff, ctx->d_vEdge, ctx->d_A);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}

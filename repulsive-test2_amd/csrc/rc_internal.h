@@ -1,0 +1,147 @@
+// Internal declarations of librcurves: the device context, edge/vertex
+// tables and the launchers each translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rcurves.h"
+
+#define RC_TILE 64          // edges per tile of the pair kernels (one wave64 lane per edge)
+#define RC_VTILE 32         // vertices per ownership tile of the Gram assembly
+#define RC_RMAX 40          // max incident edges of one vertex tile (assembly LDS bound)
+#define RC_NB 64            // Cholesky / triangular-solve block size
+
+// Structure-of-arrays per-edge geometry refreshed by k_edge_prep for the
+// current positions. Index = edge id; padded entries (id >= E) are unused.
+struct EdgeSoA {
+    double *ax, *ay, *az;   // position of endpoint i1
+    double *bx, *by, *bz;   // position of endpoint i2
+    double *ell;            // ||e|| + eps
+    double *rlen;           // ||e|| (raw)
+    double *ux, *uy, *uz;   // unit tangent, 0 when ||e|| < 1e-14
+    double *lpow;           // ell^(1-alpha)
+    double *dpc;            // (1-alpha) ell^(-alpha)
+    int32_t *i1, *i2;
+};
+
+struct DevSet;
+struct LSCtl;
+
+struct rc_ctx {
+    int V = 0, E = 0, Epad = 0, ntiles = 0, npairs = 0;
+    int nvt = 0, ngram = 0;   // vertex tiles, assembly tile pairs
+    int nblk = 0;             // Cholesky blocks of RC_NB rows
+    int maxdeg = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // topology (device)
+    int32_t* d_i1 = nullptr;
+    int32_t* d_i2 = nullptr;
+    int32_t* d_vOff = nullptr;    // vertex -> incident (edge, slot) CSR
+    int32_t* d_vEdge = nullptr;
+    int32_t* d_vSlot = nullptr;   // 0 if vertex is i1 of the edge, 1 if i2
+    int32_t* d_vLocal = nullptr;  // local index of the edge in its vertex tile's edge list
+    int32_t* d_rOff = nullptr;    // vertex tile -> incident edge list CSR
+    int32_t* d_rEdge = nullptr;
+    int32_t* d_owner = nullptr;   // owner vertex tile of each edge (tile of i1)
+    int32_t* d_interior = nullptr; // (v, I_v, J_v) for degree-2 vertices, ascending v
+    int ninterior = 0;
+    std::vector<int32_t> h_edges;  // host copy (E x 2)
+
+    // geometry (device)
+    double* d_pos = nullptr;      // SoA x[V], y[V], z[V] current curve
+    double* d_work = nullptr;     // SoA scratch positions (trial / uploads)
+    EdgeSoA ed{};
+    double* d_edge_block = nullptr;
+    int32_t* d_edge_iblock = nullptr;
+
+    // pair-kernel reductions
+    double* d_epart = nullptr;    // energy partial per tile pair
+    double* d_gpart = nullptr;    // gradient partials [ntiles][6][Epad]
+    double* d_grad = nullptr;     // SoA gradient (3V)
+    double* d_scal = nullptr;     // small FP64 scalars (energy results etc.)
+
+    // Gram assembly
+    double* d_rsp = nullptr;      // row-sum partials [nvt][2][E]
+    double* d_rs = nullptr;       // row sums W_K, Q_K (2E)
+    double* d_A = nullptr;        // V x V FP64
+    double* d_F = nullptr;        // V x V FP64 factor (lower) of A + mu 11^T
+    float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
+    double* d_Linv = nullptr;     // inverted diagonal blocks [nblk][NB][NB]
+    float* d_Linv32 = nullptr;
+    double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
+    int32_t* d_flags = nullptr;   // triangular-solve block flags
+    int32_t epoch = 0;
+    int32_t* d_status = nullptr;  // device status words (factor breakdown, spin timeouts)
+
+    // solve workspace
+    int nrhs_cap = 0;
+    double* d_rhs = nullptr;      // [nrhs][V]
+    double* d_sol = nullptr;      // [nrhs][V]
+    double* d_tmp = nullptr;      // [nrhs][V]
+    double* d_tmp2 = nullptr;
+    double* d_tmp3 = nullptr;
+    float* d_tmp32 = nullptr;
+
+    // step state (device)
+    int kcap = 0;
+    double* d_C = nullptr;        // k x 3V constraint Jacobian at gamma0
+    double* d_phi = nullptr;
+    double* d_blockdata = nullptr; // per-block targets
+    double* d_ell0 = nullptr;      // edge-length targets
+    double* d_schur = nullptr;     // (k+3)^2 factored Schur matrix + pivots
+    double* d_ls = nullptr;        // line-search state scalars
+    double* d_dir = nullptr;       // SoA direction (3V)
+    double* d_dE = nullptr;        // SoA total differential (3V)
+    double* d_gt = nullptr;        // SoA g~ (3V)
+    double* d_Y = nullptr;         // A^-1 C^T blocks [3k][V] (per coordinate block)
+    double* d_stage = nullptr;     // 3V AoS staging for uploads/downloads
+    double* d_host_stage = nullptr; // pinned staging
+    size_t host_stage_bytes = 0;
+    bool has_isolated = false;
+
+    // step scratch
+    DevSet* d_set = nullptr;
+    LSCtl* d_ctl = nullptr;
+    double* d_dots = nullptr;
+    double* d_bsum = nullptr;
+    double* d_ssol = nullptr;
+    double* d_Minv = nullptr;
+    double* d_Ax = nullptr;
+    double* d_Cg = nullptr;
+    double* d_stats = nullptr;
+    double* d_epart_ls = nullptr;
+    double* d_refine = nullptr;
+};
+
+// ---- helpers --------------------------------------------------------------
+#define RC_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) {                                             \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(_e);   \
+            return RC_HIP_ERROR;                                            \
+        }                                                                   \
+    } while (0)
+
+#define RC_CHECK_LAUNCH()                                                   \
+    do {                                                                    \
+        hipError_t _e = hipGetLastError();                                  \
+        if (_e != hipSuccess) {                                             \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(_e); \
+            return RC_HIP_ERROR;                                            \
+        }                                                                   \
+    } while (0)
+
+// launchers (defined in the .hip files)
+rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p);
+rc_status rc_launch_energy(rc_ctx* ctx, const rc_params* p, double* d_out);
+rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa);
+rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p);
+rc_status rc_upload_positions_aos(rc_ctx* ctx, const double* xyz, double* d_dst);
+rc_status rc_download_positions_aos(rc_ctx* ctx, const double* d_src, double* xyz);

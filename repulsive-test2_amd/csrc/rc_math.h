@@ -1,0 +1,119 @@
+// Device math for the FP64 pair kernels: square roots, reciprocals and the
+// fractional powers of the tangent-point kernel and the Sobolev metric.
+//
+// sqrt and 1/x use the hardware seeds (v_rsq_f64 / v_rcp_f64, ~2^-26) refined
+// by Newton iterations (two for 1/x, two + one Heron correction for sqrt),
+// giving results within ~1 ulp of the correctly rounded value. x^(-1/3) uses
+// an FP32 seed from v_log_f32 / v_exp_f32 and two Newton steps in FP64
+// (quadratic convergence: ~1e-7 -> ~1e-14 -> rounding level).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rcm {
+
+__device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    return y;
+}
+
+// sqrt(x) for x >= 0 (returns 0 for x == 0). Heron-corrected.
+__device__ __forceinline__ double sqrt_(double x) {
+    if (!(x > 0.0)) return (x == 0.0) ? 0.0 : __builtin_sqrt(x);
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y;
+    double g = x * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+// sqrt and 1/sqrt together (x > 0 required for inv; inv = 0 when x == 0).
+__device__ __forceinline__ void sqrt_rsqrt(double x, double& s, double& inv) {
+    if (!(x > 0.0)) {
+        s = (x == 0.0) ? 0.0 : __builtin_sqrt(x);
+        inv = 0.0;
+        return;
+    }
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y;
+    double g = x * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    s = fma(d, h, g);
+    inv = 2.0 * h;
+    // one more Newton step on the reciprocal for full accuracy
+    double e = fma(-s, inv, 1.0);
+    inv = fma(inv, e, inv);
+}
+
+// x^(-1/3), x > 0.
+__device__ __forceinline__ double rcbrt(double x) {
+    float xf = (float)x;
+    float yf = __builtin_amdgcn_exp2f(-__builtin_amdgcn_logf(xf) * (1.0f / 3.0f));
+    double y = (double)yf;
+    const double third = 1.0 / 3.0;
+    double y3 = y * y * y;
+    y = y * fma(-x, y3, 4.0) * third;
+    y3 = y * y * y;
+    y = y * fma(-x, y3, 4.0) * third;
+    return y;
+}
+
+// Powers by non-negative integer n, by the same binary exponentiation a
+// JavaScript engine uses for integer Math.pow exponents.
+__device__ __forceinline__ double ipow(double x, int n) {
+    double r = 1.0;
+    while (n) {
+        if (n & 1) r *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return r;
+}
+
+}  // namespace rcm
+
+// Exponent policies of the tangent-point kernel (alpha, beta) and the metric
+// distance exponent 2*sigma + 1 = 2((beta-1)/alpha - 1) + 1.
+struct PowGeneric {
+    double alpha, beta, dexp;
+    __device__ __forceinline__ double c_alpha(double c) const { return pow(c, alpha); }
+    __device__ __forceinline__ double c_alpha_m1(double c) const { return pow(c, alpha - 1.0); }
+    // d^-beta from inv = 1/d
+    __device__ __forceinline__ double inv_beta(double d, double inv) const { (void)inv; return 1.0 / pow(d, beta); }
+    __device__ __forceinline__ double inv_dexp(double d) const { return 1.0 / pow(d, dexp); }
+};
+
+// alpha = 3, beta = 6 (the default and every benchmark configuration):
+// dexp = 7/3, so d^-dexp = (d^-1/3)^7.
+struct Pow36 {
+    __device__ __forceinline__ double c_alpha(double c) const { return c * c * c; }
+    __device__ __forceinline__ double c_alpha_m1(double c) const { return c * c; }
+    __device__ __forceinline__ double inv_beta(double d, double inv) const {
+        (void)d;
+        double i2 = inv * inv;
+        double i4 = i2 * i2;
+        return i4 * i2;
+    }
+    __device__ __forceinline__ double inv_dexp(double d) const {
+        double y = rcm::rcbrt(d);
+        double y2 = y * y;
+        double y4 = y2 * y2;
+        return y4 * y2 * y;
+    }
+};

@@ -1,0 +1,312 @@
+// Mixed-precision solve of A_hat X = B: preconditioned conjugate gradients in
+// FP64 (matrix products with the FP64 A, FP64 vectors and scalars) with the
+// FP32 Cholesky factor of S = D^-1/2 A_hat D^-1/2 (FP16x3 MFMA trailing
+// updates) as preconditioner: M^-1 r = D^-1/2 S~^-1 D^-1/2 r.
+//
+// Plain iterative refinement x += M^-1 (b - A x) converges only while
+// kappa(A_hat) * u_factor < 1; for this fractional operator kappa grows like
+// V^2.4 and crosses that bound around V ~ 4096 with an FP32-quality factor,
+// so the refinement is run as PCG, which converges for any SPD
+// preconditioner and needs only a few iterations when M^-1 A_hat is close to
+// the identity. Every right-hand side column has its own CG scalars; a
+// column stops once ||r||/||b|| <= tol, and all kernels of an iteration
+// return at once when every column has stopped.
+#include "rc_internal.h"
+
+rc_status rc_solve_f32_factor(rc_ctx* ctx, double* d_B, int nrhs, const int32_t* skip);
+
+namespace {
+
+constexpr int GEMM_CHUNK = 16;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// out[c][v] = in[c][v] * d[v] (v < V), 0 on padding
+__global__ void k_scale_cols(const double* __restrict__ in, const double* __restrict__ d, int V,
+                             int Vp, int nrhs, double* __restrict__ out,
+                             const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (v >= Vp || c >= nrhs) return;
+    const size_t j = (size_t)c * Vp + v;
+    out[j] = (v < V) ? in[j] * d[v] : 0.0;
+}
+
+// out[c] = sum_v X[c][v] (v < V)
+__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ X, int V, int Vp,
+                                                double* __restrict__ out,
+                                                const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int v = threadIdx.x; v < V; v += 256) s += X[(size_t)c * Vp + v];
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Q[c][u] = sum_j A[u][j] P[c][j] + mu * colsum[c], one row u per block,
+// up to GEMM_CHUNK columns per launch. A is read once per launch.
+__global__ __launch_bounds__(256) void k_ahat_gemm(const double* __restrict__ A, int V, int Vp,
+                                                   const double* __restrict__ mu,
+                                                   const double* __restrict__ colsum,
+                                                   const double* __restrict__ P, int nc,
+                                                   double* __restrict__ Q,
+                                                   const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[4][GEMM_CHUNK];
+    const int u = blockIdx.x;
+    const double* row = A + (size_t)u * V;
+    double acc[GEMM_CHUNK];
+#pragma unroll
+    for (int c = 0; c < GEMM_CHUNK; ++c) acc[c] = 0.0;
+    for (int j = threadIdx.x; j < V; j += 256) {
+        const double a = row[j];
+#pragma unroll
+        for (int c = 0; c < GEMM_CHUNK; ++c)
+            if (c < nc) acc[c] = fma(a, P[(size_t)c * Vp + j], acc[c]);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < GEMM_CHUNK; ++c) {
+        const double s = wsum(acc[c]);
+        if (lane == 0) sh[w][c] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+        const int c = threadIdx.x;
+        Q[(size_t)c * Vp + u] = ((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c] + mu[0] * colsum[c];
+    }
+}
+
+// out[c] = <X[c], Y[c]> over v < V
+__global__ __launch_bounds__(256) void k_coldot(const double* __restrict__ X,
+                                                const double* __restrict__ Y, int V, int Vp,
+                                                double* __restrict__ out,
+                                                const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int v = threadIdx.x; v < V; v += 256) s = fma(X[(size_t)c * Vp + v], Y[(size_t)c * Vp + v], s);
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// R = B - Q
+__global__ void k_sub(const double* __restrict__ B, const double* __restrict__ Q, int Vp, int nrhs,
+                      double* __restrict__ R) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)nrhs * Vp) return;
+    R[j] = B[j] - Q[j];
+}
+
+// X += alpha P, R -= alpha Q with alpha = rz / pq per active column
+__global__ void k_pcg_update(double* __restrict__ X, double* __restrict__ R,
+                             const double* __restrict__ P, const double* __restrict__ Q,
+                             const double* __restrict__ rz, const double* __restrict__ pq,
+                             const int32_t* __restrict__ conv, int V, int Vp,
+                             const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (v >= V || conv[c]) return;
+    const double al = rz[c] / pq[c];
+    const size_t j = (size_t)c * Vp + v;
+    X[j] = fma(al, P[j], X[j]);
+    R[j] = fma(-al, Q[j], R[j]);
+}
+
+// P = Z + (rz_new / rz_old) P per active column
+__global__ void k_pcg_dir(const double* __restrict__ Z, double* __restrict__ P,
+                          const double* __restrict__ rz_new, const double* __restrict__ rz_old,
+                          const int32_t* __restrict__ conv, int V, int Vp, int first,
+                          const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (v >= V || conv[c]) return;
+    const size_t j = (size_t)c * Vp + v;
+    P[j] = first ? Z[j] : fma(rz_new[c] / rz_old[c], P[j], Z[j]);
+}
+
+// Convergence bookkeeping: ctl[0] = all converged flag (int), ctl[1] = iteration count,
+// res[0] = max relative residual, res[1] = iterations (as double, for the step stats).
+__global__ void k_pcg_check(const double* __restrict__ rr, const double* __restrict__ bn,
+                            int nrhs, double tol, int32_t* __restrict__ conv,
+                            int32_t* __restrict__ ctl, double* __restrict__ res, int count_iter) {
+    if (threadIdx.x != 0) return;
+    if (ctl[0]) return;
+    int all = 1;
+    double mx = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+        const double r = (bn[c] > 0.0) ? sqrt(rr[c]) / bn[c] : 0.0;
+        if (!(r > tol) || r != r) conv[c] = 1;  // NaN stops the column too
+        if (!conv[c]) all = 0;
+        mx = (r > mx) ? r : mx;
+    }
+    if (count_iter) ctl[1] += 1;
+    ctl[0] = all;
+    res[0] = mx;
+    res[1] = (double)ctl[1];
+}
+
+__global__ void k_sqrt_inplace(double* __restrict__ x, int n) {
+    const int i = threadIdx.x;
+    if (i < n) x[i] = sqrt(x[i]);
+}
+
+}  // namespace
+
+struct PcgBufs {
+    double *R, *Z, *P, *Q;
+    double *bn, *rr, *rzA, *rzB, *pq, *cs;
+    int32_t *conv, *ctl;
+    int cap;
+};
+
+static rc_status pcg_bufs(rc_ctx* ctx, PcgBufs& b) {
+    const int Vp = ctx->nblk * RC_NB;
+    const int cap = ctx->nrhs_cap;
+    if (!ctx->d_tmp2) {
+        RC_HIP(hipMalloc((void**)&ctx->d_tmp2, sizeof(double) * (size_t)4 * cap * Vp));
+        RC_HIP(hipMalloc((void**)&ctx->d_tmp3, sizeof(double) * (size_t)8 * cap + 64));
+    }
+    b.R = ctx->d_tmp2;
+    b.Z = b.R + (size_t)cap * Vp;
+    b.P = b.Z + (size_t)cap * Vp;
+    b.Q = b.P + (size_t)cap * Vp;
+    b.bn = ctx->d_tmp3;
+    b.rr = b.bn + cap;
+    b.rzA = b.rr + cap;
+    b.rzB = b.rzA + cap;
+    b.pq = b.rzB + cap;
+    b.cs = b.pq + cap;
+    b.conv = (int32_t*)(b.cs + cap);
+    b.ctl = b.conv + cap + 2;
+    b.cap = cap;
+    return RC_OK;
+}
+
+// M^-1 applied in place to nrhs columns of W (scale, FP32-factor solve, scale).
+static rc_status precond(rc_ctx* ctx, double* W, int nrhs, const int32_t* skip) {
+    const int Vp = ctx->nblk * RC_NB;
+    dim3 g((Vp + 255) / 256, nrhs);
+    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, W, ctx->d_dscale, ctx->V, Vp,
+                       nrhs, W, skip);
+    RC_CHECK_LAUNCH();
+    rc_status s = rc_solve_f32_factor(ctx, W, nrhs, skip);
+    if (s != RC_OK) return s;
+    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, W, ctx->d_dscale, ctx->V, Vp,
+                       nrhs, W, skip);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+static rc_status ahat_mult(rc_ctx* ctx, const double* X, double* Q, double* cs, int nrhs,
+                           const int32_t* skip) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_colsum, dim3(nrhs), dim3(256), 0, ctx->stream, X, ctx->V, Vp, cs, skip);
+    RC_CHECK_LAUNCH();
+    for (int c0 = 0; c0 < nrhs; c0 += GEMM_CHUNK) {
+        const int nc = (nrhs - c0) < GEMM_CHUNK ? (nrhs - c0) : GEMM_CHUNK;
+        hipLaunchKernelGGL(k_ahat_gemm, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_A, ctx->V,
+                           Vp, ctx->d_scal + 1, cs + c0, X + (size_t)c0 * Vp, nc,
+                           Q + (size_t)c0 * Vp, skip);
+        RC_CHECK_LAUNCH();
+    }
+    return RC_OK;
+}
+
+// Solve A_hat X = B (nrhs columns, ld Vp). d_refine_out[0] = final max
+// relative residual, [1] = PCG iterations.
+rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, double tol,
+                         int max_iter, double* d_refine_out) {
+    const int Vp = ctx->nblk * RC_NB;
+    PcgBufs b;
+    rc_status s = pcg_bufs(ctx, b);
+    if (s != RC_OK) return s;
+    if (nrhs > b.cap) {
+        ctx->err = "too many right-hand sides";
+        return RC_BAD_ARG;
+    }
+    const size_t bytes = sizeof(double) * (size_t)nrhs * Vp;
+    const dim3 gv((ctx->V + 255) / 256, nrhs);
+    RC_HIP(hipMemsetAsync(b.conv, 0, sizeof(int32_t) * (b.cap + 4), ctx->stream));
+    RC_HIP(hipMemsetAsync(d_refine_out, 0, sizeof(double) * 2, ctx->stream));
+    // X0 = M^-1 B
+    RC_HIP(hipMemcpyAsync(d_X, d_B, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((s = precond(ctx, d_X, nrhs, nullptr)) != RC_OK) return s;
+    // R = B - A_hat X0, ||B||
+    if ((s = ahat_mult(ctx, d_X, b.Q, b.cs, nrhs, nullptr)) != RC_OK) return s;
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)(((size_t)nrhs * Vp + 255) / 256)), dim3(256), 0,
+                       ctx->stream, d_B, b.Q, Vp, nrhs, b.R);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, d_B, d_B, ctx->V, Vp, b.bn,
+                       (const int32_t*)nullptr);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(256), 0, ctx->stream, b.bn, nrhs);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.R, ctx->V, Vp, b.rr,
+                       (const int32_t*)nullptr);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(64), 0, ctx->stream, b.rr, b.bn, nrhs, tol,
+                       b.conv, b.ctl, d_refine_out, 0);
+    RC_CHECK_LAUNCH();
+    const int32_t* skip = b.ctl;
+    // Z = M^-1 R, rz = <R, Z>, P = Z
+    RC_HIP(hipMemcpyAsync(b.Z, b.R, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((s = precond(ctx, b.Z, nrhs, skip)) != RC_OK) return s;
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.Z, ctx->V, Vp, b.rzA,
+                       skip);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pcg_dir, gv, dim3(256), 0, ctx->stream, b.Z, b.P, b.rzA, b.rzA, b.conv,
+                       ctx->V, Vp, 1, skip);
+    RC_CHECK_LAUNCH();
+    double* rz_old = b.rzA;
+    double* rz_new = b.rzB;
+    const int batch = 6;
+    for (int it = 0; it < max_iter; ++it) {
+        if ((s = ahat_mult(ctx, b.P, b.Q, b.cs, nrhs, skip)) != RC_OK) return s;
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.P, b.Q, ctx->V, Vp,
+                           b.pq, skip);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_pcg_update, gv, dim3(256), 0, ctx->stream, d_X, b.R, b.P, b.Q, rz_old,
+                           b.pq, b.conv, ctx->V, Vp, skip);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.R, ctx->V, Vp,
+                           b.rr, skip);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(64), 0, ctx->stream, b.rr, b.bn, nrhs, tol,
+                           b.conv, b.ctl, d_refine_out, 1);
+        RC_CHECK_LAUNCH();
+        if (it + 1 == max_iter) break;
+        RC_HIP(hipMemcpyAsync(b.Z, b.R, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        if ((s = precond(ctx, b.Z, nrhs, skip)) != RC_OK) return s;
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.Z, ctx->V, Vp,
+                           rz_new, skip);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_pcg_dir, gv, dim3(256), 0, ctx->stream, b.Z, b.P, rz_new, rz_old,
+                           b.conv, ctx->V, Vp, 0, skip);
+        RC_CHECK_LAUNCH();
+        double* tswap = rz_old;
+        rz_old = rz_new;
+        rz_new = tswap;
+        if ((it + 1) % batch == 0) {
+            RC_HIP(hipMemcpyAsync(ctx->d_host_stage + 32, b.ctl, sizeof(int32_t) * 2,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            RC_HIP(hipStreamSynchronize(ctx->stream));
+            if (((int32_t*)(ctx->d_host_stage + 32))[0]) break;
+        }
+    }
+    return RC_OK;
+}

@@ -1,0 +1,477 @@
+// Tangent-point energy and its adjoint gradient over all disjoint edge pairs.
+//
+// Reference: calculateEnergy (src/core/tangentPointEnergy.ts:50-111) and
+// gradientAnalytical with kernelDerivs (:139-392). The reference visits every
+// ORDERED disjoint pair (I,J) and halves the sum; here every UNORDERED pair
+// {I,J} is visited once and evaluates both ordered terms E_IJ + E_JI, which
+// share the four endpoint distances. E = 1/2 * sum_{I<J} (E_IJ + E_JI).
+//
+// Work decomposition: edges are cut into tiles of 64; a wave64 owns one
+// unordered tile pair (a <= b), lane t owns edge I = 64a + t. The energy
+// kernel walks the 64 edges of tile b with wave-uniform J. The gradient
+// kernel walks them in a lane rotation (lane t visits J = 64b + (t+s)%64 at
+// step s) so the J-side accumulators can travel with their edge through a
+// lane shuffle: every accumulator has one owner at every step, so there are
+// no atomics and the sum order is fixed (bit-reproducible run to run).
+#include "rc_internal.h"
+#include "rc_math.h"
+
+namespace {
+
+__device__ __forceinline__ void tri_decode(int p, int& a, int& b) {
+    // p = b(b+1)/2 + a, 0 <= a <= b
+    int bb = (int)((__builtin_sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((bb + 1) * (bb + 2) / 2 <= p) ++bb;
+    while (bb * (bb + 1) / 2 > p) --bb;
+    b = bb;
+    a = p - bb * (bb + 1) / 2;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void k_edge_prep(const double* __restrict__ pos, int V, const int32_t* __restrict__ i1,
+                            const int32_t* __restrict__ i2, int E, int Epad, double alpha,
+                            double eps, EdgeSoA ed) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= Epad) return;
+    if (K < E) {
+        const int a = i1[K], b = i2[K];
+        const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
+        const double bx = pos[b], by = pos[V + b], bz = pos[2 * V + b];
+        const double ex = bx - ax, ey = by - ay, ez = bz - az;
+        const double r = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+        const double ell = r + eps;
+        double ux = 0, uy = 0, uz = 0;
+        if (!(r < 1e-14)) {
+            const double inv = 1.0 / r;
+            ux = inv * ex;
+            uy = inv * ey;
+            uz = inv * ez;
+        }
+        ed.ax[K] = ax; ed.ay[K] = ay; ed.az[K] = az;
+        ed.bx[K] = bx; ed.by[K] = by; ed.bz[K] = bz;
+        ed.ell[K] = ell;
+        ed.rlen[K] = r;
+        ed.ux[K] = ux; ed.uy[K] = uy; ed.uz[K] = uz;
+        ed.lpow[K] = pow(ell, 1.0 - alpha);
+        ed.dpc[K] = (1.0 - alpha) * pow(ell, -alpha);
+        ed.i1[K] = a;
+        ed.i2[K] = b;
+    } else {
+        ed.ax[K] = ed.ay[K] = ed.az[K] = 0;
+        ed.bx[K] = ed.by[K] = ed.bz[K] = 0;
+        ed.ell[K] = 1;
+        ed.rlen[K] = 0;
+        ed.ux[K] = ed.uy[K] = ed.uz[K] = 0;
+        ed.lpow[K] = 0;
+        ed.dpc[K] = 0;
+        ed.i1[K] = -1;
+        ed.i2[K] = -2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// energy
+
+template <class P>
+__device__ __forceinline__ double pair_energy(const P& pw, double eps, const double* PI,
+                                              const double* PJ, double eIx, double eIy,
+                                              double eIz, double eJx, double eJy, double eJz,
+                                              double ellI, double ellJ, double lpI, double lpJ) {
+    double sFI = 0.0, sFJ = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const double dx = PI[3 * ia] - PJ[3 * jb];
+            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
+            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
+            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+            const double invB = pw.inv_beta(deps, rcm::rcp(deps));
+            const double cIx = eIy * dz - eIz * dy;
+            const double cIy = eIz * dx - eIx * dz;
+            const double cIz = eIx * dy - eIy * dx;
+            const double cJx = eJy * dz - eJz * dy;
+            const double cJy = eJz * dx - eJx * dz;
+            const double cJz = eJx * dy - eJy * dx;
+            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
+            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
+            sFI += pw.c_alpha(nI) * invB;
+            sFJ += pw.c_alpha(nJ) * invB;
+        }
+    }
+    return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_energy(EdgeSoA ed, int E, int npairs, P pw, double eps,
+                                                double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int base = blockIdx.x * 4; base < npairs; base += gridDim.x * 4) {
+        const int p = __builtin_amdgcn_readfirstlane(base + wave);
+        if (p >= npairs) break;
+        int a, b;
+        tri_decode(p, a, b);
+        const int I = a * RC_TILE + lane;
+        const bool Iok = I < E;
+        const int Ic = Iok ? I : 0;
+        double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
+        const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
+        const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
+        const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
+        const bool diag = (a == b);
+        double sum = 0.0;
+        for (int s = 0; s < RC_TILE; ++s) {
+            const int J = b * RC_TILE + s;
+            if (J >= E) break;
+            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
+            const int j1 = ed.i1[J], j2 = ed.i2[J];
+            const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
+                               i2 != j2;
+            const double eJx = PJ[3] - PJ[0], eJy = PJ[4] - PJ[1], eJz = PJ[5] - PJ[2];
+            const double e = pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, eJx, eJy, eJz, ellI,
+                                         ed.ell[J], lpI, ed.lpow[J]);
+            sum += valid ? e : 0.0;
+        }
+        sum = wave_sum(sum);
+        if (lane == 0) part[p] = sum;
+    }
+}
+
+// Fixed-order reduction of per-tile-pair partials: thread t sums
+// part[t], part[t+1024], ... then a shared-memory tree. out = scale * sum.
+__global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ part, int n,
+                                                     double scale, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = scale * sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// gradient
+
+struct KD {
+    double f, ddx, ddy, ddz, dex, dey, dez;
+};
+
+// kernelDerivs (tangentPointEnergy.ts:165-244) for f(e,d) = (|e x d|+eps)^a/(|d|+eps)^b
+// given the shared distance parts dh (guarded unit d), invB = deps^-b,
+// invB1 = deps^-(b+1).
+template <class P>
+__device__ __forceinline__ KD kd(const P& pw, double eps, double alpha, double beta, double ex,
+                                 double ey, double ez, double dx, double dy, double dz, double dhx,
+                                 double dhy, double dhz, double invB, double invB1) {
+    KD r;
+    const double cx = ey * dz - ez * dy;
+    const double cy = ez * dx - ex * dz;
+    const double cz = ex * dy - ey * dx;
+    double rc, invRc;
+    rcm::sqrt_rsqrt(cx * cx + cy * cy + cz * cz, rc, invRc);
+    const double ceps = rc + eps;
+    const double cA = pw.c_alpha(ceps);
+    const double cA1 = pw.c_alpha_m1(ceps);
+    r.f = cA * invB;
+    const double coeff_c = alpha * cA1 * invB;
+    const double coeff_d = -beta * cA * invB1;
+    const double k = (rc >= 1e-14) ? coeff_c * invRc : 0.0;
+    // cross3D(c, e) and cross3D(d, c)
+    const double ccex = cy * ez - cz * ey;
+    const double ccey = cz * ex - cx * ez;
+    const double ccez = cx * ey - cy * ex;
+    const double dccx = dy * cz - dz * cy;
+    const double dccy = dz * cx - dx * cz;
+    const double dccz = dx * cy - dy * cx;
+    r.ddx = k * ccex + coeff_d * dhx;
+    r.ddy = k * ccey + coeff_d * dhy;
+    r.ddz = k * ccez + coeff_d * dhz;
+    r.dex = k * dccx;
+    r.dey = k * dccy;
+    r.dez = k * dccz;
+    return r;
+}
+
+struct EdgeRegs {
+    double P[6];   // endpoint positions a, b
+    double e[3];   // b - a
+    double u[3];   // unit tangent (guarded)
+    double ell, lp, dpc;
+    int i1, i2;
+};
+
+// Contributions of one unordered disjoint pair {I,J} (both ordered terms)
+// to the four endpoint slots: cI[0..2] -> I.i1, cI[3..5] -> I.i2, same for J.
+template <class P>
+__device__ __forceinline__ void pair_grad(const P& pw, double eps, double alpha, double beta,
+                                          const EdgeRegs& I, const EdgeRegs& J, double* cI,
+                                          double* cJ) {
+    double sFI = 0.0, sFJ = 0.0;
+    double DI[2][3] = {{0, 0, 0}, {0, 0, 0}}, DJfI[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double DJ[2][3] = {{0, 0, 0}, {0, 0, 0}}, DIfJ[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double SdeI[3] = {0, 0, 0}, SdeJ[3] = {0, 0, 0};
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const double dx = I.P[3 * ia] - J.P[3 * jb];
+            const double dy = I.P[3 * ia + 1] - J.P[3 * jb + 1];
+            const double dz = I.P[3 * ia + 2] - J.P[3 * jb + 2];
+            double rd, invRd;
+            rcm::sqrt_rsqrt(dx * dx + dy * dy + dz * dz, rd, invRd);
+            const double hs = (rd < 1e-14) ? 0.0 : invRd;
+            const double dhx = hs * dx, dhy = hs * dy, dhz = hs * dz;
+            const double deps = rd + eps;
+            const double inv = rcm::rcp(deps);
+            const double invB = pw.inv_beta(deps, inv);
+            const double invB1 = invB * inv;
+            const KD a = kd(pw, eps, alpha, beta, I.e[0], I.e[1], I.e[2], dx, dy, dz, dhx, dhy,
+                            dhz, invB, invB1);
+            const KD c = kd(pw, eps, alpha, beta, J.e[0], J.e[1], J.e[2], -dx, -dy, -dz, -dhx,
+                            -dhy, -dhz, invB, invB1);
+            sFI += a.f;
+            sFJ += c.f;
+            DI[ia][0] += a.ddx; DI[ia][1] += a.ddy; DI[ia][2] += a.ddz;
+            DJfI[jb][0] += a.ddx; DJfI[jb][1] += a.ddy; DJfI[jb][2] += a.ddz;
+            SdeI[0] += a.dex; SdeI[1] += a.dey; SdeI[2] += a.dez;
+            DJ[jb][0] += c.ddx; DJ[jb][1] += c.ddy; DJ[jb][2] += c.ddz;
+            DIfJ[ia][0] += c.ddx; DIfJ[ia][1] += c.ddy; DIfJ[ia][2] += c.ddz;
+            SdeJ[0] += c.dex; SdeJ[1] += c.dey; SdeJ[2] += c.dez;
+        }
+    }
+    const double bIJ = 0.25 * I.lp * J.ell;
+    const double bJI = 0.25 * J.lp * I.ell;
+    const double gIs = 0.25 * J.ell * sFI * I.dpc + 0.25 * J.lp * sFJ;
+    const double gJs = 0.25 * I.lp * sFI + 0.25 * I.ell * sFJ * J.dpc;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double gI = gIs * I.u[c];
+        const double gJ = gJs * J.u[c];
+        cI[c] = bIJ * DI[0][c] - bJI * DIfJ[0][c] - bIJ * SdeI[c] - gI;
+        cI[3 + c] = bIJ * DI[1][c] - bJI * DIfJ[1][c] + bIJ * SdeI[c] + gI;
+        cJ[c] = bJI * DJ[0][c] - bIJ * DJfI[0][c] - bJI * SdeJ[c] - gJ;
+        cJ[3 + c] = bJI * DJ[1][c] - bIJ * DJfI[1][c] + bJI * SdeJ[c] + gJ;
+    }
+}
+
+__device__ __forceinline__ void load_edge(const EdgeSoA& ed, int K, EdgeRegs& r) {
+    r.P[0] = ed.ax[K]; r.P[1] = ed.ay[K]; r.P[2] = ed.az[K];
+    r.P[3] = ed.bx[K]; r.P[4] = ed.by[K]; r.P[5] = ed.bz[K];
+    r.e[0] = r.P[3] - r.P[0]; r.e[1] = r.P[4] - r.P[1]; r.e[2] = r.P[5] - r.P[2];
+    r.u[0] = ed.ux[K]; r.u[1] = ed.uy[K]; r.u[2] = ed.uz[K];
+    r.ell = ed.ell[K]; r.lp = ed.lpow[K]; r.dpc = ed.dpc[K];
+    r.i1 = ed.i1[K]; r.i2 = ed.i2[K];
+}
+
+// LDS copy of one J tile (SoA, 64 entries per field).
+struct JTile {
+    double P[6][RC_TILE];
+    double u[3][RC_TILE];
+    double ell[RC_TILE], lp[RC_TILE], dpc[RC_TILE];
+    int i1[RC_TILE], i2[RC_TILE];
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void k_gradient(EdgeSoA ed, int E, int Epad, int npairs, P pw,
+                                                  double eps, double alpha, double beta,
+                                                  double* __restrict__ gpart) {
+    __shared__ JTile jt[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    JTile& T = jt[wave];
+    for (int base = blockIdx.x * 4; base < npairs; base += gridDim.x * 4) {
+        const int p = __builtin_amdgcn_readfirstlane(base + wave);
+        if (p >= npairs) break;
+        int a, b;
+        tri_decode(p, a, b);
+        const bool diag = (a == b);
+        const int I = a * RC_TILE + lane;
+        EdgeRegs eI;
+        load_edge(ed, I, eI);  // padded slots hold benign values
+        {
+            const int J = b * RC_TILE + lane;
+            T.P[0][lane] = ed.ax[J]; T.P[1][lane] = ed.ay[J]; T.P[2][lane] = ed.az[J];
+            T.P[3][lane] = ed.bx[J]; T.P[4][lane] = ed.by[J]; T.P[5][lane] = ed.bz[J];
+            T.u[0][lane] = ed.ux[J]; T.u[1][lane] = ed.uy[J]; T.u[2][lane] = ed.uz[J];
+            T.ell[lane] = ed.ell[J]; T.lp[lane] = ed.lpow[J]; T.dpc[lane] = ed.dpc[J];
+            T.i1[lane] = ed.i1[J]; T.i2[lane] = ed.i2[J];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double accI[6] = {0, 0, 0, 0, 0, 0};
+        double accJ[6] = {0, 0, 0, 0, 0, 0};  // held for J-lane (lane + s) % 64
+        const int s0 = diag ? 1 : 0;
+        const int s1 = diag ? 32 : 63;
+        const bool Iok = I < E;
+        for (int s = s0; s <= s1; ++s) {
+            const int jl = (lane + s) & (RC_TILE - 1);
+            const int J = b * RC_TILE + jl;
+            EdgeRegs eJ;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) eJ.P[c] = T.P[c][jl];
+            eJ.e[0] = eJ.P[3] - eJ.P[0]; eJ.e[1] = eJ.P[4] - eJ.P[1]; eJ.e[2] = eJ.P[5] - eJ.P[2];
+            eJ.u[0] = T.u[0][jl]; eJ.u[1] = T.u[1][jl]; eJ.u[2] = T.u[2][jl];
+            eJ.ell = T.ell[jl]; eJ.lp = T.lp[jl]; eJ.dpc = T.dpc[jl];
+            eJ.i1 = T.i1[jl]; eJ.i2 = T.i2[jl];
+            const bool valid = Iok && J < E && (!diag || s < 32 || lane < 32) &&
+                               eI.i1 != eJ.i1 && eI.i1 != eJ.i2 && eI.i2 != eJ.i1 &&
+                               eI.i2 != eJ.i2;
+            double cI[6], cJ[6];
+            pair_grad(pw, eps, alpha, beta, eI, eJ, cI, cJ);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                // addToGrad's `x || 0` (tangentPointEnergy.ts:152-156): NaN adds 0
+                accI[c] += (valid && cI[c] == cI[c]) ? cI[c] : 0.0;
+                accJ[c] += (valid && cJ[c] == cJ[c]) ? cJ[c] : 0.0;
+            }
+            // pass every J accumulator one lane down so lane t holds J-lane (t+s+1)%64
+#pragma unroll
+            for (int c = 0; c < 6; ++c) accJ[c] = __shfl(accJ[c], (lane + 1) & 63, 64);
+        }
+        // lane t now holds J-lane (t + nsteps) % 64; realign to J-lane t
+        const int nsteps = s1 - s0 + 1;
+        const int src = (lane - nsteps) & 63;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) accJ[c] = __shfl(accJ[c], src, 64);
+        // slot writes: gpart[(t * 6 + c) * Epad + K]
+        if (diag) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                gpart[((size_t)a * 6 + c) * Epad + I] = accI[c] + accJ[c];
+        } else {
+            const int J = b * RC_TILE + lane;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                gpart[((size_t)b * 6 + c) * Epad + I] = accI[c];
+                gpart[((size_t)a * 6 + c) * Epad + J] = accJ[c];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Per-vertex gather of the per-edge-endpoint partials in fixed order, times
+// the reference's final 1/2 (tangentPointEnergy.ts:385-389). Output SoA.
+__global__ void k_grad_gather(const double* __restrict__ gpart, int ntiles, int Epad, int V,
+                              const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
+                              const int32_t* __restrict__ vSlot, double* __restrict__ g) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double gx = 0, gy = 0, gz = 0;
+    for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+        const int K = vEdge[q];
+        const int sl = vSlot[q] * 3;
+        for (int t = 0; t < ntiles; ++t) {
+            const size_t base = ((size_t)t * 6 + sl) * Epad + K;
+            gx += gpart[base];
+            gy += gpart[base + Epad];
+            gz += gpart[base + 2 * (size_t)Epad];
+        }
+    }
+    g[v] = 0.5 * gx;
+    g[V + v] = 0.5 * gy;
+    g[2 * V + v] = 0.5 * gz;
+}
+
+__global__ void k_aos_to_soa(const double* __restrict__ aos, int V, double* __restrict__ soa) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    soa[v] = aos[3 * v];
+    soa[V + v] = aos[3 * v + 1];
+    soa[2 * V + v] = aos[3 * v + 2];
+}
+
+__global__ void k_soa_to_aos(const double* __restrict__ soa, int V, double* __restrict__ aos) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    aos[3 * v] = soa[v];
+    aos[3 * v + 1] = soa[V + v];
+    aos[3 * v + 2] = soa[2 * V + v];
+}
+
+bool is36(const rc_params* p) { return p->alpha == 3.0 && p->beta == 6.0; }
+
+int pair_grid(int npairs) {
+    int blocks = (npairs + 3) / 4;
+    return blocks < 4096 ? blocks : 4096;
+}
+
+}  // namespace
+
+rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p) {
+    hipLaunchKernelGGL(k_edge_prep, dim3((ctx->Epad + 255) / 256), dim3(256), 0, ctx->stream, pos,
+                       ctx->V, ctx->d_i1, ctx->d_i2, ctx->E, ctx->Epad, p->alpha, p->epsilon,
+                       ctx->ed);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_energy(rc_ctx* ctx, const rc_params* p, double* d_out) {
+    const int grid = pair_grid(ctx->npairs);
+    if (is36(p)) {
+        hipLaunchKernelGGL(k_energy<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed, ctx->E,
+                           ctx->npairs, Pow36{}, p->epsilon, ctx->d_epart);
+    } else {
+        const double s = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+        hipLaunchKernelGGL(k_energy<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->npairs, pw, p->epsilon, ctx->d_epart);
+    }
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_epart, ctx->npairs,
+                       0.5, d_out);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa) {
+    const int grid = pair_grid(ctx->npairs);
+    if (is36(p)) {
+        hipLaunchKernelGGL(k_gradient<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->Epad, ctx->npairs, Pow36{}, p->epsilon, p->alpha, p->beta,
+                           ctx->d_gpart);
+    } else {
+        const double s = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+        hipLaunchKernelGGL(k_gradient<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->Epad, ctx->npairs, pw, p->epsilon, p->alpha, p->beta,
+                           ctx->d_gpart);
+    }
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_gpart, ctx->ntiles, ctx->Epad, ctx->V, ctx->d_vOff, ctx->d_vEdge,
+                       ctx->d_vSlot, d_grad_soa);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_upload_positions_aos(rc_ctx* ctx, const double* xyz, double* d_dst) {
+    // stage AoS into d_tmp then transpose on device
+    RC_HIP(hipMemcpyAsync(ctx->d_stage, xyz, sizeof(double) * 3 * ctx->V, hipMemcpyHostToDevice,
+                          ctx->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_stage, ctx->V, d_dst);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_download_positions_aos(rc_ctx* ctx, const double* d_src, double* xyz) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream, d_src,
+                       ctx->V, ctx->d_stage);
+    RC_CHECK_LAUNCH();
+    RC_HIP(hipMemcpyAsync(xyz, ctx->d_stage, sizeof(double) * 3 * ctx->V, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    return RC_OK;
+}

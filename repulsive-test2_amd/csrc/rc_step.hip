@@ -1,0 +1,984 @@
+// Constraint rows, penalties, the (k+3)^2 Schur system, the FP64 residual and
+// the device-resident Armijo line search with frozen back-projection.
+//
+// Saddle reduction. With A_hat = A + mu 1 1^T (SPD; A 1 = 0), N = [1 (x) e_c]
+// and Y = (I3 (x) A_hat)^{-1} C^T, the reference system
+//   [[I3 (x) A, C^T],[C, 0]] [x; lambda] = [r; s]     (linsolve.ts:742-803)
+// is equivalent to x = (I3 (x) A_hat)^{-1}(r - C^T lambda) + N t / V with
+//   [[C Y, -C N / V],[(C N)^T, 0]] [lambda; t] = [C (I3 (x) A_hat)^{-1} r - s; N^T r].
+// The gradient solve uses r = dE, s = 0; every frozen projection solve
+// (lineSearch.ts:253-262) uses r = 0, s = -Phi and needs only Y and the
+// inverse of the small matrix: O(V k) per projection iterate, no solve.
+//
+// Line search (lineSearch.ts:423-562): each trial is two kernels, a
+// single-workgroup controller (candidate, projection loop, penalty energy,
+// edge geometry of the projected curve) and the grid-wide energy kernel whose
+// last workgroup applies the Armijo test. Trials are enqueued in batches; a
+// finished line search turns the remaining kernels into immediate returns.
+#include <cmath>
+
+#include "rc_internal.h"
+#include "rc_math.h"
+#include "rc_step.h"
+
+namespace {
+
+constexpr int CT = 1024;  // controller workgroup size
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v || o != o) ? o : v;  // NaN propagates like Math.max
+    }
+    return v;
+}
+
+// Block-wide sum over blockDim.x threads (fixed order). sh needs 32 doubles.
+__device__ double block_sum(double v, double* sh) {
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < nw; ++i) s += sh[i];
+    __syncthreads();
+    return s;
+}
+__device__ double block_max(double v, double* sh) {
+    v = wmax(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = sh[0];
+    for (int i = 1; i < nw; ++i) s = (sh[i] > s || sh[i] != sh[i]) ? sh[i] : s;
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double js_max(double a, double b) {
+    if (a != a || b != b) return NAN;
+    return a > b ? a : b;
+}
+
+// ---------------------------------------------------------------------------
+// constraint Jacobian C (k x 3Vp, row-major, coordinate blocks of stride Vp),
+// evaluated at the step base point from the edge table (raw lengths, guarded
+// unit tangents). One thread per vertex gathers its incident edges in edge
+// order (constraints.ts:67-120, constraintSet.ts:119-301). C pre-zeroed.
+__global__ void k_constraint_jacobian(EdgeSoA ed, int V, int Vp, const int32_t* __restrict__ vOff,
+                                      const int32_t* __restrict__ vEdge,
+                                      const int32_t* __restrict__ vSlot,
+                                      const DevSet* __restrict__ set, double* __restrict__ C) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const size_t m = 3 * (size_t)Vp;
+    for (int b = 0; b < set->nblocks; ++b) {
+        const int kind = set->kind[b];
+        const int row = set->row0[b];
+        if (kind == RC_BLOCK_BARYCENTER) {
+            double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+                const int K = vEdge[q];
+                const double sg = vSlot[q] ? 1.0 : -1.0;
+                const double T[3] = {ed.ux[K], ed.uy[K], ed.uz[K]};
+                const double ell = ed.rlen[K];
+                const double rv[3] = {0.5 * (ed.ax[K] + ed.bx[K]) - set->x0[0],
+                                      0.5 * (ed.ay[K] + ed.by[K]) - set->x0[1],
+                                      0.5 * (ed.az[K] + ed.bz[K]) - set->x0[2]};
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        acc[r][c] += sg * T[c] * rv[r] + 0.5 * ell * (r == c ? 1.0 : 0.0);
+            }
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) C[(row + r) * m + (size_t)c * Vp + v] = acc[r][c];
+        } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
+            double acc[3] = {0, 0, 0};
+            for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+                const int K = vEdge[q];
+                const double sg = vSlot[q] ? -1.0 : 1.0;
+                acc[0] += sg * ed.ux[K];
+                acc[1] += sg * ed.uy[K];
+                acc[2] += sg * ed.uz[K];
+            }
+            for (int c = 0; c < 3; ++c) C[row * m + (size_t)c * Vp + v] = acc[c];
+        } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
+            for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+                const int K = vEdge[q];
+                const double sg = vSlot[q] ? -1.0 : 1.0;
+                C[(row + K) * m + v] = sg * ed.ux[K];
+                C[(row + K) * m + (size_t)Vp + v] = sg * ed.uy[K];
+                C[(row + K) * m + 2 * (size_t)Vp + v] = sg * ed.uz[K];
+            }
+        } else if (kind == RC_BLOCK_POINT) {
+            const int pv = set->pin_vertex[set->pin_of_block[b]];
+            if (pv == v)
+                for (int r = 0; r < 3; ++r) C[(row + r) * m + (size_t)r * Vp + v] = 1.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// penalties (penalties.ts:139-241), per-vertex gather of the edge stencils.
+__global__ void k_penalty_grad_add(EdgeSoA ed, int V, int Vsrc, int Vdst,
+                                   const int32_t* __restrict__ vOff,
+                                   const int32_t* __restrict__ vEdge,
+                                   const int32_t* __restrict__ vSlot, DevPen pen,
+                                   const double* __restrict__ gsrc, double* __restrict__ gdst) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double g[3] = {gsrc[v], gsrc[Vsrc + v], gsrc[2 * (size_t)Vsrc + v]};
+    if (pen.active) {
+        double p[3] = {0, 0, 0};
+        for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+            const int K = vEdge[q];
+            const double sg = vSlot[q] ? 1.0 : -1.0;  // -at a, +at b
+            const double T[3] = {ed.ux[K], ed.uy[K], ed.uz[K]};
+            if (pen.wL != 0.0)
+                for (int c = 0; c < 3; ++c) p[c] += sg * pen.wL * T[c];
+            if (pen.wF != 0.0 && !(ed.rlen[K] < 1e-14)) {
+                const double u = T[0] * pen.X[0] + T[1] * pen.X[1] + T[2] * pen.X[2];
+                const double cc = 1.0 + u * u;
+                for (int c = 0; c < 3; ++c) p[c] += sg * pen.wF * (cc * T[c] - 2.0 * u * pen.X[c]);
+            }
+            if (pen.wD != 0.0) {
+                // interior (degree-2) endpoints w of K: K is w's first or second edge
+                for (int e2 = 0; e2 < 2; ++e2) {
+                    const int w = e2 ? ed.i2[K] : ed.i1[K];
+                    if (vOff[w + 1] - vOff[w] != 2) continue;
+                    const int rI = vEdge[vOff[w]], rJ = vEdge[vOff[w] + 1];
+                    const double s = 2.0 * pen.wD * (ed.rlen[rI] - ed.rlen[rJ]);
+                    const double sgn = (K == rI) ? 1.0 : -1.0;  // rI: -a +b ; rJ: +a -b
+                    for (int c = 0; c < 3; ++c) p[c] += sgn * sg * s * T[c];
+                }
+            }
+        }
+        g[0] += p[0];
+        g[1] += p[1];
+        g[2] += p[2];
+    }
+    gdst[v] = g[0];
+    gdst[Vdst + v] = g[1];
+    gdst[2 * (size_t)Vdst + v] = g[2];
+}
+
+// Penalty energy of positions held in the edge table (one workgroup).
+__device__ double penalty_energy_block(const EdgeSoA& ed, int E, const int32_t* vOff,
+                                       const int32_t* vEdge, int V, const DevPen& pen,
+                                       double* sh) {
+    double sL = 0.0, sF = 0.0, sD = 0.0;
+    for (int K = threadIdx.x; K < E; K += blockDim.x) {
+        const double l = ed.rlen[K];
+        sL += l;
+        if (pen.wF != 0.0) {
+            const double T[3] = {ed.ux[K], ed.uy[K], ed.uz[K]};
+            const double cx = T[1] * pen.X[2] - T[2] * pen.X[1];
+            const double cy = T[2] * pen.X[0] - T[0] * pen.X[2];
+            const double cz = T[0] * pen.X[1] - T[1] * pen.X[0];
+            sF += l * (cx * cx + cy * cy + cz * cz);
+        }
+    }
+    if (pen.wD != 0.0)
+        for (int w = threadIdx.x; w < V; w += blockDim.x)
+            if (vOff[w + 1] - vOff[w] == 2) {
+                const double d = ed.rlen[vEdge[vOff[w]]] - ed.rlen[vEdge[vOff[w] + 1]];
+                sD += d * d;
+            }
+    sL = block_sum(sL, sh);
+    sF = block_sum(sF, sh);
+    sD = block_sum(sD, sh);
+    double tot = 0.0;
+    if (pen.wL != 0.0) tot += pen.wL * sL;
+    if (pen.wD != 0.0) tot += pen.wD * sD;
+    if (pen.wF != 0.0) tot += pen.wF * sF;
+    return tot;
+}
+
+__global__ __launch_bounds__(CT) void k_penalty_energy(EdgeSoA ed, int E, int V,
+                                                       const int32_t* __restrict__ vOff,
+                                                       const int32_t* __restrict__ vEdge,
+                                                       DevPen pen, double* __restrict__ out) {
+    __shared__ double sh[32];
+    const double e = pen.active ? penalty_energy_block(ed, E, vOff, vEdge, V, pen, sh) : 0.0;
+    if (threadIdx.x == 0) out[0] = e;
+}
+
+// ---------------------------------------------------------------------------
+// right-hand sides [dE_x, dE_y, dE_z, C_0 blocks, C_1 blocks, ...] (ld = Vp)
+__global__ void k_build_rhs(const double* __restrict__ dE, const double* __restrict__ C, int V,
+                            int Vp, int k, double* __restrict__ B) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (v >= Vp) return;
+    double val = 0.0;
+    if (v < V) {
+        if (col < 3)
+            val = dE[(size_t)col * Vp + v];
+        else
+            val = C[(size_t)(col - 3) * Vp + v];  // row q = (col-3)/3, block c = (col-3)%3
+    }
+    B[(size_t)col * Vp + v] = val;
+}
+
+// out[p * nr + q] = <L_p, R_q> over len entries (L stride ls, R stride rs).
+__global__ __launch_bounds__(256) void k_dots(const double* __restrict__ L, size_t ls,
+                                              const double* __restrict__ R, size_t rs, int nr,
+                                              size_t len, double* __restrict__ out) {
+    __shared__ double sh[32];
+    const int p = blockIdx.x / nr, q = blockIdx.x % nr;
+    const double* a = L + p * ls;
+    const double* b = R + q * rs;
+    double s = 0.0;
+    for (size_t j = threadIdx.x; j < len; j += 256) s = fma(a[j], b[j], s);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// Column sums of the coordinate blocks of C rows: CN[p][c] = sum_v C[p][c*Vp+v];
+// also Nr[c] = sum_v dE[c*Vp + v] at CN[k*3 + c].
+__global__ __launch_bounds__(256) void k_block_sums(const double* __restrict__ C,
+                                                    const double* __restrict__ dE, int k, int V,
+                                                    int Vp, double* __restrict__ out) {
+    __shared__ double sh[32];
+    const int p = blockIdx.x / 3, c = blockIdx.x % 3;
+    const double* src = (p < k) ? C + (size_t)p * 3 * Vp + (size_t)c * Vp : dE + (size_t)c * Vp;
+    double s = 0.0;
+    for (int v = threadIdx.x; v < V; v += 256) s += src[v];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// Small system M = [[S, -CN/V],[CN^T, 0]], n = k + 3. Partial-pivot LU in
+// LDS; solves the gradient right-hand side and forms M^{-1} for the
+// projection solves. dots[p*(k+1) + 0] = <C_p, X_r>, dots[p*(k+1)+1+q] = S_pq.
+// out: sol[n] (lambda, t), Minv[n*n]. status bit 4 on a zero pivot.
+__global__ __launch_bounds__(256) void k_schur(const double* __restrict__ dots,
+                                               const double* __restrict__ bsum, int k, int V,
+                                               double* __restrict__ sol,
+                                               double* __restrict__ Minv,
+                                               int32_t* __restrict__ status) {
+    extern __shared__ double smem[];
+    const int n = k + 3;
+    double* M = smem;              // n x n
+    double* Z = smem + n * n;      // n x (n + 1) right-hand sides
+    int* piv = (int*)(Z + n * (n + 1));
+    const int t = threadIdx.x;
+    const double* CN = bsum;       // [k][3]
+    const double* Nr = bsum + 3 * k;
+    for (int q = t; q < n * n; q += blockDim.x) {
+        const int i = q / n, j = q % n;
+        double v = 0.0;
+        if (i < k && j < k) v = dots[i * (k + 1) + 1 + j];
+        else if (i < k) v = -CN[i * 3 + (j - k)] / V;
+        else if (j < k) v = CN[j * 3 + (i - k)];
+        M[q] = v;
+    }
+    for (int q = t; q < n * (n + 1); q += blockDim.x) {
+        const int i = q / (n + 1), j = q % (n + 1);
+        double v;
+        if (j == 0) v = (i < k) ? dots[i * (k + 1)] : Nr[i - k];
+        else v = (i == j - 1) ? 1.0 : 0.0;
+        Z[q] = v;
+    }
+    __syncthreads();
+    __shared__ int bad;
+    if (t == 0) bad = 0;
+    for (int c = 0; c < n; ++c) {
+        __syncthreads();
+        if (t == 0) {
+            int pr = c;
+            double mx = fabs(M[c * n + c]);
+            for (int r = c + 1; r < n; ++r)
+                if (fabs(M[r * n + c]) > mx) {
+                    mx = fabs(M[r * n + c]);
+                    pr = r;
+                }
+            piv[c] = pr;
+            if (!(mx > 0.0)) bad = 1;
+        }
+        __syncthreads();
+        const int pr = piv[c];
+        if (pr != c) {
+            for (int j = t; j < n; j += blockDim.x) {
+                const double tmp = M[c * n + j];
+                M[c * n + j] = M[pr * n + j];
+                M[pr * n + j] = tmp;
+            }
+            for (int j = t; j < n + 1; j += blockDim.x) {
+                const double tmp = Z[c * (n + 1) + j];
+                Z[c * (n + 1) + j] = Z[pr * (n + 1) + j];
+                Z[pr * (n + 1) + j] = tmp;
+            }
+        }
+        __syncthreads();
+        const double pv = M[c * n + c];
+        for (int q = t; q < (n - c - 1) * (n + 1 + n - c - 1); q += blockDim.x) {
+            const int w = n + 1 + n - c - 1;
+            const int r = c + 1 + q / w, j = q % w;
+            const double f = (pv != 0.0) ? M[r * n + c] / pv : 0.0;
+            if (j < n - c - 1)
+                M[r * n + c + 1 + j] -= f * M[c * n + c + 1 + j];
+            else
+                Z[r * (n + 1) + (j - (n - c - 1))] -= f * Z[c * (n + 1) + (j - (n - c - 1))];
+        }
+        __syncthreads();
+    }
+    // back substitution for all n+1 columns
+    for (int j = t; j < n + 1; j += blockDim.x) {
+        for (int i = n - 1; i >= 0; --i) {
+            double s = Z[i * (n + 1) + j];
+            for (int m = i + 1; m < n; ++m) s -= M[i * n + m] * Z[m * (n + 1) + j];
+            Z[i * (n + 1) + j] = (M[i * n + i] != 0.0) ? s / M[i * n + i] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int q = t; q < n; q += blockDim.x) sol[q] = Z[q * (n + 1)];
+    for (int q = t; q < n * n; q += blockDim.x) {
+        const int i = q / n, j = q % n;
+        Minv[q] = Z[i * (n + 1) + 1 + j];
+    }
+    if (t == 0 && bad) atomicOr(status, 4);
+}
+
+// g~_c[v] = X_c[v] - sum_q lambda_q Y_{q,c}[v] + t_c / V   (stride Vp)
+__global__ void k_gtilde(const double* __restrict__ X, const double* __restrict__ sol, int k,
+                         int V, int Vp, double* __restrict__ gt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (v >= Vp) return;
+    double s = 0.0;
+    if (v < V) {
+        s = X[(size_t)c * Vp + v];
+        for (int q = 0; q < k; ++q) s -= sol[q] * X[(size_t)(3 + 3 * q + c) * Vp + v];
+        s += sol[k + c] / V;
+    }
+    gt[(size_t)c * Vp + v] = s;
+}
+
+// y_c = A x_c for the three coordinate blocks (x stride Vp, y stride Vp).
+__global__ __launch_bounds__(256) void k_gemv3(const double* __restrict__ A, int V, int Vp,
+                                               const double* __restrict__ x,
+                                               double* __restrict__ y) {
+    __shared__ double sh[32];
+    const int u = blockIdx.x;
+    const double* row = A + (size_t)u * V;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int j = threadIdx.x; j < V; j += 256) {
+        const double a = row[j];
+        s0 = fma(a, x[j], s0);
+        s1 = fma(a, x[Vp + j], s1);
+        s2 = fma(a, x[2 * (size_t)Vp + j], s2);
+    }
+    s0 = block_sum(s0, sh);
+    s1 = block_sum(s1, sh);
+    s2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        y[u] = s0;
+        y[Vp + u] = s1;
+        y[2 * (size_t)Vp + u] = s2;
+    }
+}
+
+// ||K z - rhs|| / max(1, ||rhs||) (linsolve.ts:810-849) with z = [g~; lambda],
+// rhs = [dE; 0]; Ax = A g~ blocks, Cg = C g~.
+__global__ __launch_bounds__(CT) void k_residual(const double* __restrict__ Ax,
+                                                 const double* __restrict__ C,
+                                                 const double* __restrict__ sol,
+                                                 const double* __restrict__ dE,
+                                                 const double* __restrict__ Cg, int k, int V,
+                                                 int Vp, LSCtl* ctl) {
+    __shared__ double sh[32];
+    double r2 = 0.0, b2 = 0.0;
+    for (int q = threadIdx.x; q < 3 * V; q += CT) {
+        const int c = q / V, v = q % V;
+        const size_t j = (size_t)c * Vp + v;
+        double s = Ax[j];
+        for (int p = 0; p < k; ++p) s += C[(size_t)p * 3 * Vp + j] * sol[p];
+        const double d = s - dE[j];
+        r2 += d * d;
+        b2 += dE[j] * dE[j];
+    }
+    for (int p = threadIdx.x; p < k; p += CT) r2 += Cg[p] * Cg[p];
+    r2 = block_sum(r2, sh);
+    b2 = block_sum(b2, sh);
+    if (threadIdx.x == 0) ctl->residual = sqrt(r2) / js_max(1.0, sqrt(b2));
+}
+
+// ---------------------------------------------------------------------------
+// line search
+
+// Initialise the control block after the solve: L2h norm, direction, slope,
+// converged / failure reasons (optimizer.ts:331-367, lineSearch.ts:459-497).
+__global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int Vp,
+                                                 const double* __restrict__ gt,
+                                                 const double* __restrict__ dE,
+                                                 double* __restrict__ dir,
+                                                 const double* __restrict__ e0tpe,
+                                                 const double* __restrict__ e0pen,
+                                                 const int32_t* __restrict__ status,
+                                                 int has_e0, double e0_given, double tau0,
+                                                 LSCtl* ctl) {
+    __shared__ double sh[32];
+    __shared__ double gn_s;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int K = t; K < E; K += CT) {
+        const int a = ed.i1[K], b = ed.i2[K];
+        const double l = ed.rlen[K];
+        double n1 = 0.0, n2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double x1 = gt[(size_t)c * Vp + a], x2 = gt[(size_t)c * Vp + b];
+            n1 += x1 * x1;
+            n2 += x2 * x2;
+        }
+        s += 0.5 * l * (n1 + n2);
+    }
+    s = block_sum(s, sh);
+    const double e0 = has_e0 ? e0_given : (e0tpe[0] + e0pen[0]);
+    // finite check of g~ (a non-finite solve is the reference's "treating as singular")
+    double nf = 0.0;
+    for (int q = t; q < 3 * V; q += CT) {
+        const double x = gt[(size_t)(q / V) * Vp + q % V];
+        if (!(x - x == 0.0)) nf = 1.0;
+    }
+    nf = block_sum(nf, sh);
+    if (t == 0) gn_s = sqrt(s > 0.0 ? s : 0.0);
+    __syncthreads();
+    const double gn = gn_s;
+    const bool singular = (status[0] & (1 | 4)) != 0 || nf > 0.0;
+    double sl = 0.0;
+    if (!singular && gn >= 1e-4 && gn > 0.0 && gn - gn == 0.0) {
+        for (int q = t; q < 3 * V; q += CT) {
+            const size_t j = (size_t)(q / V) * Vp + q % V;
+            const double d = gt[j] / gn;
+            dir[j] = d;
+            sl += dE[j] * d;
+        }
+    }
+    sl = block_sum(sl, sh);
+    if (t == 0) {
+        ctl->e0 = e0;
+        ctl->e1 = e0;
+        ctl->tau = tau0;
+        ctl->gnorm = gn;
+        ctl->slope = sl;
+        ctl->accepted = 0;
+        ctl->converged = 0;
+        ctl->trials = 0;
+        ctl->proj_iters = 0;
+        ctl->proj_iters_acc = -1;
+        ctl->tau_acc = 0.0;
+        ctl->phi_norm = NAN;
+        ctl->skip_energy = 1;
+        ctl->counter = 0;
+        ctl->last_reason = RC_REASON_TAU_BELOW_MIN;
+        ctl->reason = RC_REASON_NONE;
+        ctl->done = 0;
+        if (singular) {
+            ctl->done = 1;
+            ctl->reason = RC_REASON_SINGULAR_SYSTEM;
+            ctl->residual = NAN;
+            ctl->gnorm = NAN;
+        } else if (gn < 1e-4) {
+            // optimizer.ts:355-367 (a NaN norm fails this test and goes on)
+            ctl->done = 1;
+            ctl->converged = 1;
+            ctl->proj_iters_acc = 0;
+        } else if (!(gn - gn == 0.0) || gn <= 0.0) {
+            ctl->done = 1;
+            ctl->reason = RC_REASON_ZERO_OR_NONFINITE_GRADIENT_NORM;
+            ctl->proj_iters_acc = 0;
+        } else if (!(sl - sl == 0.0) || sl <= 0.0) {
+            ctl->done = 1;
+            ctl->reason = RC_REASON_NOT_A_DESCENT_DIRECTION;
+            ctl->proj_iters_acc = 0;
+        }
+    }
+}
+
+__device__ void edge_prep_block(EdgeSoA& ed, const double* pos, int V, int E, double alpha,
+                                double eps) {
+    for (int K = threadIdx.x; K < E; K += blockDim.x) {
+        const int a = ed.i1[K], b = ed.i2[K];
+        const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
+        const double bx = pos[b], by = pos[V + b], bz = pos[2 * V + b];
+        const double ex = bx - ax, ey = by - ay, ez = bz - az;
+        const double r = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+        const double ell = r + eps;
+        double ux = 0, uy = 0, uz = 0;
+        if (!(r < 1e-14)) {
+            const double inv = 1.0 / r;
+            ux = inv * ex;
+            uy = inv * ey;
+            uz = inv * ez;
+        }
+        ed.ax[K] = ax; ed.ay[K] = ay; ed.az[K] = az;
+        ed.bx[K] = bx; ed.by[K] = by; ed.bz[K] = bz;
+        ed.ell[K] = ell;
+        ed.rlen[K] = r;
+        ed.ux[K] = ux; ed.uy[K] = uy; ed.uz[K] = uz;
+        ed.lpow[K] = pow(ell, 1.0 - alpha);
+        ed.dpc[K] = (1.0 - alpha) * pow(ell, -alpha);
+    }
+}
+
+// One trial's candidate + frozen projection (lineSearch.ts:196-295,501-548).
+// Single workgroup. Y = X columns 3.. (A_hat^{-1} C^T blocks), Minv = M^{-1}.
+__global__ __launch_bounds__(CT) void k_ls_project(
+    LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0, DevPen pen,
+    const double* __restrict__ pos0, const double* __restrict__ dir, double* __restrict__ cur,
+    EdgeSoA ed, int V, int E, int Vp, const double* __restrict__ Y,
+    const double* __restrict__ Minv, const int32_t* __restrict__ vOff,
+    const int32_t* __restrict__ vEdge, double alpha, double eps) {
+    __shared__ double sh[32];
+    __shared__ double phi[RC_KMAX + 8];
+    __shared__ double solv[RC_KMAX + 8];
+    __shared__ double Rpin[RC_MAXPIN];
+    __shared__ int flag_s;
+    if (ctl->done) {
+        if (threadIdx.x == 0) ctl->skip_energy = 1;
+        return;
+    }
+    const int t = threadIdx.x;
+    const double tau = ctl->tau;
+    const int k = set->k;
+    const int n = k + 3;
+    for (int q = t; q < 3 * V; q += CT) {
+        const int c = q / V, v = q % V;
+        cur[(size_t)c * V + v] = pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v];
+    }
+    __syncthreads();
+    int ok = 0, it = 0;
+    double phin = INFINITY;
+    const int maxit = ctl->max_iter;
+    for (it = 0; it <= maxit; ++it) {
+        // ---- evaluate Phi and the per-block scales at cur
+        double L = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pe2 = 0.0;
+        for (int K = t; K < E; K += CT) {
+            const int a = ed.i1[K], b = ed.i2[K];
+            const double ax = cur[a], ay = cur[V + a], az = cur[2 * V + a];
+            const double bx = cur[b], by = cur[V + b], bz = cur[2 * V + b];
+            const double ex = bx - ax, ey = by - ay, ez = bz - az;
+            const double l = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+            L += l;
+            if (set->has_bary) {
+                m0 += l * (0.5 * (ax + bx) - set->x0[0]);
+                m1 += l * (0.5 * (ay + by) - set->x0[1]);
+                m2 += l * (0.5 * (az + bz) - set->x0[2]);
+            }
+            if (set->len_per_edge) {
+                const double ph = ell0[K] - l;
+                phi[set->len_row0 + K] = ph;
+                pe2 += ph * ph;
+            }
+        }
+        L = block_sum(L, sh);
+        m0 = block_sum(m0, sh);
+        m1 = block_sum(m1, sh);
+        m2 = block_sum(m2, sh);
+        pe2 = block_sum(pe2, sh);
+        double Rb = 0.0;
+        if (set->has_bary)
+            for (int v = t; v < V; v += CT) {
+                if (vOff[v + 1] == vOff[v]) continue;  // endpoints of edges only
+                const double dx = cur[v] - set->x0[0], dy = cur[V + v] - set->x0[1],
+                             dz = cur[2 * V + v] - set->x0[2];
+                Rb = js_max(Rb, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        Rb = block_max(Rb, sh);
+        for (int pn = 0; pn < set->npin; ++pn) {
+            double R = 0.0;
+            const double* tg = set->pin_target[pn];
+            for (int v = t; v < V; v += CT) {
+                const double dx = cur[v] - tg[0], dy = cur[V + v] - tg[1], dz = cur[2 * V + v] - tg[2];
+                R = js_max(R, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+            }
+            R = block_max(R, sh);
+            if (t == 0) Rpin[pn] = R;
+        }
+        if (t == 0) {
+            // stack Phi in set order and test every block (lineSearch.ts:221-249)
+            bool conv = true;
+            double tot = 0.0;
+            for (int b = 0; b < set->nblocks; ++b) {
+                const int kind = set->kind[b], r0 = set->row0[b];
+                double bn2 = 0.0, scale = 1.0;
+                if (kind == RC_BLOCK_BARYCENTER) {
+                    phi[r0] = m0; phi[r0 + 1] = m1; phi[r0 + 2] = m2;
+                    bn2 = m0 * m0 + m1 * m1 + m2 * m2;
+                    scale = js_max(1.0, L * js_max(1.0, Rb));
+                } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
+                    phi[r0] = set->L0 - L;
+                    bn2 = phi[r0] * phi[r0];
+                    scale = js_max(1.0, L);
+                } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
+                    bn2 = pe2;
+                    scale = js_max(1.0, L);
+                } else {
+                    const int pn = set->pin_of_block[b];
+                    const int pv = set->pin_vertex[pn];
+                    const double* tg = set->pin_target[pn];
+                    if (pv >= 0 && pv < V) {
+                        phi[r0] = cur[pv] - tg[0];
+                        phi[r0 + 1] = cur[V + pv] - tg[1];
+                        phi[r0 + 2] = cur[2 * V + pv] - tg[2];
+                    } else {
+                        phi[r0] = phi[r0 + 1] = phi[r0 + 2] = NAN;
+                    }
+                    bn2 = phi[r0] * phi[r0] + phi[r0 + 1] * phi[r0 + 1] + phi[r0 + 2] * phi[r0 + 2];
+                    scale = js_max(1.0, Rpin[pn]);
+                }
+                tot += bn2;
+                const double bn = sqrt(bn2);
+                if (conv && bn > js_max(ctl->tol_abs, ctl->tol_rel * scale)) conv = false;
+            }
+            phin = sqrt(tot);
+            flag_s = conv ? 1 : 0;
+            sh[31] = phin;
+        }
+        __syncthreads();
+        phin = sh[31];
+        if (flag_s) {
+            ok = 1;
+            break;
+        }
+        if (it == maxit) break;
+        // ---- frozen correction: [lambda; t] = M^{-1} [Phi; 0], x = -Y lambda + N t / V
+        for (int i = t; i < n; i += CT) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += Minv[i * n + j] * phi[j];
+            solv[i] = s;
+        }
+        __syncthreads();
+        double nf = 0.0;
+        for (int q = t; q < 3 * V; q += CT) {
+            const int c = q / V, v = q % V;
+            double x = solv[k + c] / V;
+            for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+            if (!(x - x == 0.0)) nf = 1.0;
+            cur[(size_t)c * V + v] += x;
+        }
+        nf = block_sum(nf, sh);
+        if (nf > 0.0) break;  // non-finite step -> ok = false (lineSearch.ts:283-288)
+        __syncthreads();
+    }
+    // finite check of the candidate (lineSearch.ts:525-527)
+    double nfc = 0.0;
+    for (int q = t; q < 3 * V; q += CT)
+        if (!(cur[q] - cur[q] == 0.0)) nfc = 1.0;
+    nfc = block_sum(nfc, sh);
+    if (it > maxit) it = maxit;
+    if (ok && nfc == 0.0) {
+        edge_prep_block(ed, cur, V, E, alpha, eps);
+        __syncthreads();
+        __threadfence_block();
+        const double pe = pen.active ? penalty_energy_block(ed, E, vOff, vEdge, V, pen, sh) : 0.0;
+        if (t == 0) {
+            ctl->penE = pe;
+            ctl->proj_iters = it;
+            ctl->phi_norm = phin;
+            ctl->proj_ok = 1;
+            ctl->skip_energy = 0;
+        }
+    } else if (t == 0) {
+        ctl->proj_ok = 0;
+        ctl->skip_energy = 1;
+        ctl->trials += 1;
+        ctl->last_reason = RC_REASON_PROJECTION_FAILED;
+        ctl->tau = tau * ctl->shrink;
+        if (!(ctl->tau >= ctl->tau_min)) {
+            ctl->done = 1;
+            ctl->reason = ctl->last_reason;
+        }
+    }
+}
+
+// Energy of the projected trial (edge table) with the Armijo decision in the
+// last workgroup to finish (integer ticket; no float atomics).
+template <class P>
+__global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs, P pw,
+                                                   double eps, double* __restrict__ part,
+                                                   LSCtl* ctl);
+
+__global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restrict__ pos,
+                                                    const double* __restrict__ cur, int V,
+                                                    int update, double* __restrict__ stats) {
+    const int accepted = ctl->accepted;
+    if (update && accepted)
+        for (int q = threadIdx.x; q < 3 * V; q += CT) pos[q] = cur[q];
+    if (threadIdx.x == 0) {
+        // layout mirrors rc_step_stats' leading doubles then ints
+        stats[0] = accepted ? ctl->e1 : ctl->e0;
+        stats[1] = ctl->e0;
+        stats[2] = accepted ? ctl->tau_acc : 0.0;
+        stats[3] = ctl->residual;
+        stats[4] = ctl->gnorm;
+        stats[5] = ctl->slope;
+        stats[6] = accepted ? ctl->phi_norm : NAN;
+        stats[7] = ctl->refine_res;
+        int32_t* is = (int32_t*)(stats + 8);
+        is[0] = accepted;
+        is[1] = ctl->converged;
+        is[2] = accepted ? ctl->proj_iters_acc : (ctl->converged ? 0 : ctl->proj_iters_acc);
+        is[3] = ctl->reason;
+        is[4] = ctl->trials;
+        is[5] = ctl->refine_iters;
+    }
+}
+
+}  // namespace
+
+// The energy kernel with Armijo decision lives here so it can share the
+// pair_energy device code through a local copy (identical arithmetic to
+// rc_pairs.hip's forward energy).
+namespace {
+
+__device__ __forceinline__ void tri_decode_e(int p, int& a, int& b) {
+    int bb = (int)((__builtin_sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((bb + 1) * (bb + 2) / 2 <= p) ++bb;
+    while (bb * (bb + 1) / 2 > p) --bb;
+    b = bb;
+    a = p - bb * (bb + 1) / 2;
+}
+
+template <class P>
+__device__ __forceinline__ double pair_energy_ls(const P& pw, double eps, const double* PI,
+                                                 const double* PJ, double eIx, double eIy,
+                                                 double eIz, double eJx, double eJy, double eJz,
+                                                 double ellI, double ellJ, double lpI,
+                                                 double lpJ) {
+    double sFI = 0.0, sFJ = 0.0;
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const double dx = PI[3 * ia] - PJ[3 * jb];
+            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
+            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
+            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+            const double invB = pw.inv_beta(deps, rcm::rcp(deps));
+            const double cIx = eIy * dz - eIz * dy;
+            const double cIy = eIz * dx - eIx * dz;
+            const double cIz = eIx * dy - eIy * dx;
+            const double cJx = eJy * dz - eJz * dy;
+            const double cJy = eJz * dx - eJx * dz;
+            const double cJz = eJx * dy - eJy * dx;
+            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
+            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
+            sFI += pw.c_alpha(nI) * invB;
+            sFJ += pw.c_alpha(nJ) * invB;
+        }
+    }
+    return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs, P pw,
+                                                   double eps, double* __restrict__ part,
+                                                   LSCtl* ctl) {
+    __shared__ double wsums[4];
+    __shared__ int last;
+    if (ctl->skip_energy) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double sum = 0.0;
+    for (int base = blockIdx.x * 4; base < npairs; base += gridDim.x * 4) {
+        const int p = __builtin_amdgcn_readfirstlane(base + wave);
+        if (p >= npairs) break;
+        int a, b;
+        tri_decode_e(p, a, b);
+        const int I = a * RC_TILE + lane;
+        const bool Iok = I < E;
+        const int Ic = Iok ? I : 0;
+        const double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
+        const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
+        const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
+        const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
+        const bool diag = (a == b);
+        for (int s = 0; s < RC_TILE; ++s) {
+            const int J = b * RC_TILE + s;
+            if (J >= E) break;
+            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
+            const int j1 = ed.i1[J], j2 = ed.i2[J];
+            const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
+                               i2 != j2;
+            const double e = pair_energy_ls(pw, eps, PI, PJ, eIx, eIy, eIz, PJ[3] - PJ[0],
+                                            PJ[4] - PJ[1], PJ[5] - PJ[2], ellI, ed.ell[J], lpI,
+                                            ed.lpow[J]);
+            sum += valid ? e : 0.0;
+        }
+    }
+    sum = wsum(sum);
+    if (lane == 0) wsums[wave] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ((wsums[0] + wsums[1]) + wsums[2]) + wsums[3];
+        __threadfence();
+        const int ticket = atomicAdd(&ctl->counter, 1);
+        last = (ticket == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    __shared__ double sh[32];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += 256)
+        s += __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const double e1 = 0.5 * s + ctl->penE;
+        ctl->trials += 1;
+        ctl->counter = 0;
+        const double tau = ctl->tau;
+        if (e1 - e1 == 0.0 && e1 <= ctl->e0 - ctl->c1 * tau * ctl->slope) {
+            ctl->accepted = 1;
+            ctl->done = 1;
+            ctl->tau_acc = tau;
+            ctl->e1 = e1;
+            ctl->proj_iters_acc = ctl->proj_iters;
+            ctl->reason = RC_REASON_NONE;
+        } else {
+            ctl->last_reason = RC_REASON_ARMIJO_FAILED;
+            ctl->tau = tau * ctl->shrink;
+            if (!(ctl->tau >= ctl->tau_min)) {
+                ctl->done = 1;
+                ctl->reason = ctl->last_reason;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host launchers
+
+rc_status rc_launch_constraint_jacobian(rc_ctx* ctx, const DevSet* d_set, int k) {
+    const int Vp = ctx->nblk * RC_NB;
+    RC_HIP(hipMemsetAsync(ctx->d_C, 0, sizeof(double) * (size_t)k * 3 * Vp, ctx->stream));
+    hipLaunchKernelGGL(k_constraint_jacobian, dim3((ctx->V + 255) / 256), dim3(256), 0,
+                       ctx->stream, ctx->ed, ctx->V, Vp, ctx->d_vOff, ctx->d_vEdge, ctx->d_vSlot,
+                       d_set, ctx->d_C);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_penalty_grad(rc_ctx* ctx, const DevPen& pen, const double* gsrc, int Vsrc,
+                                 double* gdst, int Vdst) {
+    hipLaunchKernelGGL(k_penalty_grad_add, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->ed, ctx->V, Vsrc, Vdst, ctx->d_vOff, ctx->d_vEdge, ctx->d_vSlot, pen,
+                       gsrc, gdst);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_penalty_energy(rc_ctx* ctx, const DevPen& pen, double* d_out) {
+    hipLaunchKernelGGL(k_penalty_energy, dim3(1), dim3(CT), 0, ctx->stream, ctx->ed, ctx->E,
+                       ctx->V, ctx->d_vOff, ctx->d_vEdge, pen, d_out);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_build_rhs, dim3((Vp + 255) / 256, 3 + 3 * k), dim3(256), 0, ctx->stream,
+                       dE, ctx->d_C, ctx->V, Vp, k, B);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+// Schur phase after X = A_hat^{-1} B: dots, block sums, small system, g~.
+rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k, double* gt,
+                          double* d_dots, double* d_bsum, double* d_sol, double* d_Minv) {
+    const int Vp = ctx->nblk * RC_NB;
+    const size_t len = 3 * (size_t)Vp;
+    if (k > 0) {
+        // dots[p][0] = <C_p, X_r>, dots[p][1+q] = <C_p, Y_q>; X columns are
+        // consecutive Vp blocks, so X_r and Y_q are contiguous 3Vp vectors.
+        hipLaunchKernelGGL(k_dots, dim3(k * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, len, X,
+                           len, k + 1, len, d_dots);
+        RC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_block_sums, dim3(3 * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, dE, k,
+                       ctx->V, Vp, d_bsum);
+    RC_CHECK_LAUNCH();
+    const int n = k + 3;
+    const size_t shm = sizeof(double) * ((size_t)n * n + (size_t)n * (n + 1)) + sizeof(int) * n;
+    if (k > 0) {
+        hipLaunchKernelGGL(k_schur, dim3(1), dim3(256), shm, ctx->stream, d_dots, d_bsum, k, ctx->V,
+                           d_sol, d_Minv, ctx->d_status);
+        RC_CHECK_LAUNCH();
+    } else {
+        RC_HIP(hipMemsetAsync(d_sol, 0, sizeof(double) * 3, ctx->stream));
+    }
+    hipLaunchKernelGGL(k_gtilde, dim3((Vp + 255) / 256, 3), dim3(256), 0, ctx->stream, X, d_sol, k,
+                       ctx->V, Vp, gt);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
+                             const double* d_sol, double* d_Ax, double* d_Cg, LSCtl* d_ctl) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_gemv3, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_A, ctx->V, Vp, gt,
+                       d_Ax);
+    RC_CHECK_LAUNCH();
+    if (k > 0) {
+        hipLaunchKernelGGL(k_dots, dim3(k), dim3(256), 0, ctx->stream, ctx->d_C, 3 * (size_t)Vp, gt,
+                           (size_t)0, 1, 3 * (size_t)Vp, d_Cg);
+        RC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(CT), 0, ctx->stream, d_Ax, ctx->d_C, d_sol, dE,
+                       d_Cg, k, ctx->V, Vp, d_ctl);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
+                             const double* e0tpe, const double* e0pen, int has_e0,
+                             double e0_given, double tau0, LSCtl* d_ctl) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_ls_setup, dim3(1), dim3(CT), 0, ctx->stream, ctx->ed, ctx->E, ctx->V, Vp,
+                       gt, dE, dir, e0tpe, e0pen, ctx->d_status, has_e0, e0_given, tau0, d_ctl);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
+                             const rc_params* p, const double* pos0, const double* dir,
+                             double* cur, const double* Y, const double* Minv,
+                             double* d_epart_ls) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_ls_project, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, d_set, ctx->d_ell0,
+                       pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv, ctx->d_vOff,
+                       ctx->d_vEdge, p->alpha, p->epsilon);
+    RC_CHECK_LAUNCH();
+    int grid = (ctx->npairs + 3) / 4;
+    if (grid > 1024) grid = 1024;
+    if (p->alpha == 3.0 && p->beta == 6.0) {
+        hipLaunchKernelGGL(k_energy_ls<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->npairs, Pow36{}, p->epsilon, d_epart_ls, d_ctl);
+    } else {
+        const double s = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+        hipLaunchKernelGGL(k_energy_ls<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->npairs, pw, p->epsilon, d_epart_ls, d_ctl);
+    }
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
+                                int update, double* d_stats) {
+    hipLaunchKernelGGL(k_ls_finalize, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, pos, cur, ctx->V,
+                       update, d_stats);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
