@@ -101,6 +101,8 @@ def accuracy(v, e, cs, pen):
     rmx = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
         penalties=pen, collect_field=True, precision="mixed"))
     g64, gmx = r64["descentField"], rmx["descentField"]
+    if g64 is None or gmx is None:
+        return {"accuracy_error": "singular saddle: " + str(r64["stats"]) + " / " + str(rmx["stats"])}
     return {
         "energy_rel_err": abs(rmx["energy"] - r64["energy"]) / abs(r64["energy"]),
         "direction_rel_err": float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),

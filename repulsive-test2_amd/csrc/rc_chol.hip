@@ -129,9 +129,11 @@ __global__ __launch_bounds__(256) void k_potrf_panel(T* __restrict__ F, int Vp, 
     }
     potrf64_lds(D, Li, &bad);
     if (blockIdx.x == 0) {
+        // Only L_kk^-1 is stored: the solves use it for the diagonal blocks and
+        // never read F's diagonal blocks, which the other blocks of this launch
+        // are still reading (so they must not be overwritten here).
         for (int q = t; q < NB * NB; q += 256) {
             const int i = q / NB, j = q % NB;
-            if (j <= i) F[(size_t)(k0 + i) * Vp + k0 + j] = (T)D[i][j];
             Linv[((size_t)kb * NB + i) * NB + j] = (T)Li[i][j];
         }
         if (t == 0 && bad) atomicOr(status, bad_bit);

@@ -216,10 +216,16 @@ template <class P>
 __device__ __forceinline__ void pair_grad(const P& pw, double eps, double alpha, double beta,
                                           const EdgeRegs& I, const EdgeRegs& J, double* cI,
                                           double* cJ) {
+    // Contributions accumulate straight into the four endpoint slots, scaled
+    // by the ordered-pair bases b_IJ = ell_I^(1-a) ell_J / 4, b_JI likewise.
+    const double bIJ = 0.25 * I.lp * J.ell;
+    const double bJI = 0.25 * J.lp * I.ell;
     double sFI = 0.0, sFJ = 0.0;
-    double DI[2][3] = {{0, 0, 0}, {0, 0, 0}}, DJfI[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double DJ[2][3] = {{0, 0, 0}, {0, 0, 0}}, DIfJ[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double SdeI[3] = {0, 0, 0}, SdeJ[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        cI[c] = 0.0;
+        cJ[c] = 0.0;
+    }
 #pragma unroll
     for (int ia = 0; ia < 2; ++ia) {
 #pragma unroll
@@ -235,32 +241,47 @@ __device__ __forceinline__ void pair_grad(const P& pw, double eps, double alpha,
             const double inv = rcm::rcp(deps);
             const double invB = pw.inv_beta(deps, inv);
             const double invB1 = invB * inv;
-            const KD a = kd(pw, eps, alpha, beta, I.e[0], I.e[1], I.e[2], dx, dy, dz, dhx, dhy,
-                            dhz, invB, invB1);
-            const KD c = kd(pw, eps, alpha, beta, J.e[0], J.e[1], J.e[2], -dx, -dy, -dz, -dhx,
-                            -dhy, -dhz, invB, invB1);
-            sFI += a.f;
-            sFJ += c.f;
-            DI[ia][0] += a.ddx; DI[ia][1] += a.ddy; DI[ia][2] += a.ddz;
-            DJfI[jb][0] += a.ddx; DJfI[jb][1] += a.ddy; DJfI[jb][2] += a.ddz;
-            SdeI[0] += a.dex; SdeI[1] += a.dey; SdeI[2] += a.dez;
-            DJ[jb][0] += c.ddx; DJ[jb][1] += c.ddy; DJ[jb][2] += c.ddz;
-            DIfJ[ia][0] += c.ddx; DIfJ[ia][1] += c.ddy; DIfJ[ia][2] += c.ddz;
-            SdeJ[0] += c.dex; SdeJ[1] += c.dey; SdeJ[2] += c.dez;
+            {
+                // ordered term (I, J): d = p_i - q_j, tangent e_I
+                const KD a = kd(pw, eps, alpha, beta, I.e[0], I.e[1], I.e[2], dx, dy, dz, dhx,
+                                dhy, dhz, invB, invB1);
+                sFI += a.f;
+                cI[3 * ia + 0] += bIJ * a.ddx;
+                cI[3 * ia + 1] += bIJ * a.ddy;
+                cI[3 * ia + 2] += bIJ * a.ddz;
+                cJ[3 * jb + 0] -= bIJ * a.ddx;
+                cJ[3 * jb + 1] -= bIJ * a.ddy;
+                cJ[3 * jb + 2] -= bIJ * a.ddz;
+                cI[0] -= bIJ * a.dex; cI[1] -= bIJ * a.dey; cI[2] -= bIJ * a.dez;
+                cI[3] += bIJ * a.dex; cI[4] += bIJ * a.dey; cI[5] += bIJ * a.dez;
+            }
+            {
+                // ordered term (J, I): d = q_j - p_i, tangent e_J
+                const KD c = kd(pw, eps, alpha, beta, J.e[0], J.e[1], J.e[2], -dx, -dy, -dz,
+                                -dhx, -dhy, -dhz, invB, invB1);
+                sFJ += c.f;
+                cJ[3 * jb + 0] += bJI * c.ddx;
+                cJ[3 * jb + 1] += bJI * c.ddy;
+                cJ[3 * jb + 2] += bJI * c.ddz;
+                cI[3 * ia + 0] -= bJI * c.ddx;
+                cI[3 * ia + 1] -= bJI * c.ddy;
+                cI[3 * ia + 2] -= bJI * c.ddz;
+                cJ[0] -= bJI * c.dex; cJ[1] -= bJI * c.dey; cJ[2] -= bJI * c.dez;
+                cJ[3] += bJI * c.dex; cJ[4] += bJI * c.dey; cJ[5] += bJI * c.dez;
+            }
         }
     }
-    const double bIJ = 0.25 * I.lp * J.ell;
-    const double bJI = 0.25 * J.lp * I.ell;
+    // length derivatives: d(ell_I^(1-a)) and d(ell_I) carry -u_I at i1, +u_I at i2
     const double gIs = 0.25 * J.ell * sFI * I.dpc + 0.25 * J.lp * sFJ;
     const double gJs = 0.25 * I.lp * sFI + 0.25 * I.ell * sFJ * J.dpc;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double gI = gIs * I.u[c];
         const double gJ = gJs * J.u[c];
-        cI[c] = bIJ * DI[0][c] - bJI * DIfJ[0][c] - bIJ * SdeI[c] - gI;
-        cI[3 + c] = bIJ * DI[1][c] - bJI * DIfJ[1][c] + bIJ * SdeI[c] + gI;
-        cJ[c] = bJI * DJ[0][c] - bIJ * DJfI[0][c] - bJI * SdeJ[c] - gJ;
-        cJ[3 + c] = bJI * DJ[1][c] - bIJ * DJfI[1][c] + bJI * SdeJ[c] + gJ;
+        cI[c] -= gI;
+        cI[3 + c] += gI;
+        cJ[c] -= gJ;
+        cJ[3 + c] += gJ;
     }
 }
 
@@ -339,9 +360,9 @@ __global__ __launch_bounds__(256) void k_gradient(EdgeSoA ed, int E, int Epad, i
 #pragma unroll
             for (int c = 0; c < 6; ++c) accJ[c] = __shfl(accJ[c], (lane + 1) & 63, 64);
         }
-        // lane t now holds J-lane (t + nsteps) % 64; realign to J-lane t
-        const int nsteps = s1 - s0 + 1;
-        const int src = (lane - nsteps) & 63;
+        // lane t now holds J-lane (t + s0 + nsteps) % 64 = (t + s1 + 1) % 64;
+        // realign so that lane t holds J-lane t
+        const int src = (lane - s1 - 1) & 63;
 #pragma unroll
         for (int c = 0; c < 6; ++c) accJ[c] = __shfl(accJ[c], src, 64);
         // slot writes: gpart[(t * 6 + c) * Epad + K]

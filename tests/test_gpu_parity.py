@@ -135,6 +135,31 @@ def test_constrained_gradient_goldens(goldens, gname, fname):
     assert res <= 1e-10
 
 
+@pytest.mark.parametrize("V", [128, 256, 1024])
+def test_constrained_gradient_multiblock_vs_oracle(V):
+    """Several 64-row factor blocks (the goldens have V <= 50, one block):
+    g~ and lambda against the oracle's full-saddle LDL^T with the same dE."""
+    v, e = curves.torus_knot(3, 5, V, jitter=0.02, seed=1)
+    x0 = rco.barycenter_target(v, e)
+    L0 = rco.total_length(v, e)
+    pins = curves.seeded_pins(V, 2, seed=1)
+    cs = [core.barycenter_block(x0), core.total_length_block(L0)]
+    ocs = rco.ConstraintSet().barycenter(x0).total_length(L0)
+    for p in pins:
+        cs.append(core.point_block(int(p), v[p]))
+        ocs.point(int(p), v[p])
+    dE = rco.gradient(v, e)
+    rc, gto, lamo, reso = rco.solve_constrained_gradient(v, e, dE, ocs)
+    assert rc == 0
+    for prec in ("fp64", "mixed"):
+        gt, lam, res = core.solve_constrained_gradient_set_frozen(v, e, None, 3.0, 6.0, 1e-10, dE,
+                                                                 cs, precision=prec)
+        log(V, prec, rel(gt, gto), rel(lam, lamo), res, reso)
+        assert rel(gt, gto) <= 1e-9
+        assert rel(lam, lamo) <= 1e-8
+        assert res <= 1e-10
+
+
 def pen_objects(gd):
     p = gd.get("penalties")
     if not p:
