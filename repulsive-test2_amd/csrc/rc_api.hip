@@ -89,7 +89,7 @@ static void free_all(rc_ctx* ctx) {
                     ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_interior, ctx->d_pos,
                     ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
                     ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_A,
-                    ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv32, ctx->d_dscale,
+                    ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
                     ctx->d_flags, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
                     ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
@@ -209,7 +209,8 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ctx->npairs = ctx->ntiles * (ctx->ntiles + 1) / 2;
     ctx->nvt = (V + RC_VTILE - 1) / RC_VTILE;
     ctx->ngram = ctx->nvt * (ctx->nvt + 1) / 2;
-    ctx->nblk = (V + RC_NB - 1) / RC_NB;
+    // an even number of 64-row blocks: the FP64 factor works on 128-column panels
+    ctx->nblk = 2 * ((V + 2 * RC_NB - 1) / (2 * RC_NB));
     auto fail = [&](rc_status s) {
         *out = ctx;  // keep the context so rc_last_error can report; caller destroys
         return s;
@@ -251,6 +252,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_A, (size_t)V * V);
     ALLOC(ctx->d_F, (size_t)Vp * Vp);
     ALLOC(ctx->d_Linv, (size_t)ctx->nblk * RC_NB * RC_NB);
+    ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, ctx->nblk);
     ALLOC(ctx->d_status, 4);
     ctx->nrhs_cap = 3 + 3 * RC_KMAX;

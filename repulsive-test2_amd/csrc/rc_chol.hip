@@ -7,23 +7,28 @@
 // right-hand sides plus a (k+3)^2 system (see rc_step.hip). Only the V x V
 // Cholesky factor is formed: V^3/3 flops instead of (3V+k)^3/3.
 //
-// Storage: F is Vp x Vp row-major (Vp = nblk * 64), lower triangle, element
-// type T: double for the plain form, float for the mixed form (which factors
-// the diagonally scaled S = D^-1/2 A_hat D^-1/2, unit diagonal). Padding
-// rows/columns are the identity. Blocked right-looking Cholesky, 64 columns
-// per panel:
-//   k_potrf_panel<T>  every block factors the 64x64 diagonal block in FP64
-//                     in LDS and inverts it; block 0 stores L_kk and
-//                     L_kk^-1, the others form their rows L_rk = A_rk L_kk^-T
-//   k_syrk_f64        trailing update A_ij -= L_ik L_jk^T, 64x64 lower tiles,
-//                     v_mfma_f64_16x16x4_f64
-//   k_syrk_f16x3      same update for the FP32 factor: each operand is split
-//                     into fp16 hi + lo (x = hi + lo to ~2^-22) and
-//                     hi*hi^T + hi*lo^T + lo*hi^T runs on
-//                     v_mfma_f32_16x16x32_f16 with FP32 accumulation
-// Triangular solves: one block per 64-row block, dependency flags in global
-// memory (sync-free substitution, FP64 arithmetic on the stored factor); the
-// diagonal solve is a GEMM with the stored L_kk^-1.
+// Storage: F is Vp x Vp row-major (Vp = nblk * 64, nblk even), lower
+// triangle, element type T: double for the plain form, float for the mixed
+// form (which factors the diagonally scaled S = D^-1/2 A_hat D^-1/2, unit
+// diagonal). Padding rows/columns are the identity.
+//
+// FP64 form, right-looking with 128-column panels (3 launches per panel):
+//   k_potrf128    one workgroup factors the 128x128 diagonal block in LDS
+//                 (16-column sub-panels, MFMA updates) and inverts it
+//   k_panel_trsm  L_ik = A_ik L_kk^-T as a GEMM with the stored inverse
+//   k_syrk128     A_ij -= L_ik L_jk^T on 128x128 lower tiles, K = 128,
+//                 v_mfma_f64_16x16x4_f64, XCD-grouped tile order
+// Mixed form, 64-column panels:
+//   k_potrf_panel<float>  every block factors the 64x64 diagonal block in
+//                 FP64 in LDS and inverts it; block 0 stores L_kk^-1, the
+//                 others form their rows L_rk = A_rk L_kk^-T
+//   k_syrk_f16x3  trailing update on the FP32 factor: each operand is split
+//                 into fp16 hi + lo (x = hi + lo to ~2^-22) and
+//                 hi*hi^T + hi*lo^T + lo*hi^T runs on
+//                 v_mfma_f32_16x16x32_f16 with FP32 accumulation
+// Triangular solves (k_trsm): one workgroup per 64-row block, dependency
+// flags in global memory (sync-free substitution), FP64 MFMA products on the
+// stored factor; the diagonal solve is a product with the stored L_kk^-1.
 #include "rc_internal.h"
 #include "rc_math.h"
 
@@ -178,54 +183,314 @@ __device__ __forceinline__ void decode_lower(int p, int& ti, int& tj) {
     tj = p - a * (a + 1) / 2;
 }
 
-// FP64 trailing update on 64x64 lower tiles: C_ij -= P_i P_j^T.
-// 4 waves, each a 32x32 quadrant = 2x2 MFMA 16x16 tiles, K = 64 in steps of 4.
-// v_mfma_f64_16x16x4_f64 layout: A[lane%16][lane/16], B[lane/16][lane%16],
-// D[4*(lane/16)+r][lane%16], r = 0..3.
-__global__ __launch_bounds__(256) void k_syrk_f64(double* __restrict__ F, int Vp, int kb) {
-    __shared__ double Pi[NB][NB + 1];
-    __shared__ double Pj[NB][NB + 1];
-    int ti, tj;
-    decode_lower(blockIdx.x, ti, tj);
-    const int k0 = kb * NB;
-    const int ri = (kb + 1 + ti) * NB, rj = (kb + 1 + tj) * NB;
-    const int t = threadIdx.x;
-    for (int q = t; q < NB * NB; q += 256) {
-        const int i = q / NB, m = q % NB;
-        Pi[i][m] = F[(size_t)(ri + i) * Vp + k0 + m];
-        Pj[i][m] = F[(size_t)(rj + i) * Vp + k0 + m];
+// ---- FP64 form: 128-column panels --------------------------------------
+constexpr int PB = 128;       // panel width of the FP64 factor
+constexpr int KC = 16;        // K chunk staged through LDS by the MFMA tiles
+constexpr int LDT = PB + 4;   // LDS row length of the k-major operand tiles
+
+// Factor the 128x128 diagonal block of panel kb and invert it, in one
+// workgroup of 4 waves, block held in LDS. 16-column sub-panels p = 0..7:
+//   (b) wave 0 factors the 16x16 diagonal block (lane (g, c) = (lane/16,
+//       lane%16) keeps rows 4g..4g+3 of column c in registers; column j is
+//       published through LDS each pivot) and forms its inverse D_p in the
+//       same sweep (right-looking substitution of L_pp X = I, applying row
+//       j-1 of X at pivot j);
+//   (c) L_ip = A_ip D_p^T for the sub-panel rows below (MFMA);
+//   (d) A_ij -= L_ip L_jp^T on the trailing 16x16 lower tiles (MFMA).
+// Then L^-1 is assembled by distance: X_pp = D_p and, for d = 1..7,
+// X_{p+d,p} = -D_{p+d} sum_{k=p}^{p+d-1} L_{p+d,k} X_kp (MFMA), each X_ip
+// kept transposed in the unused upper 16x16 block (p, i) of the LDS matrix.
+// Writes L (lower) into F, L^-1 into Linv128[kb] and the two 64x64 diagonal
+// blocks of L^-1 (the inverses of L's 64x64 diagonal blocks) into
+// Linv64[2kb], [2kb+1].
+// MFMA operands: NT products fetch both operands as P[row = lane%16][k =
+// 4kk + lane/16]; NN products fetch B as Q[4kk + lane/16][lane%16]; result
+// element r of a lane is D[4r + lane/16][lane%16] (this is also the B
+// operand layout of k-step r, which chains two NN products in registers).
+constexpr int SB = 16;
+constexpr int LDA = PB + 4;
+
+__global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp, int kb,
+                                                  double* __restrict__ Linv128,
+                                                  double* __restrict__ Linv64,
+                                                  int32_t* __restrict__ status) {
+    __shared__ double Am[PB][LDA];
+    __shared__ double Dv[PB / SB][SB][SB + 1];
+    __shared__ int bad;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const size_t k0 = (size_t)kb * PB;
+    if (t == 0) bad = 0;
+#pragma unroll 8
+    for (int q = t; q < PB * PB / 2; q += 256) {
+        const int i = q / (PB / 2), m = (q % (PB / 2)) * 2;
+        const double2 v = *reinterpret_cast<const double2*>(F + (k0 + i) * Vp + k0 + m);
+        Am[i][m] = (m <= i) ? v.x : 0.0;
+        Am[i][m + 1] = (m + 1 <= i) ? v.y : 0.0;
     }
     __syncthreads();
-    const int lane = t & 63, wave = t >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    d4 acc[2][2];
+    for (int p = 0; p < PB / SB; ++p) {
+        const int o = p * SB;
+        if (w == 0) {
+            // (b) lane (g, c): rows 4g + rr of column c. Elements above the
+            // diagonal are updated and published too (never read as data),
+            // so the updates need masks only on the column (c > j) and row
+            // (i > j) and no branches.
+            const int g = lk, c = lr;
+            double e[4], x[4], lprev[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                e[rr] = Am[o + 4 * g + rr][o + c];
+                x[rr] = (4 * g + rr == c) ? 1.0 : 0.0;
+                lprev[rr] = 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                double piv = Am[o + j][o + j];
+                double aic[4];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) aic[rr] = Am[o + 4 * g + rr][o + j];
+                const double acj = Am[o + c][o + j];
+                const double xprev = (j > 0) ? Dv[p][j - 1][c] : 0.0;
+                // X rows below j-1 take row j-1 (final since pivot j-1)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) x[rr] = fma(-lprev[rr], xprev, x[rr]);
+                if (!(piv > 0.0)) {
+                    if (lane == 0) bad = 1;
+                    piv = 1.0;
+                }
+                double s, is;
+                rcm::sqrt_rsqrt(piv, s, is);
+                const double f = (c > j) ? acj * (is * is) : 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) e[rr] = fma(-aic[rr], f, e[rr]);
+                // column j is final: L_ij = a_ij / s (the reads above precede
+                // these writes in this wave's LDS order)
+                if (c == j) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        Am[o + 4 * g + rr][o + j] = (4 * g + rr == j) ? s : aic[rr] * is;
+                }
+                if (c == j + 1) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) Am[o + 4 * g + rr][o + c] = e[rr];
+                }
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) lprev[rr] = (4 * g + rr > j) ? aic[rr] * is : 0.0;
+                // row j of X is final: scale by 1/L_jj and publish
+                if (g == (j >> 2)) {
+                    x[j & 3] *= is;
+                    Dv[p][j][c] = x[j & 3];
+                }
+                // lanes exchange columns through LDS: order this pivot's
+                // stores before the next pivot's loads (without the fence the
+                // compiler may hoist the loads of a lane that did not store)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        __syncthreads();
+        // (c) rows below the diagonal block: L_ip = A_ip D_p^T
+        for (int i = p + 1 + w; i < PB / SB; i += 4) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][o + 4 * kk + lk],
+                                                           Dv[p][lr][4 * kk + lk], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][o + lr] = acc[r];
+        }
+        __syncthreads();
+        // (d) trailing tiles (i, j), p < j <= i, numbered row by row
+        const int nt = PB / SB - 1 - p;
+        for (int q = w; q < nt * (nt + 1) / 2; q += 4) {
+            int a, b;
+            decode_lower(q, a, b);
+            const int i = p + 1 + a, j = p + 1 + b;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][o + 4 * kk + lk],
+                                                           Am[16 * j + lr][o + 4 * kk + lk], acc,
+                                                           0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][16 * j + lr] -= acc[r];
+        }
+        __syncthreads();
+    }
+    // X = L^-1 by distance from the diagonal; X_ip (i > p) is stored
+    // transposed in Am's upper block (p, i): Am[16p + b][16i + a] = X_ip[a][b]
+    for (int d = 1; d < PB / SB; ++d) {
+        for (int p = w; p + d < PB / SB; p += 4) {
+            const int i = p + d;
+            d4 sacc = {0.0, 0.0, 0.0, 0.0};
+            for (int k = p; k < i; ++k) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double xb = (k == p) ? Dv[p][4 * kk + lk][lr]
+                                               : Am[16 * p + lr][16 * k + 4 * kk + lk];
+                    sacc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][16 * k + 4 * kk + lk],
+                                                                xb, sacc, 0, 0, 0);
+                }
+            }
+            d4 xacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Dv[i][lr][4 * kk + lk], sacc[kk], xacc,
+                                                            0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Am[16 * p + lr][16 * i + 4 * r + lk] = xacc[r];
+        }
+        __syncthreads();
+    }
+    if (t == 0 && bad) atomicOr(status, 1);
+    double* X = Linv128 + k0 * PB;
+#pragma unroll 8
+    for (int q = t; q < PB * PB; q += 256) {
+        const int i = q / PB, m = q % PB;
+        const int bi = i / SB, bm = m / SB;
+        if (m <= i) F[(k0 + i) * Vp + k0 + m] = Am[i][m];
+        const double v = (bm > bi) ? 0.0
+                         : (bm == bi) ? Dv[bi][i % SB][m % SB]
+                                      : Am[16 * bm + m % SB][16 * bi + i % SB];
+        X[(size_t)i * PB + m] = v;
+        if ((i >> 6) == (m >> 6))
+            Linv64[((size_t)(2 * kb + (i >> 6)) * NB + (i & 63)) * NB + (m & 63)] = v;
+    }
+}
+
+// acc += A B^T for an MT x 128 block of C (MT = 128 or 64), K = 128:
+// A is MT x K and B is 128 x K, both row-major (leading dims lda, ldb).
+// 4 waves in a 2 x 2 grid, each an (MT/2) x 64 sub-tile of (MT/32) x 4
+// v_mfma_f64_16x16x4_f64 tiles. K is staged through LDS in chunks of KC,
+// k-major (so the 16 rows an MFMA operand reads are contiguous), double
+// buffered with the next chunk's global loads in flight during the MFMAs.
+// MFMA operand layout: A[lane%16][lane/16], B[lane/16][lane%16];
+// result element r of a lane is D[4r + lane/16][lane%16].
+template <int MT>
+__device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int lda,
+                                            const double* __restrict__ B, int ldb,
+                                            double (*As)[KC][LDT], double (*Bs)[KC][LDT],
+                                            d4 (&acc)[MT / 32][4]) {
+    constexpr int TA = MT / 32;
+    constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk (8 or 4)
+    constexpr int NBv = PB * KC / 256;  // B doubles per thread per chunk (8)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wr = (w >> 1) * (MT / 2), wc = (w & 1) * 64;
+    const int arow = t / (KC / NA), acol = (t % (KC / NA)) * NA;
+    const int brow = t / (KC / NBv), bcol = (t % (KC / NBv)) * NBv;
+    const double* pa = A + (size_t)arow * lda + acol;
+    const double* pb = B + (size_t)brow * ldb + bcol;
+    double ra[NA], rb[NBv];
+    auto gload = [&](int k) {
+#pragma unroll
+        for (int q = 0; q < NA; q += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(pa + k + q);
+            ra[q] = v.x;
+            ra[q + 1] = v.y;
+        }
+#pragma unroll
+        for (int q = 0; q < NBv; q += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(pb + k + q);
+            rb[q] = v.x;
+            rb[q + 1] = v.y;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) As[buf][acol + q][arow] = ra[q];
+#pragma unroll
+        for (int q = 0; q < NBv; ++q) Bs[buf][bcol + q][brow] = rb[q];
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int ch = 0; ch < PB / KC; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < PB / KC) gload((ch + 1) * KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            double av[TA], bv[4];
+#pragma unroll
+            for (int a = 0; a < TA; ++a) av[a] = As[buf][kk + lk][wr + 16 * a + lr];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[buf][kk + lk][wc + 16 * b + lr];
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        // keep the next chunk's LDS stores (and their wait on the global
+        // loads) after this chunk's MFMAs, so the loads overlap the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+        if (ch + 1 < PB / KC) sstore(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+// Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
+// workgroup, in place.
+__global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int Vp, int kb,
+                                                    const double* __restrict__ Linv128) {
+    __shared__ double As[2][KC][LDT];
+    __shared__ double Bs[2][KC][LDT];
+    const size_t k0 = (size_t)kb * PB;
+    const size_t r0 = k0 + PB + (size_t)blockIdx.x * 64;
+    d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    const int lr = lane & 15, lk = lane >> 4;
-    for (int k = 0; k < NB; k += 4) {
-        double av[2], bv[2];
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    gemm_nt_f64<64>(F + r0 * Vp + k0, Vp, Linv128 + k0 * PB, PB, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) av[a] = Pi[wr + 16 * a + lr][k + lk];
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) bv[b] = Pj[wc + 16 * b + lr][k + lk];
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int r = 0; r < 4; ++r)
+                F[(r0 + wr + 16 * a + 4 * r + (lane >> 4)) * Vp + k0 + wc + 16 * b + (lane & 15)] =
+                    acc[a][b][r];
+}
+
+// Trailing update A_ij -= L_ik L_jk^T over the 128x128 lower tiles of the
+// trailing matrix (ntiles = nt (nt+1)/2 tiles, nt trailing panels). Tiles are
+// numbered row by row; workgroup b runs on XCD b % 8, so each XCD is given a
+// contiguous range of tile numbers (tiles of one row share their A panel in
+// that XCD's L2).
+__global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
+                                                 int ntiles) {
+    __shared__ double As[2][KC][LDT];
+    __shared__ double Bs[2][KC][LDT];
+    const int per = (ntiles + 7) >> 3;
+    const int p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (p >= ntiles) return;
+    int ti, tj;
+    decode_lower(p, ti, tj);
+    const size_t k0 = (size_t)kb * PB;
+    const size_t ri = k0 + PB + (size_t)ti * PB, rj = k0 + PB + (size_t)tj * PB;
+    d4 acc[4][4];
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    gemm_nt_f64<128>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
     const bool diag = (ti == tj);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = wr + 16 * a + 4 * lk + r;
-                const int col = wc + 16 * b + lr;
-                if (!diag || col <= row) F[(size_t)(ri + row) * Vp + rj + col] -= acc[a][b][r];
+                const int i = wr + 16 * a + 4 * r + (lane >> 4);
+                const int j = wc + 16 * b + (lane & 15);
+                if (!diag || j <= i) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
             }
 }
 
@@ -295,19 +560,18 @@ __global__ __launch_bounds__(256) void k_syrk_f16x3(float* __restrict__ F, int V
 __device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t epoch, int32_t* status) {
     if (threadIdx.x == 0) {
         int spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        // poll with relaxed loads (an acquire load would invalidate the L2
+        // on every spin), then one acquire fence once the flag is seen
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             __builtin_amdgcn_s_sleep(2);
             if (++spins > (1 << 24)) {
                 atomicOr(status, 2);
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-}
-
-__device__ __forceinline__ double coherent_load(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void publish(int32_t* flag, int32_t epoch) {
@@ -316,137 +580,151 @@ __device__ __forceinline__ void publish(int32_t* flag, int32_t epoch) {
     if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Forward substitution L Y = B on nrhs <= RHS_CHUNK columns (column-major,
-// leading dimension ldb), in place. Block r handles rows [64r, 64r+64).
-// skip (nullable): when *skip != 0 every block returns at once.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_fwd(const T* __restrict__ F, int Vp,
-                                                  const T* __restrict__ Linv,
-                                                  double* __restrict__ B, int ldb, int nrhs,
-                                                  int32_t* __restrict__ flags, int32_t epoch,
-                                                  int32_t* __restrict__ status,
-                                                  const int32_t* __restrict__ skip) {
+// Sync-free blocked substitution on nrhs <= RHS_CHUNK columns of B
+// (column-major, leading dimension ldb), in place. FWD: L Y = B, block b
+// solves row block r = b after waiting for blocks 0..r-1. Backward:
+// L^T X = Y, block b solves row block r = nblk-1-b after blocks r+1..nblk-1.
+// Each block keeps its 64 x RHS_CHUNK accumulator in MFMA registers (wave w:
+// rows 16w..16w+15, two 16-column tiles) and subtracts L_{r,cb} Y_cb (or
+// L_{cb,r}^T X_cb) with v_mfma_f64_16x16x4_f64 as each dependency is
+// published; the next factor tile is prefetched into registers meanwhile
+// (the factor does not depend on the flags). The diagonal solve is the
+// product with the stored L_rr^-1 (or its transpose). skip (nullable): when
+// *skip != 0 every block returns at once.
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
+                                              const T* __restrict__ Linv,
+                                              double* B, int ldb, int nrhs, int nblk,
+                                              int32_t* __restrict__ flags, int32_t epoch,
+                                              int32_t* __restrict__ status,
+                                              const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double acc[NB][RHS_CHUNK + 1];
-    __shared__ double Lt[NB][NB + 1];
-    __shared__ double Yc[NB][RHS_CHUNK + 1];
-    const int r = blockIdx.x;
-    const int t = threadIdx.x;
-    for (int q = t; q < NB * nrhs; q += 256) {
-        const int i = q % NB, c = q / NB;
-        acc[i][c] = B[(size_t)c * ldb + r * NB + i];
-    }
-    const int ti = t >> 2;
-    const int tc = t & 3;
-    for (int cb = 0; cb < r; ++cb) {
-        wait_flag(flags + cb, epoch, status);
-        for (int q = t; q < NB * NB; q += 256) {
-            const int i = q / NB, m = q % NB;
-            Lt[i][m] = (double)F[(size_t)(r * NB + i) * Vp + cb * NB + m];
+    __shared__ double Lt[NB][NB + 1];          // MFMA A operand, Lt[i][m]
+    __shared__ double Ys[NB][RHS_CHUNK + 1];   // MFMA B operand, Ys[m][c]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int r = FWD ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    const size_t rb = (size_t)r * NB;
+    d4 acc[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
+            acc[b][q] = (c < nrhs) ? B[(size_t)c * ldb + rb + i] : 0.0;
         }
-        for (int q = t; q < NB * nrhs; q += 256) {
-            const int m = q % NB, c = q / NB;
-            Yc[m][c] = coherent_load(B + (size_t)c * ldb + cb * NB + m);
+    // factor tile rows (t >> 2), 16 consecutive columns from (t & 3) * 16
+    const int tr = t >> 2, tc = (t & 3) * 16;
+    double v[16];
+    auto gload = [&](int cb) {
+        const T* p = FWD ? F + (rb + tr) * Vp + (size_t)cb * NB + tc
+                         : F + ((size_t)cb * NB + tr) * Vp + rb + tc;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = (double)p[q];
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (FWD) Lt[tr][tc + q] = v[q];
+            else Lt[tc + q][tr] = v[q];
         }
+    };
+    auto mfma_tile = [&](d4 (&d)[2], bool neg) {
+#pragma unroll 4
+        for (int k = 0; k < NB; k += 4) {
+            const double a = neg ? -Lt[16 * w + lr][k + lk] : Lt[16 * w + lr][k + lk];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                d[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ys[k + lk][16 * b + lr], d[b], 0, 0, 0);
+        }
+    };
+    const int ndep = FWD ? r : nblk - 1 - r;
+    if (ndep > 0) gload(FWD ? 0 : nblk - 1);
+    for (int s = 0; s < ndep; ++s) {
+        const int cb = FWD ? s : nblk - 1 - s;
+        wait_flag(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
+#pragma unroll
+        for (int q = 0; q < NB * RHS_CHUNK / 256; ++q) {
+            const int e = t + 256 * q, m = e % NB, c = e / NB;
+            // plain loads: thread 0's acquire fence in wait_flag and the
+            // barrier after it order them after block cb's release
+            // (columns >= nrhs read column nrhs-1, in bounds, and store 0:
+            // no branch around the load, so the loads are issued together)
+            const int cc = c < nrhs ? c : nrhs - 1;
+            const double y = B[(size_t)cc * ldb + (size_t)cb * NB + m];
+            Ys[m][c] = (c < nrhs) ? y : 0.0;
+        }
+        sstore();
+        if (s + 1 < ndep) gload(FWD ? s + 1 : nblk - 2 - s);
         __syncthreads();
-        for (int c = tc; c < nrhs; c += 4) {
-            double s = 0.0;
-            for (int m = 0; m < NB; ++m) s = fma(Lt[ti][m], Yc[m][c], s);
-            acc[ti][c] -= s;
+        mfma_tile(acc, true);
+    }
+    __syncthreads();
+    // diagonal block: X_r = L_rr^-1 acc (forward) or L_rr^-T acc (backward)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ys[16 * w + 4 * q + lk][16 * b + lr] = acc[b][q];
+    const T* Li = Linv + rb * NB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const double x = (double)Li[(size_t)tr * NB + tc + q];
+        if (FWD) Lt[tr][tc + q] = x;
+        else Lt[tc + q][tr] = x;
+    }
+    __syncthreads();
+    d4 xo[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) xo[b] = (d4){0.0, 0.0, 0.0, 0.0};
+    mfma_tile(xo, false);
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
+            if (c < nrhs) B[(size_t)c * ldb + rb + i] = xo[b][q];
         }
-        __syncthreads();
-    }
-    const T* Li = Linv + (size_t)r * NB * NB;
-    for (int q = t; q < NB * NB; q += 256) Lt[q / NB][q % NB] = (double)Li[q];
-    __syncthreads();
-    for (int c = tc; c < nrhs; c += 4) {
-        double s = 0.0;
-        for (int m = 0; m <= ti; ++m) s = fma(Lt[ti][m], acc[m][c], s);
-        Yc[ti][c] = s;
-    }
-    __syncthreads();
-    for (int q = t; q < NB * nrhs; q += 256) {
-        const int i = q % NB, c = q / NB;
-        B[(size_t)c * ldb + r * NB + i] = Yc[i][c];
-    }
     publish(flags + r, epoch);
 }
 
-// Backward substitution L^T X = Y in place. Block b handles block row
-// r = nblk - 1 - b so that every block only waits on lower-indexed blocks.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_bwd(const T* __restrict__ F, int Vp,
-                                                  const T* __restrict__ Linv,
-                                                  double* __restrict__ B, int ldb, int nrhs,
-                                                  int nblk, int32_t* __restrict__ flags,
-                                                  int32_t epoch, int32_t* __restrict__ status,
-                                                  const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    __shared__ double acc[NB][RHS_CHUNK + 1];
-    __shared__ double Lt[NB][NB + 1];
-    __shared__ double Xc[NB][RHS_CHUNK + 1];
-    const int r = nblk - 1 - (int)blockIdx.x;
-    const int t = threadIdx.x;
-    for (int q = t; q < NB * nrhs; q += 256) {
-        const int i = q % NB, c = q / NB;
-        acc[i][c] = B[(size_t)c * ldb + r * NB + i];
-    }
-    const int ti = t >> 2;
-    const int tc = t & 3;
-    for (int cb = nblk - 1; cb > r; --cb) {
-        wait_flag(flags + cb, epoch, status);
-        for (int q = t; q < NB * NB; q += 256) {
-            const int m = q / NB, i = q % NB;
-            Lt[i][m] = (double)F[(size_t)(cb * NB + m) * Vp + r * NB + i];
-        }
-        for (int q = t; q < NB * nrhs; q += 256) {
-            const int m = q % NB, c = q / NB;
-            Xc[m][c] = coherent_load(B + (size_t)c * ldb + cb * NB + m);
-        }
-        __syncthreads();
-        for (int c = tc; c < nrhs; c += 4) {
-            double s = 0.0;
-            for (int m = 0; m < NB; ++m) s = fma(Lt[ti][m], Xc[m][c], s);
-            acc[ti][c] -= s;
-        }
-        __syncthreads();
-    }
-    const T* Li = Linv + (size_t)r * NB * NB;
-    for (int q = t; q < NB * NB; q += 256) Lt[q % NB][q / NB] = (double)Li[q];  // Lt = Linv^T
-    __syncthreads();
-    for (int c = tc; c < nrhs; c += 4) {
-        double s = 0.0;
-        for (int m = ti; m < NB; ++m) s = fma(Lt[ti][m], acc[m][c], s);
-        Xc[ti][c] = s;
-    }
-    __syncthreads();
-    for (int q = t; q < NB * nrhs; q += 256) {
-        const int i = q % NB, c = q / NB;
-        B[(size_t)c * ldb + r * NB + i] = Xc[i][c];
-    }
-    publish(flags + r, epoch);
-}
-
-template <typename T>
-rc_status factor_loop(rc_ctx* ctx, T* F, T* Linv) {
+// Mixed form: 64-column panels on the FP32 factor.
+rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
     const int Vp = ctx->nblk * NB;
     for (int kb = 0; kb < ctx->nblk; ++kb) {
         const int nrow = ctx->nblk - kb;
-        // a breakdown of the FP32 factor only weakens the preconditioner (bit 8);
-        // in the FP64 form it means the saddle system is singular (bit 1)
-        hipLaunchKernelGGL(k_potrf_panel<T>, dim3(nrow), dim3(256), 0, ctx->stream, F, Vp, kb,
-                           Linv, ctx->d_status, sizeof(T) == 8 ? 1 : 8);
+        // a breakdown of the FP32 factor only weakens the preconditioner (bit 8)
+        hipLaunchKernelGGL(k_potrf_panel<float>, dim3(nrow), dim3(256), 0, ctx->stream, F, Vp, kb,
+                           Linv, ctx->d_status, 8);
         RC_CHECK_LAUNCH();
         const int ntr = nrow - 1;
         if (ntr > 0) {
-            if constexpr (sizeof(T) == 8)
-                hipLaunchKernelGGL(k_syrk_f64, dim3(ntr * (ntr + 1) / 2), dim3(256), 0,
-                                   ctx->stream, F, Vp, kb);
-            else
-                hipLaunchKernelGGL(k_syrk_f16x3, dim3(ntr * (ntr + 1) / 2), dim3(256), 0,
-                                   ctx->stream, F, Vp, kb);
+            hipLaunchKernelGGL(k_syrk_f16x3, dim3(ntr * (ntr + 1) / 2), dim3(256), 0,
+                               ctx->stream, F, Vp, kb);
             RC_CHECK_LAUNCH();
         }
+    }
+    return RC_OK;
+}
+
+// FP64 form: 128-column panels (diagonal factor + inverse, panel TRSM as a
+// GEMM with the inverse, MFMA trailing update). A non-positive pivot sets
+// status bit 1 (singular saddle system).
+rc_status factor_loop_fp64(rc_ctx* ctx) {
+    const int Vp = ctx->nblk * NB;
+    const int np = Vp / PB;
+    for (int kb = 0; kb < np; ++kb) {
+        hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
+                           ctx->d_Linv128, ctx->d_Linv, ctx->d_status);
+        RC_CHECK_LAUNCH();
+        const int nt = np - kb - 1;
+        if (nt == 0) break;
+        hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
+                           (const double*)ctx->d_Linv128);
+        RC_CHECK_LAUNCH();
+        const int ntiles = nt * (nt + 1) / 2;
+        hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream,
+                           ctx->d_F, Vp, kb, ntiles);
+        RC_CHECK_LAUNCH();
     }
     return RC_OK;
 }
@@ -464,7 +742,7 @@ rc_status rc_factor_fp64(rc_ctx* ctx) {
     hipLaunchKernelGGL(k_factor_prep<double>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
                        ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, ctx->d_F);
     RC_CHECK_LAUNCH();
-    return factor_loop<double>(ctx, ctx->d_F, ctx->d_Linv);
+    return factor_loop_fp64(ctx);
 }
 
 // Factor S = D^-1/2 A_hat D^-1/2 into ctx->d_F32 (FP32 storage, FP16x3 MFMA
@@ -487,7 +765,7 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
                        ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)ctx->d_dscale,
                        ctx->d_F32);
     RC_CHECK_LAUNCH();
-    return factor_loop<float>(ctx, ctx->d_F32, ctx->d_Linv32);
+    return factor_loop_mixed(ctx, ctx->d_F32, ctx->d_Linv32);
 }
 
 template <typename T>
@@ -498,12 +776,12 @@ static rc_status solve_impl(rc_ctx* ctx, const T* F, const T* Linv, double* d_B,
         const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
         double* Bc = d_B + (size_t)c0 * Vp;
         const int32_t ef = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_fwd<T>, dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp, Linv,
-                           Bc, Vp, nc, ctx->d_flags, ef, ctx->d_status, skip);
+        hipLaunchKernelGGL((k_trsm<T, true>), dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp,
+                           Linv, Bc, Vp, nc, ctx->nblk, ctx->d_flags, ef, ctx->d_status, skip);
         RC_CHECK_LAUNCH();
         const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_bwd<T>, dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp, Linv,
-                           Bc, Vp, nc, ctx->nblk, ctx->d_flags, eb, ctx->d_status, skip);
+        hipLaunchKernelGGL((k_trsm<T, false>), dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp,
+                           Linv, Bc, Vp, nc, ctx->nblk, ctx->d_flags, eb, ctx->d_status, skip);
         RC_CHECK_LAUNCH();
     }
     return RC_OK;
@@ -525,13 +803,13 @@ int rc_trsm_max_coresident(void) {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsm_fwd<double>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsm<double, true>, 256, 0) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsm_bwd<double>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsm<double, false>, 256, 0) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, k_trsm_fwd<float>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, k_trsm<float, true>, 256, 0) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_trsm_bwd<float>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_trsm<float, false>, 256, 0) != hipSuccess)
         return 0;
     int m = per;
     if (per2 < m) m = per2;

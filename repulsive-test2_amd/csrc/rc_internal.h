@@ -34,7 +34,7 @@ struct LSCtl;
 struct rc_ctx {
     int V = 0, E = 0, Epad = 0, ntiles = 0, npairs = 0;
     int nvt = 0, ngram = 0;   // vertex tiles, assembly tile pairs
-    int nblk = 0;             // Cholesky blocks of RC_NB rows
+    int nblk = 0;             // Cholesky blocks of RC_NB rows (always even)
     int maxdeg = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -73,6 +73,7 @@ struct rc_ctx {
     double* d_F = nullptr;        // V x V FP64 factor (lower) of A + mu 11^T
     float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
     double* d_Linv = nullptr;     // inverted diagonal blocks [nblk][NB][NB]
+    double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]
     float* d_Linv32 = nullptr;
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
     int32_t* d_flags = nullptr;   // triangular-solve block flags
