@@ -12,10 +12,14 @@ search with back-projection), continuing the flow from the previous step.
 Timed: exactly --steps steps after --warmup untimed steps, each ending in the
 library's stream synchronisation (the single host round trip of a step).
 
-Prints one JSON line. value = steps/s of the chosen form (mixed when built);
-the plain FP64 form is measured in the same run for the speedup and the
-accuracy gates (energy and direction relative error of the mixed form
-against FP64 at the same curve).
+Prints one JSON line. value = steps/s of the headline form: the mixed form
+when every timed step is accepted and it passes the accuracy gates (energy
+relative error <= 1e-10 and direction relative error <= 1e-8 against the FP64
+form at the same curve), else the plain FP64 form; both are reported.
+roofline: the dominant kernel of the FP64 step, the trailing update
+k_syrk128, timed with HIP events around each of its launches on the
+library's stream during the timed steps; achieved = algorithmic flops
+(128 m (m + 1) per launch, m trailing rows) / summed launch time.
 """
 from __future__ import annotations
 
@@ -53,7 +57,9 @@ def make_workload(V):
     return v, e, cs, pen
 
 
-def run_form(ctx, v0, cs, pen, precision, steps, warmup):
+def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
+    """Warm up, then time exactly `steps` resident steps from v0. With kprof,
+    HIP events bracket every trailing-update launch of the timed steps."""
     from rcurves import _lib as L
     from rcurves import core
     arr, nb, keep = core._blocks_array(cs)
@@ -64,7 +70,7 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup):
     tm = L.Timings()
     e_prev = None
     phase = {k: 0.0 for k, _ in L.Timings._fields_}
-    trials = []
+    trials, accepted = [], []
 
     def one(record):
         nonlocal e_prev
@@ -78,19 +84,29 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup):
             for k, _ in L.Timings._fields_:
                 phase[k] += getattr(tm, k)
             trials.append(st.trials)
+            accepted.append(st.accepted)
 
     for i in range(warmup):
         one(False)
         log(f"[{precision}] warmup {i}: accepted={st.accepted} tau={st.tau} E={st.energy:.12g} "
             f"trials={st.trials} refine={st.refine_iters}")
+    prof = None
+    if kprof:
+        ctx.check(L.lib().rc_kernel_profile_enable(ctx.handle, 1))
     t0 = time.perf_counter()
     for i in range(steps):
         one(True)
     dt = time.perf_counter() - t0
+    if kprof:
+        kp = L.KernelProfile()
+        ctx.check(L.lib().rc_kernel_profile_read(ctx.handle, C.byref(kp)))
+        ctx.check(L.lib().rc_kernel_profile_enable(ctx.handle, 0))
+        prof = {"ms": kp.syrk_ms, "flop": kp.syrk_flop, "launches": kp.syrk_launches}
     log(f"[{precision}] {steps} steps in {dt:.4f}s; last accepted={st.accepted} tau={st.tau} "
         f"E={st.energy:.12g} trials={trials}")
     del keep
-    return dt / steps, {k: v / steps for k, v in phase.items()}, trials
+    return {"t": dt / steps, "phase": {k: v / steps for k, v in phase.items()}, "trials": trials,
+            "accepted": accepted, "kprof": prof}
 
 
 def accuracy(v, e, cs, pen):
@@ -116,35 +132,65 @@ def accuracy(v, e, cs, pen):
 
 def cpu_baseline():
     """The oracle (FP64 C restatement of the reference, 1 thread) on a bounded
-    sample, scaled to E = 8192: energy, dE and assembly measured at E = 2048
-    and scaled by (8192/2048)^2; the reference's saddle LDL^T factor measured
-    at V = 1024 (n = 3V+4 = 3076) and scaled by (24580/3076)^3."""
+    sample, scaled to E = 8192: energy, dE and assembly measured at E = 4096
+    and scaled by (8192/4096)^2; the reference's saddle LDL^T (the structure
+    of its linsolve, n = 3V + k) measured at V = 1536 (n = 4612) and scaled by
+    (24580/4612)^3."""
     from oracle import rc_oracle as rco
     from rcurves import curves
-    v, e = curves.fourier_curve(2048, seed=1)
+    v, e = curves.fourier_curve(4096, seed=1)
     t = time.perf_counter(); rco.energy(v, e); t_en = time.perf_counter() - t
     t = time.perf_counter(); rco.gradient(v, e); t_de = time.perf_counter() - t
     t = time.perf_counter(); rco.assemble("A", v, e); t_as = time.perf_counter() - t
-    v1, e1 = curves.fourier_curve(1024, seed=1)
+    Vs = 1536
+    v1, e1 = curves.fourier_curve(Vs, seed=1)
     A = rco.assemble("A", v1, e1)
-    n = 3 * 1024 + 4
+    n = 3 * Vs + 4
     K = np.zeros((n, n))
     for b in range(3):
-        K[b * 1024:(b + 1) * 1024, b * 1024:(b + 1) * 1024] = A
+        K[b * Vs:(b + 1) * Vs, b * Vs:(b + 1) * Vs] = A
     rng = np.random.default_rng(0)
-    Cm = rng.standard_normal((4, 3 * 1024))
-    K[3072:, :3072] = Cm
-    K[:3072, 3072:] = Cm.T
+    Cm = rng.standard_normal((4, 3 * Vs))
+    K[3 * Vs:, :3 * Vs] = Cm
+    K[:3 * Vs, 3 * Vs:] = Cm.T
     t = time.perf_counter(); rco.ldlt(K); t_fa = time.perf_counter() - t
-    s2 = (8192 / 2048) ** 2
+    s2 = (8192 / 4096) ** 2
     s3 = (3 * 8192 + 4) ** 3 / n ** 3
     step = (t_en + t_de + t_as) * s2 + t_fa * s3
     return {
         "value": 1.0 / step, "unit": "steps/s", "cores": 1, "kind": "port",
         "sample": (f"oracle C FP64, 1 thread: energy {t_en:.2f}s + dE {t_de:.2f}s + assembleA "
-                   f"{t_as:.2f}s at E=2048 (x{s2:.0f}), reference-structure LDL^T {t_fa:.2f}s at "
+                   f"{t_as:.2f}s at E=4096 (x{s2:.0f}), reference-structure LDL^T {t_fa:.2f}s at "
                    f"n={n} (x{s3:.0f}); extrapolated step {step:.0f}s at E=8192"),
     }
+
+
+def measured_traffic():
+    """HBM bytes per launch of k_syrk128 from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "k_syrk128_fourier_v8192_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch k_syrk128:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib):
+    """One device-to-device hipMemcpy of `mib` MiB (a plain copy kernel of
+    known traffic, for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def main():
@@ -155,52 +201,58 @@ def main():
     ap.add_argument("--V", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="also run one device-to-device copy of this size (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the Sobolev step has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+              "(DESIGN.md describes how it would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from rcurves import _lib as L
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     log("building workload")
     v, e, cs, pen = make_workload(args.V)
     ctx = L.Context(e, len(v))
-    t64, ph64, tr64 = run_form(ctx, v, cs, pen, "fp64", args.steps, args.warmup)
-    mixed_ok = False
-    acc = {}
+    r64 = run_form(ctx, v, cs, pen, "fp64", args.steps, args.warmup, kprof=True)
+    rmx, acc = None, {}
     if not args.fp64_only:
         try:
-            tmx, phmx, trmx = run_form(ctx, v, cs, pen, "mixed", args.steps, args.warmup)
-            mixed_ok = True
+            rmx = run_form(ctx, v, cs, pen, "mixed", args.steps, args.warmup)
             acc = accuracy(v, e, cs, pen)
         except L.RcError as err:
             log("mixed form unavailable:", err)
-    if mixed_ok:
-        t_sel, ph_sel, prec = tmx, phmx, "mixed"
-    else:
-        t_sel, ph_sel, prec = t64, ph64, "fp64"
+    # headline form: mixed when it passes the accuracy gates, else FP64
+    mixed_ok = (rmx is not None and all(rmx["accepted"]) and "energy_rel_err" in acc
+                and acc["energy_rel_err"] <= 1e-10 and acc["direction_rel_err"] <= 1e-8)
+    sel, prec = (rmx, "mixed") if mixed_ok else (r64, "fp64")
     V = args.V
-    # dominant phase of the headline form and its roofline
-    fac_flops = V ** 3 / 3.0
-    fac_ms = ph64["factor"]
+    kp = r64["kprof"]
+    achieved = kp["flop"] / (kp["ms"] * 1e-3) / 1e12 if kp and kp["ms"] > 0 else None
     roof = {
-        "bound": "mfma", "kernel": "FP64 blocked Cholesky of A + mu 11^T (k_potrf_panel + "
-                                   "k_syrk_mfma), fp64 form",
-        "achieved": fac_flops / (fac_ms * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
-        "unit": "TFLOP/s", "traffic": None,
+        "bound": "mfma", "kernel": "k_syrk128 (FP64 trailing update of the Cholesky factor, "
+                                   "v_mfma_f64_16x16x4_f64), fp64 form",
+        "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": (achieved / PEAK_FP64_TFLOPS) if achieved else None,
+        "traffic": measured_traffic() if V == 8192 else None,
+        "launches": kp["launches"] if kp else 0,
+        "avg_launch_us": (kp["ms"] * 1e3 / kp["launches"]) if kp and kp["launches"] else None,
     }
-    roof["frac"] = roof["achieved"] / roof["peak"]
     out = {
-        "metric": METRIC, "value": 1.0 / t_sel, "unit": "steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_sel * 1e3,
+        "metric": METRIC, "value": 1.0 / sel["t"], "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sel["t"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == "fp64" else "f64 pairs/refinement + f32/f16-mfma factor",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "V": V, "E": V, "form": prec,
-                   "fp64_steps_per_s": 1.0 / t64,
-                   "speedup_vs_fp64": (t64 / t_sel) if mixed_ok else 1.0,
-                   "phase_ms": {k: round(x, 4) for k, x in ph_sel.items()},
-                   "phase_ms_fp64": {k: round(x, 4) for k, x in ph64.items()},
-                   "trials_per_step": tr64, **acc},
+                   "fp64_steps_per_s": 1.0 / r64["t"],
+                   "mixed_steps_per_s": (1.0 / rmx["t"]) if rmx else None,
+                   "mixed_accepted": rmx["accepted"] if rmx else None,
+                   "mixed_passes_gates": mixed_ok,
+                   "speedup_mixed_vs_fp64": (r64["t"] / rmx["t"]) if rmx else None,
+                   "phase_ms": {k: round(x, 4) for k, x in sel["phase"].items()},
+                   "phase_ms_fp64": {k: round(x, 4) for k, x in r64["phase"].items()},
+                   "trials_per_step": r64["trials"], **acc},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -161,6 +161,22 @@ rc_status rc_sobolev_step_resident(rc_ctx* ctx, const rc_block* blocks, int32_t 
                                    const rc_penalties* pen, const rc_step_opts* o,
                                    rc_step_stats* stats, rc_timings* timings);
 
+/* Kernel-level profile of the dominant kernel of the FP64 step, the
+ * trailing update of the Cholesky factorisation (k_syrk128): while enabled,
+ * a pair of HIP events brackets each of its launches on the context's stream.
+ * No counterpart in the reference (measurement support for benchmarks).
+ * Enabling (or disabling) resets the totals; reading synchronises the
+ * context's stream and returns the totals since the reset. */
+typedef struct {
+    double syrk_ms;         /* summed launch durations */
+    double syrk_flop;       /* algorithmic flops: 128 m (m + 1) per launch, m trailing rows */
+    int32_t syrk_launches;
+    int32_t _pad;
+} rc_kernel_profile;
+
+rc_status rc_kernel_profile_enable(rc_ctx* ctx, int32_t enable);
+rc_status rc_kernel_profile_read(rc_ctx* ctx, rc_kernel_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,8 @@ static void free_all(rc_ctx* ctx) {
 
 extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     free_all(ctx);
     delete ctx;
 }
@@ -650,7 +652,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     const bool tm = o->collect_timings && timings;
     hipEvent_t ev[10] = {};
     if (tm)
-        for (auto& e : ev) RC_HIP(hipEventCreate(&e));
+        for (auto& e : ev) RC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     if (tm) RC_HIP(hipEventRecord(ev[0], ctx->stream));
 
     s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
@@ -798,4 +800,44 @@ extern "C" rc_status rc_sobolev_step(rc_ctx* ctx, const double* xyz_in, const rc
     rc_status s = rc_upload_positions_aos(ctx, xyz_in, ctx->d_pos);
     if (s != RC_OK) return s;
     return step_impl(ctx, blocks, nblocks, pen, o, stats, timings, gtilde_out, 0, xyz_out);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel-level profile of the trailing update (measurement support).
+
+static rc_status kprof_collect(rc_ctx* ctx) {
+    if (ctx->kev_used == 0) return RC_OK;
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i + 1 < ctx->kev_used; i += 2) {
+        float ms = 0.f;
+        RC_HIP(hipEventElapsedTime(&ms, ctx->kev[i], ctx->kev[i + 1]));
+        ctx->kprof_ms += ms;
+        ctx->kprof_flop += ctx->kev_flop[i / 2];
+        ++ctx->kprof_launches;
+    }
+    ctx->kev_used = 0;
+    ctx->kev_flop.clear();
+    return RC_OK;
+}
+
+extern "C" rc_status rc_kernel_profile_enable(rc_ctx* ctx, int32_t enable) {
+    if (!ctx) return RC_BAD_ARG;
+    rc_status s = kprof_collect(ctx);
+    if (s != RC_OK) return s;
+    ctx->kprof = enable ? 1 : 0;
+    ctx->kprof_ms = 0.0;
+    ctx->kprof_flop = 0.0;
+    ctx->kprof_launches = 0;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_kernel_profile_read(rc_ctx* ctx, rc_kernel_profile* out) {
+    if (!ctx || !out) return RC_BAD_ARG;
+    rc_status s = kprof_collect(ctx);
+    if (s != RC_OK) return s;
+    out->syrk_ms = ctx->kprof_ms;
+    out->syrk_flop = ctx->kprof_flop;
+    out->syrk_launches = ctx->kprof_launches;
+    out->_pad = 0;
+    return RC_OK;
 }

@@ -185,16 +185,17 @@ __device__ __forceinline__ void decode_lower(int p, int& ti, int& tj) {
 
 // ---- FP64 form: 128-column panels --------------------------------------
 constexpr int PB = 128;       // panel width of the FP64 factor
-constexpr int KC = 16;        // K chunk staged through LDS by the MFMA tiles
+constexpr int KC_TRSM = 16;   // K chunk staged through LDS: panel TRSM and trailing update
+constexpr int KC_SYRK = 16;   // (2 workgroups/CU; a 32 chunk at 1 workgroup/CU measured slower)
 constexpr int LDT = PB + 4;   // LDS row length of the k-major operand tiles
 
 // Factor the 128x128 diagonal block of panel kb and invert it, in one
 // workgroup of 4 waves, block held in LDS. 16-column sub-panels p = 0..7:
-//   (b) wave 0 factors the 16x16 diagonal block (lane (g, c) = (lane/16,
-//       lane%16) keeps rows 4g..4g+3 of column c in registers; column j is
-//       published through LDS each pivot) and forms its inverse D_p in the
-//       same sweep (right-looking substitution of L_pp X = I, applying row
-//       j-1 of X at pivot j);
+//   (b) wave 0 factors the 16x16 diagonal block in registers (lane (g, c) =
+//       (lane/16, lane%16) keeps rows 4g..4g+3 of column c; pivots exchanged
+//       by readlane and lane shuffles) and forms its inverse D_p in the same
+//       sweep (right-looking substitution of L_pp X = I, applying row j-1 of
+//       X at pivot j);
 //   (c) L_ip = A_ip D_p^T for the sub-panel rows below (MFMA);
 //   (d) A_ij -= L_ip L_jp^T on the trailing 16x16 lower tiles (MFMA).
 // Then L^-1 is assembled by distance: X_pp = D_p and, for d = 1..7,
@@ -210,6 +211,27 @@ constexpr int LDT = PB + 4;   // LDS row length of the k-major operand tiles
 constexpr int SB = 16;
 constexpr int LDA = PB + 4;
 
+// sqrt(x) and 1/sqrt(x) for the pivots (x > 0): hardware rsqrt seed, two
+// Newton steps on y = x^-1/2 (quadratic: seed error ~2^-23 -> rounding
+// level), then one correction of s = x y; 9 dependent operations.
+__device__ __forceinline__ void pivot_sqrt(double x, double& s, double& is) {
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-hx, y * y, 1.5);
+    y = y * fma(-hx, y * y, 1.5);
+    s = x * y;
+    s = fma(fma(-s, s, x), 0.5 * y, s);
+    is = y;
+}
+
+// 64-bit readlane (the value of lane `src`, wave-uniform)
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp, int kb,
                                                   double* __restrict__ Linv128,
                                                   double* __restrict__ Linv64,
@@ -221,74 +243,76 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
     const int lr = lane & 15, lk = lane >> 4;
     const size_t k0 = (size_t)kb * PB;
     if (t == 0) bad = 0;
-#pragma unroll 8
-    for (int q = t; q < PB * PB / 2; q += 256) {
-        const int i = q / (PB / 2), m = (q % (PB / 2)) * 2;
-        const double2 v = *reinterpret_cast<const double2*>(F + (k0 + i) * Vp + k0 + m);
-        Am[i][m] = (m <= i) ? v.x : 0.0;
-        Am[i][m + 1] = (m + 1 <= i) ? v.y : 0.0;
+    {
+        // wave w loads rows w, w + 4, ...: lane l the column pair 2l, 2l + 1
+        // (one 1 KiB row per wave instruction), 32 loads in flight
+        const int m = 2 * lane;
+        double2 v[32];
+#pragma unroll
+        for (int q = 0; q < 32; ++q)
+            v[q] = *reinterpret_cast<const double2*>(F + (k0 + w + 4 * q) * Vp + k0 + m);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int i = w + 4 * q;
+            Am[i][m] = (m <= i) ? v[q].x : 0.0;
+            Am[i][m + 1] = (m + 1 <= i) ? v[q].y : 0.0;
+        }
     }
     __syncthreads();
     for (int p = 0; p < PB / SB; ++p) {
         const int o = p * SB;
         if (w == 0) {
-            // (b) lane (g, c): rows 4g + rr of column c. Elements above the
-            // diagonal are updated and published too (never read as data),
-            // so the updates need masks only on the column (c > j) and row
-            // (i > j) and no branches.
+            // (b) lane (g, c) keeps rows i = 4g + rr of column c of the
+            // trailing block, mirrored from the lower triangle so that it is
+            // symmetric: a_jc = a_cj then serves as the pivot row. Pivot j:
+            // the pivot by readlane, the pivot column and row by lane
+            // shuffles; rows and columns <= j take masked (unused) updates so
+            // that every update is a plain FMA. The inverse X = L_pp^-1 is
+            // formed alongside (row j-1 of X is applied at pivot j, when it is
+            // final). Results stay in registers (selects, no stores and no
+            // branches in the pivot loop) and are written once at the end.
             const int g = lk, c = lr;
-            double e[4], x[4], lprev[4];
+            double e[4], x[4], lprev[4], lcol[4];
+            bool badl = false;
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
-                e[rr] = Am[o + 4 * g + rr][o + c];
-                x[rr] = (4 * g + rr == c) ? 1.0 : 0.0;
+                const int i = 4 * g + rr;
+                e[rr] = Am[o + (i > c ? i : c)][o + (i > c ? c : i)];
+                x[rr] = (i == c) ? 1.0 : 0.0;
                 lprev[rr] = 0.0;
+                lcol[rr] = 0.0;
             }
 #pragma unroll
             for (int j = 0; j < SB; ++j) {
-                double piv = Am[o + j][o + j];
+                double piv = readlane_f64(e[j & 3], (j >> 2) * 16 + j);
                 double aic[4];
 #pragma unroll
-                for (int rr = 0; rr < 4; ++rr) aic[rr] = Am[o + 4 * g + rr][o + j];
-                const double acj = Am[o + c][o + j];
-                const double xprev = (j > 0) ? Dv[p][j - 1][c] : 0.0;
-                // X rows below j-1 take row j-1 (final since pivot j-1)
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) x[rr] = fma(-lprev[rr], xprev, x[rr]);
-                if (!(piv > 0.0)) {
-                    if (lane == 0) bad = 1;
-                    piv = 1.0;
-                }
+                for (int rr = 0; rr < 4; ++rr) aic[rr] = __shfl(e[rr], g * 16 + j);
+                const double acj = __shfl(e[j & 3], (j >> 2) * 16 + c);
+                const double xprev = (j > 0) ? __shfl(x[(j - 1) & 3], ((j - 1) >> 2) * 16 + c) : 0.0;
+                badl |= !(piv > 0.0);
+                piv = (piv > 0.0) ? piv : 1.0;
                 double s, is;
-                rcm::sqrt_rsqrt(piv, s, is);
+                pivot_sqrt(piv, s, is);
                 const double f = (c > j) ? acj * (is * is) : 0.0;
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) e[rr] = fma(-aic[rr], f, e[rr]);
-                // column j is final: L_ij = a_ij / s (the reads above precede
-                // these writes in this wave's LDS order)
-                if (c == j) {
+                // off the pivot chain: inverse rows, column j of L
 #pragma unroll
-                    for (int rr = 0; rr < 4; ++rr)
-                        Am[o + 4 * g + rr][o + j] = (4 * g + rr == j) ? s : aic[rr] * is;
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int i = 4 * g + rr;
+                    x[rr] = fma(-lprev[rr], xprev, x[rr]);
+                    if (rr == (j & 3)) x[rr] = (g == (j >> 2)) ? x[rr] * is : x[rr];
+                    lcol[rr] = (c == j) ? ((i == j) ? s : aic[rr] * is) : lcol[rr];
+                    lprev[rr] = (i > j) ? aic[rr] * is : 0.0;
                 }
-                if (c == j + 1) {
-#pragma unroll
-                    for (int rr = 0; rr < 4; ++rr) Am[o + 4 * g + rr][o + c] = e[rr];
-                }
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) lprev[rr] = (4 * g + rr > j) ? aic[rr] * is : 0.0;
-                // row j of X is final: scale by 1/L_jj and publish
-                if (g == (j >> 2)) {
-                    x[j & 3] *= is;
-                    Dv[p][j][c] = x[j & 3];
-                }
-                // lanes exchange columns through LDS: order this pivot's
-                // stores before the next pivot's loads (without the fence the
-                // compiler may hoist the loads of a lane that did not store)
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                Am[o + 4 * g + rr][o + c] = lcol[rr];
+                Dv[p][4 * g + rr][c] = x[rr];
+            }
+            if (badl) bad = 1;
         }
         __syncthreads();
         // (c) rows below the diagonal block: L_ip = A_ip D_p^T
@@ -345,18 +369,27 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
         __syncthreads();
     }
     if (t == 0 && bad) atomicOr(status, 1);
-    double* X = Linv128 + k0 * PB;
-#pragma unroll 8
-    for (int q = t; q < PB * PB; q += 256) {
-        const int i = q / PB, m = q % PB;
-        const int bi = i / SB, bm = m / SB;
-        if (m <= i) F[(k0 + i) * Vp + k0 + m] = Am[i][m];
-        const double v = (bm > bi) ? 0.0
-                         : (bm == bi) ? Dv[bi][i % SB][m % SB]
-                                      : Am[16 * bm + m % SB][16 * bi + i % SB];
-        X[(size_t)i * PB + m] = v;
-        if ((i >> 6) == (m >> 6))
-            Linv64[((size_t)(2 * kb + (i >> 6)) * NB + (i & 63)) * NB + (m & 63)] = v;
+    // wave w writes rows w, w + 4, ...: lane l the column pair m = 2l, 2l + 1.
+    // F gets whole rows of the block (the part above the diagonal is never
+    // read); X = L^-1 is D_p on diagonal 16x16 blocks, the transposed upper
+    // blocks of Am below them and zero above.
+    {
+        double* X = Linv128 + k0 * PB;
+        const int m = 2 * lane, bm = m >> 4, h = m >> 6;
+#pragma unroll 4
+        for (int q = 0; q < 32; ++q) {
+            const int i = w + 4 * q, bi = i >> 4;
+            *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) = make_double2(Am[i][m], Am[i][m + 1]);
+            double2 xv;
+            if (bm > bi) xv = make_double2(0.0, 0.0);
+            else if (bm == bi) xv = make_double2(Dv[bi][i & 15][m & 15], Dv[bi][i & 15][(m & 15) + 1]);
+            else xv = make_double2(Am[16 * bm + (m & 15)][16 * bi + (i & 15)],
+                                   Am[16 * bm + (m & 15) + 1][16 * bi + (i & 15)]);
+            *reinterpret_cast<double2*>(X + (size_t)i * PB + m) = xv;
+            if ((i >> 6) == h)
+                *reinterpret_cast<double2*>(
+                    Linv64 + ((size_t)(2 * kb + h) * NB + (i & 63)) * NB + (m & 63)) = xv;
+        }
     }
 }
 
@@ -368,14 +401,14 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
 // buffered with the next chunk's global loads in flight during the MFMAs.
 // MFMA operand layout: A[lane%16][lane/16], B[lane/16][lane%16];
 // result element r of a lane is D[4r + lane/16][lane%16].
-template <int MT>
+template <int MT, int KC>
 __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int lda,
                                             const double* __restrict__ B, int ldb,
                                             double (*As)[KC][LDT], double (*Bs)[KC][LDT],
                                             d4 (&acc)[MT / 32][4]) {
     constexpr int TA = MT / 32;
-    constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk (8 or 4)
-    constexpr int NBv = PB * KC / 256;  // B doubles per thread per chunk (8)
+    constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk
+    constexpr int NBv = PB * KC / 256;  // B doubles per thread per chunk
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wr = (w >> 1) * (MT / 2), wc = (w & 1) * 64;
     const int arow = t / (KC / NA), acol = (t % (KC / NA)) * NA;
@@ -435,8 +468,8 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
 // workgroup, in place.
 __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int Vp, int kb,
                                                     const double* __restrict__ Linv128) {
-    __shared__ double As[2][KC][LDT];
-    __shared__ double Bs[2][KC][LDT];
+    __shared__ double As[2][KC_TRSM][LDT];
+    __shared__ double Bs[2][KC_TRSM][LDT];
     const size_t k0 = (size_t)kb * PB;
     const size_t r0 = k0 + PB + (size_t)blockIdx.x * 64;
     d4 acc[2][4];
@@ -444,7 +477,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<64>(F + r0 * Vp + k0, Vp, Linv128 + k0 * PB, PB, As, Bs, acc);
+    gemm_nt_f64<64, KC_TRSM>(F + r0 * Vp + k0, Vp, Linv128 + k0 * PB, PB, As, Bs, acc);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
 #pragma unroll
@@ -464,8 +497,8 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
 // that XCD's L2).
 __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
                                                  int ntiles) {
-    __shared__ double As[2][KC][LDT];
-    __shared__ double Bs[2][KC][LDT];
+    __shared__ double As[2][KC_SYRK][LDT];
+    __shared__ double Bs[2][KC_SYRK][LDT];
     const int per = (ntiles + 7) >> 3;
     const int p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (p >= ntiles) return;
@@ -478,7 +511,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<128>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
+    gemm_nt_f64<128, KC_SYRK>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
     const bool diag = (ti == tj);
@@ -722,9 +755,27 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
                            (const double*)ctx->d_Linv128);
         RC_CHECK_LAUNCH();
         const int ntiles = nt * (nt + 1) / 2;
+        hipEvent_t* ev = nullptr;
+        if (ctx->kprof) {
+            if (ctx->kev_used + 2 > ctx->kev.size()) {
+                // no system-scope fence: a timing marker must not write back
+                // the L2 (the factor leaves it full of dirty lines)
+                hipEvent_t a, b;
+                RC_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+                RC_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
+                ctx->kev.push_back(a);
+                ctx->kev.push_back(b);
+            }
+            ev = &ctx->kev[ctx->kev_used];
+            ctx->kev_used += 2;
+            const double m = (double)nt * PB;
+            ctx->kev_flop.push_back((double)PB * m * (m + 1.0));
+            RC_HIP(hipEventRecord(ev[0], ctx->stream));
+        }
         hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream,
                            ctx->d_F, Vp, kb, ntiles);
         RC_CHECK_LAUNCH();
+        if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream));
     }
     return RC_OK;
 }

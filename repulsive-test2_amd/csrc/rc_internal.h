@@ -118,6 +118,14 @@ struct rc_ctx {
     double* d_stats = nullptr;
     double* d_epart_ls = nullptr;
     double* d_refine = nullptr;
+
+    // kernel-level profile (rc_kernel_profile_enable / _read)
+    int kprof = 0;
+    std::vector<hipEvent_t> kev;   // event pool, pairs around trailing-update launches
+    std::vector<double> kev_flop;  // algorithmic flops of each recorded launch
+    size_t kev_used = 0;           // events in use since the last read
+    double kprof_ms = 0.0, kprof_flop = 0.0;
+    int kprof_launches = 0;
 };
 
 // ---- helpers --------------------------------------------------------------

@@ -107,11 +107,16 @@ class Timings(C.Structure):
                 ("dE", "energy", "assembleA", "saddle", "factor", "projection", "lineSearch", "step")]
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [("syrk_ms", C.c_double), ("syrk_flop", C.c_double),
+                ("syrk_launches", C.c_int32), ("_pad", C.c_int32)]
+
+
 EXPORTS = (
     "rc_default_step_opts", "rc_ctx_create", "rc_ctx_destroy", "rc_last_error", "rc_energy",
     "rc_gradient", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
     "rc_constraint_rows", "rc_sobolev_step", "rc_set_positions", "rc_get_positions",
-    "rc_sobolev_step_resident",
+    "rc_sobolev_step_resident", "rc_kernel_profile_enable", "rc_kernel_profile_read",
 )
 
 _lib = None
@@ -150,9 +155,12 @@ def lib():
     L.rc_sobolev_step_resident.argtypes = [vp, C.POINTER(Block), C.c_int32, C.POINTER(Penalties),
                                            C.POINTER(StepOpts), C.POINTER(StepStats),
                                            C.POINTER(Timings)]
+    L.rc_kernel_profile_enable.argtypes = [vp, C.c_int32]
+    L.rc_kernel_profile_read.argtypes = [vp, C.POINTER(KernelProfile)]
     for name in ("rc_ctx_create", "rc_energy", "rc_gradient", "rc_assemble_a", "rc_penalties_eval",
                  "rc_solve_constrained_gradient", "rc_sobolev_step", "rc_set_positions",
-                 "rc_get_positions", "rc_sobolev_step_resident"):
+                 "rc_get_positions", "rc_sobolev_step_resident", "rc_kernel_profile_enable",
+                 "rc_kernel_profile_read"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

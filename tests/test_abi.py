@@ -44,3 +44,4 @@ def test_struct_sizes_match_header_layout():
     # rc_block: 2 ints, 3 + 1 + 3 doubles, pointer, 2 ints
     assert ctypes.sizeof(_lib.Block) == 8 + 56 + 8 + 8
     assert ctypes.sizeof(_lib.StepStats) == 8 * 8 + 6 * 4
+    assert ctypes.sizeof(_lib.KernelProfile) == 8 + 8 + 4 + 4
