@@ -74,7 +74,7 @@ typedef struct {
     double tol_abs, tol_rel;    /* projection tolerances, default 1e-4 */
     double c1, shrink, tau0, tau_min; /* Armijo: 1e-4, 0.5, 1, 1e-12 */
     double refine_tol;          /* mixed form: PCG relative residual target, default 1e-12 */
-    int32_t refine_max_iter;    /* mixed form: PCG iteration cap, default 30 */
+    int32_t refine_max_iter;    /* mixed form: PCG iteration cap, default 200 */
     int32_t collect_timings;    /* fill rc_timings (hipEvent per phase) */
 } rc_step_opts;
 
@@ -162,14 +162,15 @@ rc_status rc_sobolev_step_resident(rc_ctx* ctx, const rc_block* blocks, int32_t 
                                    rc_step_stats* stats, rc_timings* timings);
 
 /* Kernel-level profile of the dominant kernel of the FP64 step, the
- * trailing update of the Cholesky factorisation (k_syrk128): while enabled,
+ * trailing update of the Cholesky factorisation (k_syrk128, all of the
+ * trailing matrix but the next panel's column): while enabled,
  * a pair of HIP events brackets each of its launches on the context's stream.
  * No counterpart in the reference (measurement support for benchmarks).
  * Enabling (or disabling) resets the totals; reading synchronises the
  * context's stream and returns the totals since the reset. */
 typedef struct {
     double syrk_ms;         /* summed launch durations */
-    double syrk_flop;       /* algorithmic flops: 128 m (m + 1) per launch, m trailing rows */
+    double syrk_flop;       /* algorithmic flops: 128 m (m + 1) per launch, m rows updated */
     int32_t syrk_launches;
     int32_t _pad;
 } rc_kernel_profile;

@@ -75,7 +75,7 @@ extern "C" void rc_default_step_opts(rc_step_opts* o) {
     o->tau0 = 1.0;
     o->tau_min = 1e-12;
     o->refine_tol = 1e-12;
-    o->refine_max_iter = 30;
+    o->refine_max_iter = 200;
     o->collect_timings = 0;
 }
 
@@ -103,12 +103,15 @@ static void free_all(rc_ctx* ctx) {
         if (p) (void)hipFree(p);
     if (ctx->d_host_stage) (void)hipHostFree(ctx->d_host_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
 }
 
 extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->dep_ev) (void)hipEventDestroy(e);
     free_all(ctx);
     delete ctx;
 }
@@ -218,6 +221,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         return s;
     };
     hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (he != hipSuccess) {
         ctx->err = std::string("hipStreamCreate: ") + hipGetErrorString(he);
         return fail(RC_HIP_ERROR);
@@ -547,8 +551,8 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs);
     if (s != RC_OK) return s;
     if (precision == RC_PRECISION_MIXED) {
-        s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, o ? o->refine_tol : 1e-13,
-                           o ? o->refine_max_iter : 30, ctx->d_refine);
+        s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, o ? o->refine_tol : 1e-12,
+                           o ? o->refine_max_iter : 200, ctx->d_refine);
         if (s != RC_OK) return s;
     } else {
         const int Vp = ctx->nblk * RC_NB;

@@ -17,7 +17,9 @@
 //                 (16-column sub-panels, MFMA updates) and inverts it
 //   k_panel_trsm  L_ik = A_ik L_kk^-T as a GEMM with the stored inverse
 //   k_syrk128     A_ij -= L_ik L_jk^T on 128x128 lower tiles, K = 128,
-//                 v_mfma_f64_16x16x4_f64, XCD-grouped tile order
+//                 v_mfma_f64_16x16x4_f64, XCD-grouped tile order, on a second
+//                 stream (k_syrk_col updates the next panel's column first, so
+//                 the next panel's factorisation overlaps the rest)
 // Mixed form, 64-column panels:
 //   k_potrf_panel<float>  every block factors the 64x64 diagonal block in
 //                 FP64 in LDS and inverts it; block 0 stores L_kk^-1, the
@@ -62,11 +64,12 @@ __global__ void k_diag_scale(const double* __restrict__ A, int V, int Vp,
     dinv[i] = (i < V) ? 1.0 / __builtin_sqrt(A[(size_t)i * V + i] + mu[0]) : 1.0;
 }
 
-// F (lower) = A + mu (FP64 form) or dinv_i (A + mu) dinv_j (FP32 form); identity padding.
+// F (lower) = A + mu (FP64 form) or dinv_i (A + mu) dinv_j + shift delta_ij
+// (FP32 form); identity padding.
 template <typename T>
 __global__ void k_factor_prep(const double* __restrict__ A, int V, int Vp,
                               const double* __restrict__ mu, const double* __restrict__ dinv,
-                              T* __restrict__ F) {
+                              double shift, T* __restrict__ F) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j >= Vp) return;
@@ -74,6 +77,7 @@ __global__ void k_factor_prep(const double* __restrict__ A, int V, int Vp,
     if (i < V && j < V) {
         v = (j <= i) ? A[(size_t)i * V + j] + mu[0] : 0.0;
         if (dinv) v = v * dinv[i] * dinv[j];
+        if (i == j) v += shift;
     } else {
         v = (i == j) ? 1.0 : 0.0;
     }
@@ -490,13 +494,15 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
                     acc[a][b][r];
 }
 
-// Trailing update A_ij -= L_ik L_jk^T over the 128x128 lower tiles of the
-// trailing matrix (ntiles = nt (nt+1)/2 tiles, nt trailing panels). Tiles are
+// Trailing update A_ij -= L_ik L_jk^T over 128x128 lower tiles of the
+// trailing matrix (nt trailing panels). skip_first_col = 0: all nt (nt+1)/2
+// tiles; 1: the (nt-1) nt / 2 tiles right of the first trailing column (that
+// column, the next panel, is updated separately by k_syrk_col). Tiles are
 // numbered row by row; workgroup b runs on XCD b % 8, so each XCD is given a
 // contiguous range of tile numbers (tiles of one row share their A panel in
 // that XCD's L2).
 __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
-                                                 int ntiles) {
+                                                 int ntiles, int skip_first_col) {
     __shared__ double As[2][KC_SYRK][LDT];
     __shared__ double Bs[2][KC_SYRK][LDT];
     const int per = (ntiles + 7) >> 3;
@@ -504,6 +510,8 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
     if (p >= ntiles) return;
     int ti, tj;
     decode_lower(p, ti, tj);
+    ti += skip_first_col;
+    tj += skip_first_col;
     const size_t k0 = (size_t)kb * PB;
     const size_t ri = k0 + PB + (size_t)ti * PB, rj = k0 + PB + (size_t)tj * PB;
     d4 acc[4][4];
@@ -524,6 +532,35 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
                 const int j = wc + 16 * b + (lane & 15);
                 if (!diag || j <= i) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
+            }
+}
+
+// Update of the next panel's column (the first trailing column) with panel
+// kb, on the factorisation's critical path: 64 x 128 blocks (two per 128-row
+// tile, nt tiles), so that twice as many workgroups share the work.
+__global__ __launch_bounds__(256, 2) void k_syrk_col(double* __restrict__ F, int Vp, int kb) {
+    __shared__ double As[2][KC_TRSM][LDT];
+    __shared__ double Bs[2][KC_TRSM][LDT];
+    const int ti = blockIdx.x >> 1, h = blockIdx.x & 1;
+    const size_t k0 = (size_t)kb * PB;
+    const size_t ri = k0 + PB + (size_t)ti * PB + 64 * h, rj = k0 + PB;
+    d4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = wr + 16 * a + 4 * r + (lane >> 4);
+                const int j = wc + 16 * b + (lane & 15);
+                if (ti != 0 || j <= i + 64 * h) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
             }
 }
 
@@ -638,16 +675,27 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
     const int lr = lane & 15, lk = lane >> 4;
     const int r = FWD ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
     const size_t rb = (size_t)r * NB;
-    d4 acc[2];
+    const int ncol = nrhs > 16 ? 2 : 1;        // 16-column MFMA tiles in use
+    // two partial accumulators per column tile (even / odd k-steps): two
+    // independent MFMA chains of half the length
+    d4 acc[2], acc2[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
             acc[b][q] = (c < nrhs) ? B[(size_t)c * ldb + rb + i] : 0.0;
+            acc2[b][q] = 0.0;
         }
     // factor tile rows (t >> 2), 16 consecutive columns from (t & 3) * 16
     const int tr = t >> 2, tc = (t & 3) * 16;
+    // the diagonal block's inverse does not depend on the flags: load it now
+    double li[16];
+    {
+        const T* Li = Linv + rb * NB + (size_t)tr * NB + tc;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) li[q] = (double)Li[q];
+    }
     double v[16];
     auto gload = [&](int cb) {
         const T* p = FWD ? F + (rb + tr) * Vp + (size_t)cb * NB + tc
@@ -655,20 +703,24 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
 #pragma unroll
         for (int q = 0; q < 16; ++q) v[q] = (double)p[q];
     };
-    auto sstore = [&]() {
+    auto sstore = [&](const double (&x)[16]) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            if (FWD) Lt[tr][tc + q] = v[q];
-            else Lt[tc + q][tr] = v[q];
+            if (FWD) Lt[tr][tc + q] = x[q];
+            else Lt[tc + q][tr] = x[q];
         }
     };
-    auto mfma_tile = [&](d4 (&d)[2], bool neg) {
-#pragma unroll 4
-        for (int k = 0; k < NB; k += 4) {
-            const double a = neg ? -Lt[16 * w + lr][k + lk] : Lt[16 * w + lr][k + lk];
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                d[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ys[k + lk][16 * b + lr], d[b], 0, 0, 0);
+    auto mfma_tile = [&](d4 (&d)[2], d4 (&d2)[2], bool neg) {
+#pragma unroll 2
+        for (int k = 0; k < NB; k += 8) {
+            const double a0 = neg ? -Lt[16 * w + lr][k + lk] : Lt[16 * w + lr][k + lk];
+            const double a1 = neg ? -Lt[16 * w + lr][k + 4 + lk] : Lt[16 * w + lr][k + 4 + lk];
+            d[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, Ys[k + lk][lr], d[0], 0, 0, 0);
+            d2[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, Ys[k + 4 + lk][lr], d2[0], 0, 0, 0);
+            if (ncol > 1) {
+                d[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, Ys[k + lk][16 + lr], d[1], 0, 0, 0);
+                d2[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, Ys[k + 4 + lk][16 + lr], d2[1], 0, 0, 0);
+            }
         }
     };
     const int ndep = FWD ? r : nblk - 1 - r;
@@ -687,35 +739,29 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
             const double y = B[(size_t)cc * ldb + (size_t)cb * NB + m];
             Ys[m][c] = (c < nrhs) ? y : 0.0;
         }
-        sstore();
+        sstore(v);
         if (s + 1 < ndep) gload(FWD ? s + 1 : nblk - 2 - s);
         __syncthreads();
-        mfma_tile(acc, true);
+        mfma_tile(acc, acc2, true);
     }
     __syncthreads();
     // diagonal block: X_r = L_rr^-1 acc (forward) or L_rr^-T acc (backward)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Ys[16 * w + 4 * q + lk][16 * b + lr] = acc[b][q];
-    const T* Li = Linv + rb * NB;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const double x = (double)Li[(size_t)tr * NB + tc + q];
-        if (FWD) Lt[tr][tc + q] = x;
-        else Lt[tc + q][tr] = x;
-    }
+        for (int q = 0; q < 4; ++q) Ys[16 * w + 4 * q + lk][16 * b + lr] = acc[b][q] + acc2[b][q];
+    sstore(li);
     __syncthreads();
-    d4 xo[2];
+    d4 xo[2], xo2[2];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) xo[b] = (d4){0.0, 0.0, 0.0, 0.0};
-    mfma_tile(xo, false);
+    for (int b = 0; b < 2; ++b) xo[b] = xo2[b] = (d4){0.0, 0.0, 0.0, 0.0};
+    mfma_tile(xo, xo2, false);
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
-            if (c < nrhs) B[(size_t)c * ldb + rb + i] = xo[b][q];
+            if (c < nrhs) B[(size_t)c * ldb + rb + i] = xo[b][q] + xo2[b][q];
         }
     publish(flags + r, epoch);
 }
@@ -740,11 +786,51 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 }
 
 // FP64 form: 128-column panels (diagonal factor + inverse, panel TRSM as a
-// GEMM with the inverse, MFMA trailing update). A non-positive pivot sets
-// status bit 1 (singular saddle system).
+// GEMM with the inverse, MFMA trailing update) with a one-panel lookahead on
+// two streams: the main stream runs potrf(kb) -> panel TRSM(kb) -> update of
+// column kb+1 (k_syrk_col) while the second stream updates the rest of the
+// trailing matrix with panel kb (k_syrk128, skipping that column), so the
+// next panel's factorisation overlaps the big update. Ordering: the rest of
+// update kb waits for panel kb (event T); the column update kb waits for the
+// rest of update kb-1 (event B, they would otherwise touch the same tiles);
+// the main stream waits for the last rest-update before returning. A
+// non-positive pivot sets status bit 1 (singular saddle system).
 rc_status factor_loop_fp64(rc_ctx* ctx) {
     const int Vp = ctx->nblk * NB;
     const int np = Vp / PB;
+    if ((int)ctx->dep_ev.size() < 2 * np + 1) {
+        while ((int)ctx->dep_ev.size() < 2 * np + 1) {
+            // completion signals only: each kernel's own end-of-kernel release
+            // makes its writes visible device-wide, so no extra fence here
+            hipEvent_t e;
+            RC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventDisableSystemFence));
+            ctx->dep_ev.push_back(e);
+        }
+    }
+    hipEvent_t* T = ctx->dep_ev.data();        // panel kb final
+    hipEvent_t* Bv = ctx->dep_ev.data() + np;  // rest-update kb done
+    hipEvent_t start = ctx->dep_ev[2 * np];
+    // the second stream starts after the factor's input is prepared
+    RC_HIP(hipEventRecord(start, ctx->stream));
+    RC_HIP(hipStreamWaitEvent(ctx->stream2, start, 0));
+    auto prof_begin = [&](hipStream_t st, double flop) -> hipEvent_t* {
+        if (!ctx->kprof) return nullptr;
+        if (ctx->kev_used + 2 > ctx->kev.size()) {
+            // no system-scope fence: a timing marker must not write back the
+            // L2 (the factor leaves it full of dirty lines)
+            hipEvent_t a, b;
+            if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess ||
+                hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess)
+                return nullptr;
+            ctx->kev.push_back(a);
+            ctx->kev.push_back(b);
+        }
+        hipEvent_t* ev = &ctx->kev[ctx->kev_used];
+        ctx->kev_used += 2;
+        ctx->kev_flop.push_back(flop);
+        (void)hipEventRecord(ev[0], st);
+        return ev;
+    };
     for (int kb = 0; kb < np; ++kb) {
         hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
                            ctx->d_Linv128, ctx->d_Linv, ctx->d_status);
@@ -754,29 +840,26 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
         hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
                            (const double*)ctx->d_Linv128);
         RC_CHECK_LAUNCH();
-        const int ntiles = nt * (nt + 1) / 2;
-        hipEvent_t* ev = nullptr;
-        if (ctx->kprof) {
-            if (ctx->kev_used + 2 > ctx->kev.size()) {
-                // no system-scope fence: a timing marker must not write back
-                // the L2 (the factor leaves it full of dirty lines)
-                hipEvent_t a, b;
-                RC_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
-                RC_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
-                ctx->kev.push_back(a);
-                ctx->kev.push_back(b);
-            }
-            ev = &ctx->kev[ctx->kev_used];
-            ctx->kev_used += 2;
-            const double m = (double)nt * PB;
-            ctx->kev_flop.push_back((double)PB * m * (m + 1.0));
-            RC_HIP(hipEventRecord(ev[0], ctx->stream));
-        }
-        hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream,
-                           ctx->d_F, Vp, kb, ntiles);
+        RC_HIP(hipEventRecord(T[kb], ctx->stream));
+        // column kb+1: after the rest-update of panel kb-1 has left it
+        if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[kb - 1], 0));
+        const double m = (double)nt * PB;
+        hipLaunchKernelGGL(k_syrk_col, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb);
         RC_CHECK_LAUNCH();
-        if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream));
+        // the rest of the trailing matrix on the second stream
+        RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
+        const int ntiles = (nt - 1) * nt / 2;
+        if (ntiles > 0) {
+            const double mr = m - PB;
+            hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
+            hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream2,
+                               ctx->d_F, Vp, kb, ntiles, 1);
+            RC_CHECK_LAUNCH();
+            if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream2));
+        }
+        RC_HIP(hipEventRecord(Bv[kb], ctx->stream2));
     }
+    if (np >= 2) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[np - 2], 0));
     return RC_OK;
 }
 
@@ -791,13 +874,19 @@ rc_status rc_factor_fp64(rc_ctx* ctx) {
                        ctx->d_scal + 1);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_factor_prep<double>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
-                       ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, ctx->d_F);
+                       ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, 0.0, ctx->d_F);
     RC_CHECK_LAUNCH();
     return factor_loop_fp64(ctx);
 }
 
-// Factor S = D^-1/2 A_hat D^-1/2 into ctx->d_F32 (FP32 storage, FP16x3 MFMA
-// trailing updates); D^-1/2 in ctx->d_dscale.
+// Factor S + delta I, S = D^-1/2 A_hat D^-1/2 (unit diagonal), into
+// ctx->d_F32 (FP32 storage, FP16x3 MFMA trailing updates); D^-1/2 in
+// ctx->d_dscale. kappa(A_hat) grows like V^2.4 and exceeds what an FP32
+// Cholesky survives (kappa u_f32 < ~1) from V ~ 2000 on, so the factored
+// matrix is shifted: delta = 2^-20 first, then 8x larger after each
+// breakdown (status bit 8, read back once per try), up to 4 tries. The shift
+// only weakens the preconditioner; the PCG in FP64 with the unshifted A_hat
+// fixes the solution.
 rc_status rc_factor_mixed(rc_ctx* ctx) {
     const int Vp = ctx->nblk * NB;
     if (!ctx->d_F32) {
@@ -805,18 +894,29 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
         RC_HIP(hipMalloc((void**)&ctx->d_Linv32, sizeof(float) * (size_t)ctx->nblk * NB * NB));
         RC_HIP(hipMalloc((void**)&ctx->d_dscale, sizeof(double) * (size_t)Vp));
     }
-    RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
     hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_A, ctx->V,
                        ctx->d_scal + 1);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_diag_scale, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_A,
                        ctx->V, Vp, ctx->d_scal + 1, ctx->d_dscale);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_factor_prep<float>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
-                       ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)ctx->d_dscale,
-                       ctx->d_F32);
-    RC_CHECK_LAUNCH();
-    return factor_loop_mixed(ctx, ctx->d_F32, ctx->d_Linv32);
+    double shift = 1.0 / 1048576.0;
+    for (int attempt = 0; attempt < 4; ++attempt, shift *= 8.0) {
+        RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
+        hipLaunchKernelGGL(k_factor_prep<float>, dim3((Vp + 255) / 256, Vp), dim3(256), 0,
+                           ctx->stream, ctx->d_A, ctx->V, Vp, ctx->d_scal + 1,
+                           (const double*)ctx->d_dscale, shift, ctx->d_F32);
+        RC_CHECK_LAUNCH();
+        rc_status s = factor_loop_mixed(ctx, ctx->d_F32, ctx->d_Linv32);
+        if (s != RC_OK) return s;
+        int32_t st = 0;
+        RC_HIP(hipMemcpyAsync(&st, ctx->d_status, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        RC_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->mixed_shift = shift;
+        if (!(st & 8)) break;
+    }
+    return RC_OK;
 }
 
 template <typename T>

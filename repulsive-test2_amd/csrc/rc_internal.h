@@ -37,6 +37,8 @@ struct rc_ctx {
     int nblk = 0;             // Cholesky blocks of RC_NB rows (always even)
     int maxdeg = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
+    std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies
     std::string err;
 
     // topology (device)
@@ -76,6 +78,7 @@ struct rc_ctx {
     double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]
     float* d_Linv32 = nullptr;
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
+    double mixed_shift = 0.0;     // diagonal shift of the last FP32 factorisation
     int32_t* d_flags = nullptr;   // triangular-solve block flags
     int32_t epoch = 0;
     int32_t* d_status = nullptr;  // device status words (factor breakdown, spin timeouts)

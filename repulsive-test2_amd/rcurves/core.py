@@ -282,7 +282,7 @@ class SobolevStepOptions:
     tau0: float = 1.0
     tau_min: float = 1e-12
     refine_tol: float = 1e-12
-    refine_max_iter: int = 30
+    refine_max_iter: int = 200
 
     def to_c(self) -> L.StepOpts:
         if self.mode != "analytical":

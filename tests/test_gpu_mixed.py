@@ -36,10 +36,21 @@ def test_mixed_constrained_gradient_goldens(goldens, name):
     assert res <= 1e-10
 
 
-@pytest.mark.parametrize("V,maker", [(1024, "torus"), (2048, "fourier")])
+@pytest.mark.parametrize("V,maker", [
+    (1024, "torus"), (2048, "fourier"), (4096, "torus27"),
+    pytest.param(8192, "fourier", marks=pytest.mark.xfail(
+        strict=False,
+        reason="V=8192, kappa(A_hat)~1e9: PCG to relres 1e-12 lands at energy rel err ~6e-9 and "
+               "direction ~2.4e-8 vs FP64 (gates 1e-10 / 1e-8); FP64 Cholesky's own forward error "
+               "here is ~1.3e-9, so the gate sits at the FP64 noise floor")),
+])
 def test_mixed_vs_fp64_step(V, maker):
+    """Sizes past the FP32 Cholesky limit (kappa u_f32 > 1) included: the
+    (2,7) torus knot at V = 4096 and the Fourier curve at V = 8192."""
     if maker == "torus":
         v, e = curves.torus_knot(3, 5, V, jitter=0.02, seed=1)
+    elif maker == "torus27":
+        v, e = curves.torus_knot(2, 7, V, jitter=0.02, seed=1)
     else:
         v, e = curves.fourier_curve(V, seed=1)
     x0 = core.barycenter_target(v, e)
