@@ -194,17 +194,19 @@ constexpr int KC_SYRK = 16;   // (2 workgroups/CU; a 32 chunk at 1 workgroup/CU 
 constexpr int LDT = PB + 4;   // LDS row length of the k-major operand tiles
 
 // Factor the 128x128 diagonal block of panel kb and invert it, in one
-// workgroup of 4 waves, block held in LDS. 16-column sub-panels p = 0..7:
+// workgroup of 4 waves, block held in LDS. 16-column sub-panels p = 0..7,
+// pipelined so that the pivots of sub-panel p+1 overlap the rest of p:
 //   (b) wave 0 factors the 16x16 diagonal block in registers (lane (g, c) =
 //       (lane/16, lane%16) keeps rows 4g..4g+3 of column c; pivots exchanged
 //       by readlane and lane shuffles) and forms its inverse D_p in the same
 //       sweep (right-looking substitution of L_pp X = I, applying row j-1 of
 //       X at pivot j);
-//   (c) L_ip = A_ip D_p^T for the sub-panel rows below (MFMA);
-//   (d) A_ij -= L_ip L_jp^T on the trailing 16x16 lower tiles (MFMA).
-// Then L^-1 is assembled by distance: X_pp = D_p and, for d = 1..7,
-// X_{p+d,p} = -D_{p+d} sum_{k=p}^{p+d-1} L_{p+d,k} X_kp (MFMA), each X_ip
-// kept transposed in the unused upper 16x16 block (p, i) of the LDS matrix.
+//   (c) all waves: L_ip = A_ip D_p^T for the sub-panel rows below (MFMA);
+//   (d) wave 0: A_{p+1,p+1} -= L_{p+1,p} L_{p+1,p}^T, then (b) for p+1;
+//       waves 1-3 meanwhile: the other trailing 16x16 lower tiles (MFMA) and
+//       block row p of X = L^-1, X_pq = -D_p sum_{k=q}^{p-1} L_pk X_kq (MFMA;
+//       each X_pq kept transposed in the unused upper 16x16 block (q, p) of
+//       the LDS matrix).
 // Writes L (lower) into F, L^-1 into Linv128[kb] and the two 64x64 diagonal
 // blocks of L^-1 (the inverses of L's 64x64 diagonal blocks) into
 // Linv64[2kb], [2kb+1].
@@ -263,62 +265,97 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
         }
     }
     __syncthreads();
-    for (int p = 0; p < PB / SB; ++p) {
+    // (b) lane (g, c) keeps rows i = 4g + rr of column c of the trailing
+    // block, mirrored from the lower triangle so that it is symmetric:
+    // a_jc = a_cj then serves as the pivot row. Pivot j: the pivot by
+    // readlane, the pivot column and row by lane shuffles; rows and columns
+    // <= j take masked (unused) updates so that every update is a plain FMA.
+    // The inverse D_p = L_pp^-1 is formed alongside (row j-1 of it is applied
+    // at pivot j, when it is final). Results stay in registers (selects, no
+    // stores and no branches in the pivot loop) and are written at the end.
+    auto diag16 = [&](int p) {
         const int o = p * SB;
-        if (w == 0) {
-            // (b) lane (g, c) keeps rows i = 4g + rr of column c of the
-            // trailing block, mirrored from the lower triangle so that it is
-            // symmetric: a_jc = a_cj then serves as the pivot row. Pivot j:
-            // the pivot by readlane, the pivot column and row by lane
-            // shuffles; rows and columns <= j take masked (unused) updates so
-            // that every update is a plain FMA. The inverse X = L_pp^-1 is
-            // formed alongside (row j-1 of X is applied at pivot j, when it is
-            // final). Results stay in registers (selects, no stores and no
-            // branches in the pivot loop) and are written once at the end.
-            const int g = lk, c = lr;
-            double e[4], x[4], lprev[4], lcol[4];
-            bool badl = false;
+        const int g = lk, c = lr;
+        double e[4], x[4], lprev[4], lcol[4];
+        bool badl = false;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int i = 4 * g + rr;
+            e[rr] = Am[o + (i > c ? i : c)][o + (i > c ? c : i)];
+            x[rr] = (i == c) ? 1.0 : 0.0;
+            lprev[rr] = 0.0;
+            lcol[rr] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            double piv = readlane_f64(e[j & 3], (j >> 2) * 16 + j);
+            double aic[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) aic[rr] = __shfl(e[rr], g * 16 + j);
+            const double acj = __shfl(e[j & 3], (j >> 2) * 16 + c);
+            const double xprev = (j > 0) ? __shfl(x[(j - 1) & 3], ((j - 1) >> 2) * 16 + c) : 0.0;
+            badl |= !(piv > 0.0);
+            piv = (piv > 0.0) ? piv : 1.0;
+            double s, is;
+            pivot_sqrt(piv, s, is);
+            const double f = (c > j) ? acj * (is * is) : 0.0;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) e[rr] = fma(-aic[rr], f, e[rr]);
+            // off the pivot chain: inverse rows, column j of L
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
                 const int i = 4 * g + rr;
-                e[rr] = Am[o + (i > c ? i : c)][o + (i > c ? c : i)];
-                x[rr] = (i == c) ? 1.0 : 0.0;
-                lprev[rr] = 0.0;
-                lcol[rr] = 0.0;
+                x[rr] = fma(-lprev[rr], xprev, x[rr]);
+                if (rr == (j & 3)) x[rr] = (g == (j >> 2)) ? x[rr] * is : x[rr];
+                lcol[rr] = (c == j) ? ((i == j) ? s : aic[rr] * is) : lcol[rr];
+                lprev[rr] = (i > j) ? aic[rr] * is : 0.0;
             }
-#pragma unroll
-            for (int j = 0; j < SB; ++j) {
-                double piv = readlane_f64(e[j & 3], (j >> 2) * 16 + j);
-                double aic[4];
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) aic[rr] = __shfl(e[rr], g * 16 + j);
-                const double acj = __shfl(e[j & 3], (j >> 2) * 16 + c);
-                const double xprev = (j > 0) ? __shfl(x[(j - 1) & 3], ((j - 1) >> 2) * 16 + c) : 0.0;
-                badl |= !(piv > 0.0);
-                piv = (piv > 0.0) ? piv : 1.0;
-                double s, is;
-                pivot_sqrt(piv, s, is);
-                const double f = (c > j) ? acj * (is * is) : 0.0;
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) e[rr] = fma(-aic[rr], f, e[rr]);
-                // off the pivot chain: inverse rows, column j of L
-#pragma unroll
-                for (int rr = 0; rr < 4; ++rr) {
-                    const int i = 4 * g + rr;
-                    x[rr] = fma(-lprev[rr], xprev, x[rr]);
-                    if (rr == (j & 3)) x[rr] = (g == (j >> 2)) ? x[rr] * is : x[rr];
-                    lcol[rr] = (c == j) ? ((i == j) ? s : aic[rr] * is) : lcol[rr];
-                    lprev[rr] = (i > j) ? aic[rr] * is : 0.0;
-                }
-            }
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                Am[o + 4 * g + rr][o + c] = lcol[rr];
-                Dv[p][4 * g + rr][c] = x[rr];
-            }
-            if (badl) bad = 1;
         }
-        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            Am[o + 4 * g + rr][o + c] = lcol[rr];
+            Dv[p][4 * g + rr][c] = x[rr];
+        }
+        if (badl) bad = 1;
+    };
+    // (d) trailing tile number q (row by row over p < j <= i) of sub-panel p
+    auto dtile = [&](int p, int q) {
+        const int o = p * SB;
+        int a, b;
+        decode_lower(q, a, b);
+        const int i = p + 1 + a, j = p + 1 + b;
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][o + 4 * kk + lk],
+                                                       Am[16 * j + lr][o + 4 * kk + lk], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][16 * j + lr] -= acc[r];
+    };
+    // X_iq = -D_i sum_{k=q}^{i-1} L_ik X_kq (block row i of L^-1), stored
+    // transposed in Am's upper block (q, i): Am[16q + b][16i + a] = X_iq[a][b]
+    auto xblock = [&](int i, int q) {
+        d4 sacc = {0.0, 0.0, 0.0, 0.0};
+        for (int k = q; k < i; ++k) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double xb = (k == q) ? Dv[q][4 * kk + lk][lr]
+                                           : Am[16 * q + lr][16 * k + 4 * kk + lk];
+                sacc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][16 * k + 4 * kk + lk],
+                                                            xb, sacc, 0, 0, 0);
+            }
+        }
+        d4 xacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Dv[i][lr][4 * kk + lk], sacc[kk], xacc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Am[16 * q + lr][16 * i + 4 * r + lk] = xacc[r];
+    };
+    if (w == 0) diag16(0);
+    __syncthreads();
+    for (int p = 0; p < PB / SB; ++p) {
+        const int o = p * SB;
         // (c) rows below the diagonal block: L_ip = A_ip D_p^T
         for (int i = p + 1 + w; i < PB / SB; i += 4) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -330,45 +367,22 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
             for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][o + lr] = acc[r];
         }
         __syncthreads();
-        // (d) trailing tiles (i, j), p < j <= i, numbered row by row
+        // wave 0: the next diagonal tile's update, then its pivots; waves 1-3
+        // meanwhile: the other trailing tiles and block row p of L^-1 (all
+        // disjoint from the next diagonal block)
         const int nt = PB / SB - 1 - p;
-        for (int q = w; q < nt * (nt + 1) / 2; q += 4) {
-            int a, b;
-            decode_lower(q, a, b);
-            const int i = p + 1 + a, j = p + 1 + b;
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][o + 4 * kk + lk],
-                                                           Am[16 * j + lr][o + 4 * kk + lk], acc,
-                                                           0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][16 * j + lr] -= acc[r];
-        }
-        __syncthreads();
-    }
-    // X = L^-1 by distance from the diagonal; X_ip (i > p) is stored
-    // transposed in Am's upper block (p, i): Am[16p + b][16i + a] = X_ip[a][b]
-    for (int d = 1; d < PB / SB; ++d) {
-        for (int p = w; p + d < PB / SB; p += 4) {
-            const int i = p + d;
-            d4 sacc = {0.0, 0.0, 0.0, 0.0};
-            for (int k = p; k < i; ++k) {
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double xb = (k == p) ? Dv[p][4 * kk + lk][lr]
-                                               : Am[16 * p + lr][16 * k + 4 * kk + lk];
-                    sacc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][16 * k + 4 * kk + lk],
-                                                                xb, sacc, 0, 0, 0);
-                }
+        if (w == 0) {
+            if (nt > 0) {
+                dtile(p, 0);
+                // lanes exchange the updated tile through LDS
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                diag16(p + 1);
             }
-            d4 xacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Dv[i][lr][4 * kk + lk], sacc[kk], xacc,
-                                                            0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Am[16 * p + lr][16 * i + 4 * r + lk] = xacc[r];
+        } else {
+            for (int q = w; q < nt * (nt + 1) / 2; q += 3) dtile(p, q);
+            for (int q = w - 1; q < p; q += 3) xblock(p, q);
         }
         __syncthreads();
     }
