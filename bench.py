@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--V", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--no-kernel-profile", action="store_true",
+                    help="time the FP64 steps without the per-launch events of the roofline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="also run one device-to-device copy of this size (counter scaling)")
     args = ap.parse_args()
@@ -214,7 +216,8 @@ def main():
     log("building workload")
     v, e, cs, pen = make_workload(args.V)
     ctx = L.Context(e, len(v))
-    r64 = run_form(ctx, v, cs, pen, "fp64", args.steps, args.warmup, kprof=True)
+    r64 = run_form(ctx, v, cs, pen, "fp64", args.steps, args.warmup,
+                   kprof=not args.no_kernel_profile)
     rmx, acc = None, {}
     if not args.fp64_only:
         try:

@@ -18,8 +18,10 @@
 //   k_panel_trsm  L_ik = A_ik L_kk^-T as a GEMM with the stored inverse
 //   k_syrk128     A_ij -= L_ik L_jk^T on 128x128 lower tiles, K = 128,
 //                 v_mfma_f64_16x16x4_f64, XCD-grouped tile order, on a second
-//                 stream (k_syrk_col updates the next panel's column first, so
-//                 the next panel's factorisation overlaps the rest)
+//                 stream; the next panel's column is updated separately
+//                 (k_col_diag for its diagonal tile on the critical path,
+//                 k_col_rows below it on the second stream), so the next
+//                 panel's factorisation overlaps the big update
 // Mixed form, 64-column panels:
 //   k_potrf_panel<float>  every block factors the 64x64 diagonal block in
 //                 FP64 in LDS and inverts it; block 0 stores L_kk^-1, the
@@ -411,24 +413,24 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
     }
 }
 
-// acc += A B^T for an MT x 128 block of C (MT = 128 or 64), K = 128:
-// A is MT x K and B is 128 x K, both row-major (leading dims lda, ldb).
-// 4 waves in a 2 x 2 grid, each an (MT/2) x 64 sub-tile of (MT/32) x 4
-// v_mfma_f64_16x16x4_f64 tiles. K is staged through LDS in chunks of KC,
-// k-major (so the 16 rows an MFMA operand reads are contiguous), double
+// acc += A B^T for an MT x NC block of C (MT, NC = 128 or 64), K = 128:
+// A is MT x K and B is NC x K, both row-major (leading dims lda, ldb).
+// 4 waves in a 2 x 2 grid, each an (MT/2) x (NC/2) sub-tile of (MT/32) x
+// (NC/32) v_mfma_f64_16x16x4_f64 tiles. K is staged through LDS in chunks of
+// KC, k-major (so the 16 rows an MFMA operand reads are contiguous), double
 // buffered with the next chunk's global loads in flight during the MFMAs.
 // MFMA operand layout: A[lane%16][lane/16], B[lane/16][lane%16];
 // result element r of a lane is D[4r + lane/16][lane%16].
-template <int MT, int KC>
+template <int MT, int KC, int NC = PB>
 __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int lda,
                                             const double* __restrict__ B, int ldb,
                                             double (*As)[KC][LDT], double (*Bs)[KC][LDT],
-                                            d4 (&acc)[MT / 32][4]) {
-    constexpr int TA = MT / 32;
+                                            d4 (&acc)[MT / 32][NC / 32]) {
+    constexpr int TA = MT / 32, TB = NC / 32;
     constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk
-    constexpr int NBv = PB * KC / 256;  // B doubles per thread per chunk
+    constexpr int NBv = NC * KC / 256;  // B doubles per thread per chunk
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wr = (w >> 1) * (MT / 2), wc = (w & 1) * 64;
+    const int wr = (w >> 1) * (MT / 2), wc = (w & 1) * (NC / 2);
     const int arow = t / (KC / NA), acol = (t % (KC / NA)) * NA;
     const int brow = t / (KC / NBv), bcol = (t % (KC / NBv)) * NBv;
     const double* pa = A + (size_t)arow * lda + acol;
@@ -463,15 +465,15 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
         if (ch + 1 < PB / KC) gload((ch + 1) * KC);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
-            double av[TA], bv[4];
+            double av[TA], bv[TB];
 #pragma unroll
             for (int a = 0; a < TA; ++a) av[a] = As[buf][kk + lk][wr + 16 * a + lr];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = Bs[buf][kk + lk][wc + 16 * b + lr];
+            for (int b = 0; b < TB; ++b) bv[b] = Bs[buf][kk + lk][wc + 16 * b + lr];
 #pragma unroll
             for (int a = 0; a < TA; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < TB; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
         // keep the next chunk's LDS stores (and their wait on the global
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
 // Trailing update A_ij -= L_ik L_jk^T over 128x128 lower tiles of the
 // trailing matrix (nt trailing panels). skip_first_col = 0: all nt (nt+1)/2
 // tiles; 1: the (nt-1) nt / 2 tiles right of the first trailing column (that
-// column, the next panel, is updated separately by k_syrk_col). Tiles are
+// column, the next panel, is updated by k_col_diag / k_col_rows). Tiles are
 // numbered row by row; workgroup b runs on XCD b % 8, so each XCD is given a
 // contiguous range of tile numbers (tiles of one row share their A panel in
 // that XCD's L2).
@@ -550,12 +552,13 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
 }
 
 // Update of the next panel's column (the first trailing column) with panel
-// kb, on the factorisation's critical path: 64 x 128 blocks (two per 128-row
-// tile, nt tiles), so that twice as many workgroups share the work.
-__global__ __launch_bounds__(256, 2) void k_syrk_col(double* __restrict__ F, int Vp, int kb) {
+// kb, below its diagonal tile: tiles (i, kb+1), i > kb+1, as 64 x 128 blocks
+// (two per 128-row tile, nt - 1 tiles). Runs beside the next panel's
+// diagonal factorisation, which only needs the diagonal tile (k_col_diag).
+__global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
-    const int ti = blockIdx.x >> 1, h = blockIdx.x & 1;
+    const int ti = 1 + (blockIdx.x >> 1), h = blockIdx.x & 1;
     const size_t k0 = (size_t)kb * PB;
     const size_t ri = k0 + PB + (size_t)ti * PB + 64 * h, rj = k0 + PB;
     d4 acc[2][4];
@@ -571,10 +574,37 @@ __global__ __launch_bounds__(256, 2) void k_syrk_col(double* __restrict__ F, int
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
+            for (int r = 0; r < 4; ++r)
+                F[(ri + wr + 16 * a + 4 * r + (lane >> 4)) * Vp + rj + wc + 16 * b + (lane & 15)] -=
+                    acc[a][b][r];
+}
+
+// Update of the next panel's 128 x 128 diagonal tile with panel kb, on the
+// factorisation's critical path: its three lower 64 x 64 blocks, one
+// workgroup each (block (hi, hj) = (0,0), (1,0), (1,1)).
+__global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int Vp, int kb) {
+    __shared__ double As[2][KC_TRSM][LDT];
+    __shared__ double Bs[2][KC_TRSM][LDT];
+    const int hi = blockIdx.x == 0 ? 0 : 1, hj = blockIdx.x == 2 ? 1 : 0;
+    const size_t k0 = (size_t)kb * PB;
+    const size_t ri = k0 + PB + 64 * hi, rj = k0 + PB + 64 * hj;
+    d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    gemm_nt_f64<64, KC_TRSM, 64>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
                 const int j = wc + 16 * b + (lane & 15);
-                if (ti != 0 || j <= i + 64 * h) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
+                if (hi != hj || j <= i) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
             }
 }
 
@@ -801,19 +831,21 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 
 // FP64 form: 128-column panels (diagonal factor + inverse, panel TRSM as a
 // GEMM with the inverse, MFMA trailing update) with a one-panel lookahead on
-// two streams: the main stream runs potrf(kb) -> panel TRSM(kb) -> update of
-// column kb+1 (k_syrk_col) while the second stream updates the rest of the
-// trailing matrix with panel kb (k_syrk128, skipping that column), so the
-// next panel's factorisation overlaps the big update. Ordering: the rest of
-// update kb waits for panel kb (event T); the column update kb waits for the
-// rest of update kb-1 (event B, they would otherwise touch the same tiles);
-// the main stream waits for the last rest-update before returning. A
-// non-positive pivot sets status bit 1 (singular saddle system).
+// two streams. Main stream, per panel kb: potrf(kb) -> panel TRSM(kb) ->
+// update of the next diagonal tile (k_col_diag), the critical path. Second
+// stream: the rest of column kb+1 (k_col_rows) and then the rest of the
+// trailing matrix (k_syrk128), both with panel kb, overlapping the next
+// panel's diagonal factorisation. Ordering by events: T (panel kb final)
+// before the second stream's updates of kb; C (column kb+1 below its
+// diagonal tile done) before panel TRSM(kb+1); B (rest-update kb done)
+// before k_col_diag(kb+1), which would otherwise touch a tile it is still
+// updating. The main stream waits for the last rest-update before returning.
+// A non-positive pivot sets status bit 1 (singular saddle system).
 rc_status factor_loop_fp64(rc_ctx* ctx) {
     const int Vp = ctx->nblk * NB;
     const int np = Vp / PB;
-    if ((int)ctx->dep_ev.size() < 2 * np + 1) {
-        while ((int)ctx->dep_ev.size() < 2 * np + 1) {
+    if ((int)ctx->dep_ev.size() < 3 * np + 2) {
+        while ((int)ctx->dep_ev.size() < 3 * np + 2) {
             // completion signals only: each kernel's own end-of-kernel release
             // makes its writes visible device-wide, so no extra fence here
             hipEvent_t e;
@@ -821,9 +853,11 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
             ctx->dep_ev.push_back(e);
         }
     }
-    hipEvent_t* T = ctx->dep_ev.data();        // panel kb final
-    hipEvent_t* Bv = ctx->dep_ev.data() + np;  // rest-update kb done
-    hipEvent_t start = ctx->dep_ev[2 * np];
+    hipEvent_t* T = ctx->dep_ev.data();            // panel kb final
+    hipEvent_t* Bv = ctx->dep_ev.data() + np;      // rest-update kb done
+    hipEvent_t* Cv = ctx->dep_ev.data() + 2 * np;  // column kb+1 below its diagonal tile done
+    hipEvent_t start = ctx->dep_ev[3 * np];
+    hipEvent_t fin = ctx->dep_ev[3 * np + 1];
     // the second stream starts after the factor's input is prepared
     RC_HIP(hipEventRecord(start, ctx->stream));
     RC_HIP(hipStreamWaitEvent(ctx->stream2, start, 0));
@@ -851,20 +885,29 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
         RC_CHECK_LAUNCH();
         const int nt = np - kb - 1;
         if (nt == 0) break;
+        // panel kb below its diagonal block: needs column kb below the
+        // diagonal tile (k_col_rows of panel kb-1, second stream)
+        if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Cv[kb - 1], 0));
         hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
                            (const double*)ctx->d_Linv128);
         RC_CHECK_LAUNCH();
         RC_HIP(hipEventRecord(T[kb], ctx->stream));
-        // column kb+1: after the rest-update of panel kb-1 has left it
+        // next diagonal tile: after the rest-update of panel kb-1 has left it
         if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[kb - 1], 0));
-        const double m = (double)nt * PB;
-        hipLaunchKernelGGL(k_syrk_col, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb);
+        hipLaunchKernelGGL(k_col_diag, dim3(3), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb);
         RC_CHECK_LAUNCH();
-        // the rest of the trailing matrix on the second stream
+        // second stream: the rest of column kb+1, then the rest of the
+        // trailing matrix, both with panel kb
         RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
+        if (nt > 1) {
+            hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1)), dim3(256), 0, ctx->stream2, ctx->d_F,
+                               Vp, kb);
+            RC_CHECK_LAUNCH();
+        }
+        RC_HIP(hipEventRecord(Cv[kb], ctx->stream2));
         const int ntiles = (nt - 1) * nt / 2;
         if (ntiles > 0) {
-            const double mr = m - PB;
+            const double mr = (double)(nt - 1) * PB;
             hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
             hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream2,
                                ctx->d_F, Vp, kb, ntiles, 1);
@@ -873,7 +916,10 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
         }
         RC_HIP(hipEventRecord(Bv[kb], ctx->stream2));
     }
-    if (np >= 2) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[np - 2], 0));
+    // the main stream continues after all second-stream work (this also
+    // joins the second stream back when the loop is captured into a graph)
+    RC_HIP(hipEventRecord(fin, ctx->stream2));
+    RC_HIP(hipStreamWaitEvent(ctx->stream, fin, 0));
     return RC_OK;
 }
 
@@ -890,7 +936,33 @@ rc_status rc_factor_fp64(rc_ctx* ctx) {
     hipLaunchKernelGGL(k_factor_prep<double>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
                        ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, 0.0, ctx->d_F);
     RC_CHECK_LAUNCH();
-    return factor_loop_fp64(ctx);
+    if (ctx->kprof) return factor_loop_fp64(ctx);   // per-launch timing events: eager launches
+    // The panel loop (3 * V/128 launches on two streams plus their events,
+    // identical for every step of a context) is captured once as a HIP graph
+    // and replayed, so its launches are submitted in one go.
+    if (!ctx->fgraph_exec) {
+        RC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc_status s = factor_loop_fp64(ctx);
+        hipGraph_t g = nullptr;
+        const hipError_t he = hipStreamEndCapture(ctx->stream, &g);
+        if (s != RC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        if (he != hipSuccess) {
+            ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(he);
+            return RC_HIP_ERROR;
+        }
+        const hipError_t hi = hipGraphInstantiate(&ctx->fgraph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (hi != hipSuccess) {
+            ctx->fgraph_exec = nullptr;
+            ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(hi);
+            return RC_HIP_ERROR;
+        }
+    }
+    RC_HIP(hipGraphLaunch(ctx->fgraph_exec, ctx->stream));
+    return RC_OK;
 }
 
 // Factor S + delta I, S = D^-1/2 A_hat D^-1/2 (unit diagonal), into
