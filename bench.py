@@ -165,6 +165,40 @@ def cpu_baseline():
     }
 
 
+# Algorithmic FP64 work per ordered disjoint edge pair (sqrt, divide and
+# reciprocal counted as one flop): energy ~150, adjoint gradient ~480, Gram
+# assembly ~250 (the executed instruction counts of the kernels, Newton
+# refinements included, are higher: ~188 for the energy and ~342 for the
+# adjoint).
+FLOP_PER_PAIR = {"energy": 150.0, "dE": 480.0, "assembleA": 250.0}
+
+
+def kernel_rates(v, e, phase_fp64, ctx):
+    """Per-kernel milliseconds (HIP events of the timed FP64 steps for dE and
+    assembleA; host-timed rc_energy calls for a full energy evaluation) and the
+    rates they imply: algorithmic TFLOP/s and, for the assembly, the GB/s of
+    its V x V FP64 write."""
+    from rcurves import _lib as L
+    from rcurves import core
+    E, V = len(e), len(v)
+    P = E * (E - 3)   # ordered disjoint pairs of a closed curve
+    p = core._params(3.0, 6.0, 1e-10)
+    out = C.c_double()
+    vv = np.ascontiguousarray(v)
+    ctx.check(L.lib().rc_energy(ctx.handle, L.dptr(vv), C.byref(p), C.byref(out)))
+    t = time.perf_counter()
+    for _ in range(5):
+        ctx.check(L.lib().rc_energy(ctx.handle, L.dptr(vv), C.byref(p), C.byref(out)))
+    en_ms = (time.perf_counter() - t) / 5 * 1e3
+    res = {"ordered_pairs": P}
+    for k, ms in (("energy", en_ms), ("dE", phase_fp64["dE"]), ("assembleA", phase_fp64["assembleA"])):
+        res[k] = {"ms": round(ms, 4),
+                  "tflops_algorithmic": round(P * FLOP_PER_PAIR[k] / (ms * 1e-3) / 1e12, 2)}
+    res["energy"]["timing"] = "host, 5 rc_energy calls incl. 196 KB position upload"
+    res["assembleA"]["write_gbs"] = round(8.0 * V * V / (res["assembleA"]["ms"] * 1e-3) / 1e9, 1)
+    return res
+
+
 def measured_traffic():
     """HBM bytes per launch of k_syrk128 from the committed counter summary
     (FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy), or None."""
@@ -258,6 +292,7 @@ def main():
                    "trials_per_step": r64["trials"], **acc},
         "roofline": roof,
     }
+    out["config"]["kernels"] = kernel_rates(v, e, r64["phase"], ctx)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline()
