@@ -42,11 +42,37 @@ __device__ __forceinline__ void tri_decode(int p, int& a, int& b) {
     a = p - bb * (bb + 1) / 2;
 }
 
+// T_I x (p_a - q_b) does not depend on the endpoint a (the two differences
+// differ by ell_I T_I), and T_J x (p_a - q_b) not on b: 2 + 2 cross-product
+// norms per pair instead of 8 (equal to the per-endpoint form to rounding).
 template <class P>
 __device__ __forceinline__ void pair_weights(const P& pw, double eps, const double* PI,
                                              const double* uI, double ellI, const double* PJ,
                                              const double* uJ, double ellJ, double& w2,
                                              double& X, double& Y) {
+    double n2I[2], n2J[2];
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const double dx = PI[0] - PJ[3 * jb];
+        const double dy = PI[1] - PJ[3 * jb + 1];
+        const double dz = PI[2] - PJ[3 * jb + 2];
+        const double cx = uI[1] * dz - uI[2] * dy;
+        const double cy = uI[2] * dx - uI[0] * dz;
+        const double cz = uI[0] * dy - uI[1] * dx;
+        const double n = rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps;
+        n2I[jb] = n * n;
+    }
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+        const double dx = PI[3 * ia] - PJ[0];
+        const double dy = PI[3 * ia + 1] - PJ[1];
+        const double dz = PI[3 * ia + 2] - PJ[2];
+        const double cx = uJ[1] * dz - uJ[2] * dy;
+        const double cy = uJ[2] * dx - uJ[0] * dz;
+        const double cz = uJ[0] * dy - uJ[1] * dx;
+        const double n = rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps;
+        n2J[ia] = n * n;
+    }
     double sumD = 0.0, sumQI = 0.0, sumQJ = 0.0;
 #pragma unroll
     for (int ia = 0; ia < 2; ++ia) {
@@ -60,17 +86,9 @@ __device__ __forceinline__ void pair_weights(const P& pw, double eps, const doub
             const double inv = rcm::rcp(deps);
             const double i2 = inv * inv;
             const double inv4pd = i2 * i2 * pd;
-            const double cIx = uI[1] * dz - uI[2] * dy;
-            const double cIy = uI[2] * dx - uI[0] * dz;
-            const double cIz = uI[0] * dy - uI[1] * dx;
-            const double cJx = uJ[1] * dz - uJ[2] * dy;
-            const double cJy = uJ[2] * dx - uJ[0] * dz;
-            const double cJz = uJ[0] * dy - uJ[1] * dx;
-            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
-            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
             sumD += pd;
-            sumQI += nI * nI * inv4pd;
-            sumQJ += nJ * nJ * inv4pd;
+            sumQI += n2I[jb] * inv4pd;
+            sumQJ += n2J[ia] * inv4pd;
         }
     }
     const double ll = ellI * ellJ;

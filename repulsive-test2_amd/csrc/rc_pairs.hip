@@ -15,6 +15,7 @@
 // no atomics and the sum order is fixed (bit-reproducible run to run).
 #include "rc_internal.h"
 #include "rc_math.h"
+#include "rc_pair_energy.h"
 
 namespace {
 
@@ -77,35 +78,6 @@ __global__ void k_edge_prep(const double* __restrict__ pos, int V, const int32_t
 // ---------------------------------------------------------------------------
 // energy
 
-template <class P>
-__device__ __forceinline__ double pair_energy(const P& pw, double eps, const double* PI,
-                                              const double* PJ, double eIx, double eIy,
-                                              double eIz, double eJx, double eJy, double eJz,
-                                              double ellI, double ellJ, double lpI, double lpJ) {
-    double sFI = 0.0, sFJ = 0.0;
-#pragma unroll
-    for (int ia = 0; ia < 2; ++ia) {
-#pragma unroll
-        for (int jb = 0; jb < 2; ++jb) {
-            const double dx = PI[3 * ia] - PJ[3 * jb];
-            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
-            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
-            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
-            const double invB = pw.inv_beta(deps, rcm::rcp(deps));
-            const double cIx = eIy * dz - eIz * dy;
-            const double cIy = eIz * dx - eIx * dz;
-            const double cIz = eIx * dy - eIy * dx;
-            const double cJx = eJy * dz - eJz * dy;
-            const double cJy = eJz * dx - eJx * dz;
-            const double cJz = eJx * dy - eJy * dx;
-            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
-            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
-            sFI += pw.c_alpha(nI) * invB;
-            sFJ += pw.c_alpha(nJ) * invB;
-        }
-    }
-    return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
-}
 
 template <class P>
 __global__ __launch_bounds__(256) void k_energy(EdgeSoA ed, int E, int npairs, P pw, double eps,
@@ -134,7 +106,7 @@ __global__ __launch_bounds__(256) void k_energy(EdgeSoA ed, int E, int npairs, P
             const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
                                i2 != j2;
             const double eJx = PJ[3] - PJ[0], eJy = PJ[4] - PJ[1], eJz = PJ[5] - PJ[2];
-            const double e = pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, eJx, eJy, eJz, ellI,
+            const double e = rcpair::pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, eJx, eJy, eJz, ellI,
                                          ed.ell[J], lpI, ed.lpow[J]);
             sum += valid ? e : 0.0;
         }
@@ -162,43 +134,32 @@ __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // gradient
 
-struct KD {
-    double f, ddx, ddy, ddz, dex, dey, dez;
+// Cross-product part of kernelDerivs (tangentPointEnergy.ts:165-244) for
+// f(e,d) = (|e x d|+eps)^a / (|d|+eps)^b: c = e x d, cA = (|c|+eps)^a, the
+// factor K = a (|c|+eps)^(a-1) / |c| (0 when |c| < 1e-14) of the c-derivative,
+// and cce = c x e. With the distance parts invB = deps^-b and
+// invB1 = deps^-(b+1) the derivatives are
+//   df/dd = K invB cce - b cA invB1 d/|d|,    df/de = K invB (d x c).
+struct CrossPart {
+    double c[3], cce[3], cA, K;
 };
 
-// kernelDerivs (tangentPointEnergy.ts:165-244) for f(e,d) = (|e x d|+eps)^a/(|d|+eps)^b
-// given the shared distance parts dh (guarded unit d), invB = deps^-b,
-// invB1 = deps^-(b+1).
 template <class P>
-__device__ __forceinline__ KD kd(const P& pw, double eps, double alpha, double beta, double ex,
-                                 double ey, double ez, double dx, double dy, double dz, double dhx,
-                                 double dhy, double dhz, double invB, double invB1) {
-    KD r;
-    const double cx = ey * dz - ez * dy;
-    const double cy = ez * dx - ex * dz;
-    const double cz = ex * dy - ey * dx;
+__device__ __forceinline__ CrossPart cross_part(const P& pw, double eps, double alpha, double ex,
+                                                double ey, double ez, double dx, double dy,
+                                                double dz) {
+    CrossPart r;
+    r.c[0] = ey * dz - ez * dy;
+    r.c[1] = ez * dx - ex * dz;
+    r.c[2] = ex * dy - ey * dx;
     double rc, invRc;
-    rcm::sqrt_rsqrt(cx * cx + cy * cy + cz * cz, rc, invRc);
+    rcm::sqrt_rsqrt(r.c[0] * r.c[0] + r.c[1] * r.c[1] + r.c[2] * r.c[2], rc, invRc);
     const double ceps = rc + eps;
-    const double cA = pw.c_alpha(ceps);
-    const double cA1 = pw.c_alpha_m1(ceps);
-    r.f = cA * invB;
-    const double coeff_c = alpha * cA1 * invB;
-    const double coeff_d = -beta * cA * invB1;
-    const double k = (rc >= 1e-14) ? coeff_c * invRc : 0.0;
-    // cross3D(c, e) and cross3D(d, c)
-    const double ccex = cy * ez - cz * ey;
-    const double ccey = cz * ex - cx * ez;
-    const double ccez = cx * ey - cy * ex;
-    const double dccx = dy * cz - dz * cy;
-    const double dccy = dz * cx - dx * cz;
-    const double dccz = dx * cy - dy * cx;
-    r.ddx = k * ccex + coeff_d * dhx;
-    r.ddy = k * ccey + coeff_d * dhy;
-    r.ddz = k * ccez + coeff_d * dhz;
-    r.dex = k * dccx;
-    r.dey = k * dccy;
-    r.dez = k * dccz;
+    r.cA = pw.c_alpha(ceps);
+    r.K = (rc >= 1e-14) ? alpha * pw.c_alpha_m1(ceps) * invRc : 0.0;
+    r.cce[0] = r.c[1] * ez - r.c[2] * ey;
+    r.cce[1] = r.c[2] * ex - r.c[0] * ez;
+    r.cce[2] = r.c[0] * ey - r.c[1] * ex;
     return r;
 }
 
@@ -212,6 +173,11 @@ struct EdgeRegs {
 
 // Contributions of one unordered disjoint pair {I,J} (both ordered terms)
 // to the four endpoint slots: cI[0..2] -> I.i1, cI[3..5] -> I.i2, same for J.
+// e_I x (p_i - q_j) does not depend on the endpoint i of I (the two
+// differences differ by e_I), so the I-ordered terms share one cross part
+// per endpoint j of J, and the J-ordered terms one per endpoint i of I; their
+// e-derivatives over both endpoints sum to one cross product each:
+// sum_i K invB_ij (d_ij x c_j) = K (sum_i invB_ij d_ij) x c_j.
 template <class P>
 __device__ __forceinline__ void pair_grad(const P& pw, double eps, double alpha, double beta,
                                           const EdgeRegs& I, const EdgeRegs& J, double* cI,
@@ -220,64 +186,96 @@ __device__ __forceinline__ void pair_grad(const P& pw, double eps, double alpha,
     // by the ordered-pair bases b_IJ = ell_I^(1-a) ell_J / 4, b_JI likewise.
     const double bIJ = 0.25 * I.lp * J.ell;
     const double bJI = 0.25 * J.lp * I.ell;
+    double d[2][2][3], dh[2][2][3], invB[2][2], invB1[2][2];
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) d[ia][jb][c] = I.P[3 * ia + c] - J.P[3 * jb + c];
+            double rd, invRd;
+            rcm::sqrt_rsqrt(d[ia][jb][0] * d[ia][jb][0] + d[ia][jb][1] * d[ia][jb][1] +
+                                d[ia][jb][2] * d[ia][jb][2],
+                            rd, invRd);
+            const double hs = (rd < 1e-14) ? 0.0 : invRd;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dh[ia][jb][c] = hs * d[ia][jb][c];
+            const double deps = rd + eps;
+            const double inv = rcm::rcp(deps);
+            invB[ia][jb] = pw.inv_beta(deps, inv);
+            invB1[ia][jb] = invB[ia][jb] * inv;
+        }
+    }
+    // I-ordered terms, tangent e_I, d = p_i - q_j: one cross part per j
+    CrossPart xi[2];
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb)
+        xi[jb] = cross_part(pw, eps, alpha, I.e[0], I.e[1], I.e[2], d[0][jb][0], d[0][jb][1],
+                            d[0][jb][2]);
+    // J-ordered terms, tangent e_J, d' = q_j - p_i = -d: one cross part per i
+    CrossPart xj[2];
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia)
+        xj[ia] = cross_part(pw, eps, alpha, J.e[0], J.e[1], J.e[2], -d[ia][0][0], -d[ia][0][1],
+                            -d[ia][0][2]);
     double sFI = 0.0, sFJ = 0.0;
+    double deI[3] = {0.0, 0.0, 0.0}, deJ[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         cI[c] = 0.0;
         cJ[c] = 0.0;
     }
 #pragma unroll
-    for (int ia = 0; ia < 2; ++ia) {
+    for (int jb = 0; jb < 2; ++jb) {
+        const CrossPart& x = xi[jb];
+        double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int jb = 0; jb < 2; ++jb) {
-            const double dx = I.P[3 * ia] - J.P[3 * jb];
-            const double dy = I.P[3 * ia + 1] - J.P[3 * jb + 1];
-            const double dz = I.P[3 * ia + 2] - J.P[3 * jb + 2];
-            double rd, invRd;
-            rcm::sqrt_rsqrt(dx * dx + dy * dy + dz * dz, rd, invRd);
-            const double hs = (rd < 1e-14) ? 0.0 : invRd;
-            const double dhx = hs * dx, dhy = hs * dy, dhz = hs * dz;
-            const double deps = rd + eps;
-            const double inv = rcm::rcp(deps);
-            const double invB = pw.inv_beta(deps, inv);
-            const double invB1 = invB * inv;
-            {
-                // ordered term (I, J): d = p_i - q_j, tangent e_I
-                const KD a = kd(pw, eps, alpha, beta, I.e[0], I.e[1], I.e[2], dx, dy, dz, dhx,
-                                dhy, dhz, invB, invB1);
-                sFI += a.f;
-                cI[3 * ia + 0] += bIJ * a.ddx;
-                cI[3 * ia + 1] += bIJ * a.ddy;
-                cI[3 * ia + 2] += bIJ * a.ddz;
-                cJ[3 * jb + 0] -= bIJ * a.ddx;
-                cJ[3 * jb + 1] -= bIJ * a.ddy;
-                cJ[3 * jb + 2] -= bIJ * a.ddz;
-                cI[0] -= bIJ * a.dex; cI[1] -= bIJ * a.dey; cI[2] -= bIJ * a.dez;
-                cI[3] += bIJ * a.dex; cI[4] += bIJ * a.dey; cI[5] += bIJ * a.dez;
-            }
-            {
-                // ordered term (J, I): d = q_j - p_i, tangent e_J
-                const KD c = kd(pw, eps, alpha, beta, J.e[0], J.e[1], J.e[2], -dx, -dy, -dz,
-                                -dhx, -dhy, -dhz, invB, invB1);
-                sFJ += c.f;
-                cJ[3 * jb + 0] += bJI * c.ddx;
-                cJ[3 * jb + 1] += bJI * c.ddy;
-                cJ[3 * jb + 2] += bJI * c.ddz;
-                cI[3 * ia + 0] -= bJI * c.ddx;
-                cI[3 * ia + 1] -= bJI * c.ddy;
-                cI[3 * ia + 2] -= bJI * c.ddz;
-                cJ[0] -= bJI * c.dex; cJ[1] -= bJI * c.dey; cJ[2] -= bJI * c.dez;
-                cJ[3] += bJI * c.dex; cJ[4] += bJI * c.dey; cJ[5] += bJI * c.dez;
+        for (int ia = 0; ia < 2; ++ia) {
+            sFI += x.cA * invB[ia][jb];
+            const double kk = x.K * invB[ia][jb];
+            const double cd = -beta * x.cA * invB1[ia][jb];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double dd = kk * x.cce[c] + cd * dh[ia][jb][c];
+                cI[3 * ia + c] += bIJ * dd;
+                cJ[3 * jb + c] -= bIJ * dd;
+                s[c] += invB[ia][jb] * d[ia][jb][c];
             }
         }
+        // K (sum_i invB_ij d_ij) x c_j
+        deI[0] += x.K * (s[1] * x.c[2] - s[2] * x.c[1]);
+        deI[1] += x.K * (s[2] * x.c[0] - s[0] * x.c[2]);
+        deI[2] += x.K * (s[0] * x.c[1] - s[1] * x.c[0]);
     }
-    // length derivatives: d(ell_I^(1-a)) and d(ell_I) carry -u_I at i1, +u_I at i2
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+        const CrossPart& x = xj[ia];
+        double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            sFJ += x.cA * invB[ia][jb];
+            const double kk = x.K * invB[ia][jb];
+            const double cd = -beta * x.cA * invB1[ia][jb];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double dd = kk * x.cce[c] - cd * dh[ia][jb][c];
+                cJ[3 * jb + c] += bJI * dd;
+                cI[3 * ia + c] -= bJI * dd;
+                s[c] -= invB[ia][jb] * d[ia][jb][c];
+            }
+        }
+        deJ[0] += x.K * (s[1] * x.c[2] - s[2] * x.c[1]);
+        deJ[1] += x.K * (s[2] * x.c[0] - s[0] * x.c[2]);
+        deJ[2] += x.K * (s[0] * x.c[1] - s[1] * x.c[0]);
+    }
+    // e = b - a: de/da = -1, de/db = +1; length derivatives d(ell_I^(1-a))
+    // and d(ell_I) carry -u_I at i1, +u_I at i2
     const double gIs = 0.25 * J.ell * sFI * I.dpc + 0.25 * J.lp * sFJ;
     const double gJs = 0.25 * I.lp * sFI + 0.25 * I.ell * sFJ * J.dpc;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double gI = gIs * I.u[c];
-        const double gJ = gJs * J.u[c];
+        const double gI = gIs * I.u[c] + bIJ * deI[c];
+        const double gJ = gJs * J.u[c] + bJI * deJ[c];
         cI[c] -= gI;
         cI[3 + c] += gI;
         cJ[c] -= gJ;

@@ -19,6 +19,7 @@
 
 #include "rc_internal.h"
 #include "rc_math.h"
+#include "rc_pair_energy.h"
 #include "rc_step.h"
 
 namespace {
@@ -735,9 +736,8 @@ __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restri
 
 }  // namespace
 
-// The energy kernel with Armijo decision lives here so it can share the
-// pair_energy device code through a local copy (identical arithmetic to
-// rc_pairs.hip's forward energy).
+// The energy kernel with Armijo decision (pair arithmetic shared with the
+// forward energy through rc_pair_energy.h).
 namespace {
 
 __device__ __forceinline__ void tri_decode_e(int p, int& a, int& b) {
@@ -748,36 +748,6 @@ __device__ __forceinline__ void tri_decode_e(int p, int& a, int& b) {
     a = p - bb * (bb + 1) / 2;
 }
 
-template <class P>
-__device__ __forceinline__ double pair_energy_ls(const P& pw, double eps, const double* PI,
-                                                 const double* PJ, double eIx, double eIy,
-                                                 double eIz, double eJx, double eJy, double eJz,
-                                                 double ellI, double ellJ, double lpI,
-                                                 double lpJ) {
-    double sFI = 0.0, sFJ = 0.0;
-#pragma unroll
-    for (int ia = 0; ia < 2; ++ia) {
-#pragma unroll
-        for (int jb = 0; jb < 2; ++jb) {
-            const double dx = PI[3 * ia] - PJ[3 * jb];
-            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
-            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
-            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
-            const double invB = pw.inv_beta(deps, rcm::rcp(deps));
-            const double cIx = eIy * dz - eIz * dy;
-            const double cIy = eIz * dx - eIx * dz;
-            const double cIz = eIx * dy - eIy * dx;
-            const double cJx = eJy * dz - eJz * dy;
-            const double cJy = eJz * dx - eJx * dz;
-            const double cJz = eJx * dy - eJy * dx;
-            const double nI = rcm::sqrt_(cIx * cIx + cIy * cIy + cIz * cIz) + eps;
-            const double nJ = rcm::sqrt_(cJx * cJx + cJy * cJy + cJz * cJz) + eps;
-            sFI += pw.c_alpha(nI) * invB;
-            sFJ += pw.c_alpha(nJ) * invB;
-        }
-    }
-    return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
-}
 
 template <class P>
 __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs, P pw,
@@ -809,7 +779,7 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
             const int j1 = ed.i1[J], j2 = ed.i2[J];
             const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
                                i2 != j2;
-            const double e = pair_energy_ls(pw, eps, PI, PJ, eIx, eIy, eIz, PJ[3] - PJ[0],
+            const double e = rcpair::pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, PJ[3] - PJ[0],
                                             PJ[4] - PJ[1], PJ[5] - PJ[2], ellI, ed.ell[J], lpI,
                                             ed.lpow[J]);
             sum += valid ? e : 0.0;
