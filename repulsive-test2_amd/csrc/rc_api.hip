@@ -10,8 +10,8 @@
 #include "rc_internal.h"
 #include "rc_step.h"
 
-rc_status rc_factor_fp64(rc_ctx* ctx);
-rc_status rc_solve_fp64(rc_ctx* ctx, double* d_B, int nrhs);
+rc_status rc_factor_fp64(rc_ctx* ctx, int nx);
+rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs);
 rc_status rc_factor_mixed(rc_ctx* ctx);
 rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, double tol,
                          int max_iter, double* d_refine_out);
@@ -20,7 +20,7 @@ rc_status rc_launch_constraint_jacobian(rc_ctx* ctx, const DevSet* d_set, int k)
 rc_status rc_launch_penalty_grad(rc_ctx* ctx, const DevPen& pen, const double* gsrc, int Vsrc,
                                  double* gdst, int Vdst);
 rc_status rc_launch_penalty_energy(rc_ctx* ctx, const DevPen& pen, double* d_out);
-rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B);
+rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B, int ncols);
 rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k, double* gt,
                           double* d_dots, double* d_bsum, double* d_sol, double* d_Minv);
 rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
@@ -257,7 +257,9 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_rsp, (size_t)ctx->nvt * 2 * ctx->E);
     ALLOC(ctx->d_rs, 2 * (size_t)ctx->E);
     ALLOC(ctx->d_A, (size_t)V * V);
-    ALLOC(ctx->d_F, (size_t)Vp * Vp);
+    // factor plus the right-hand-side rows the factorisation carries along
+    // (3 + 3k <= 3 + 3 RC_KMAX rows, whole 64-row blocks)
+    ALLOC(ctx->d_F, (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
     ALLOC(ctx->d_Linv, (size_t)ctx->nblk * RC_NB * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, ctx->nblk);
@@ -539,7 +541,18 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
         s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
         if (s != RC_OK) return s;
     }
-    s = (precision == RC_PRECISION_MIXED) ? rc_factor_mixed(ctx) : rc_factor_fp64(ctx);
+    const int nrhs = 3 + 3 * k;
+    const int Vp = ctx->nblk * RC_NB;
+    // FP64: the right-hand sides [dE, C^T] ride below the factor as 64-row
+    // blocks, so the factorisation leaves L^-1 B there
+    const int nx = (nrhs + RC_NB - 1) / RC_NB;
+    double* Frhs = ctx->d_F + (size_t)Vp * Vp;
+    if (precision == RC_PRECISION_MIXED) {
+        s = rc_factor_mixed(ctx);
+    } else {
+        s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
+        if (s == RC_OK) s = rc_factor_fp64(ctx, nx);
+    }
     if (s != RC_OK) return s;
     if (ctx->has_isolated) {
         // isolated vertex: zero rows of A and C -> singular saddle (optimizer-sobolev.test.ts:83-105)
@@ -548,18 +561,16 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
                               ctx->stream));
     }
     if (ev_fac1) RC_HIP(hipEventRecord(*ev_fac1, ctx->stream));
-    const int nrhs = 3 + 3 * k;
-    s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs);
-    if (s != RC_OK) return s;
     if (precision == RC_PRECISION_MIXED) {
+        s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs, nrhs);
+        if (s != RC_OK) return s;
         s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, o ? o->refine_tol : 1e-12,
                            o ? o->refine_max_iter : 200, ctx->d_refine);
         if (s != RC_OK) return s;
     } else {
-        const int Vp = ctx->nblk * RC_NB;
-        RC_HIP(hipMemcpyAsync(ctx->d_sol, ctx->d_rhs, sizeof(double) * (size_t)nrhs * Vp,
+        RC_HIP(hipMemcpyAsync(ctx->d_sol, Frhs, sizeof(double) * (size_t)nrhs * Vp,
                               hipMemcpyDeviceToDevice, ctx->stream));
-        s = rc_solve_fp64(ctx, ctx->d_sol, nrhs);
+        s = rc_solve_fp64_backward(ctx, ctx->d_sol, nrhs);
         if (s != RC_OK) return s;
         RC_HIP(hipMemsetAsync(ctx->d_refine, 0, sizeof(double) * 2, ctx->stream));
     }

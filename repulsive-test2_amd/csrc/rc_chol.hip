@@ -553,14 +553,27 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
 
 // Update of the next panel's column (the first trailing column) with panel
 // kb, below its diagonal tile: tiles (i, kb+1), i > kb+1, as 64 x 128 blocks
-// (two per 128-row tile, nt - 1 tiles). Runs beside the next panel's
-// diagonal factorisation, which only needs the diagonal tile (k_col_diag).
-__global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb) {
+// (two per 128-row tile, nt - 1 tiles; workgroups 0 .. 2(nt-1)-1). Runs
+// beside the next panel's diagonal factorisation, which only needs the
+// diagonal tile (k_col_diag). The workgroups after those update the
+// right-hand-side rows below the factor (rows Vp ..): 64-row block x, all nt
+// trailing columns tj, workgroup 2(nt-1) + x nt + tj.
+__global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb,
+                                                     int nt) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
-    const int ti = 1 + (blockIdx.x >> 1), h = blockIdx.x & 1;
     const size_t k0 = (size_t)kb * PB;
-    const size_t ri = k0 + PB + (size_t)ti * PB + 64 * h, rj = k0 + PB;
+    size_t ri, rj;
+    const int nreg = 2 * (nt - 1);
+    if ((int)blockIdx.x < nreg) {
+        const int ti = 1 + (blockIdx.x >> 1), h = blockIdx.x & 1;
+        ri = k0 + PB + (size_t)ti * PB + 64 * h;
+        rj = k0 + PB;
+    } else {
+        const int q = (int)blockIdx.x - nreg;
+        ri = (size_t)Vp + 64 * (size_t)(q / nt);
+        rj = k0 + PB + (size_t)(q % nt) * PB;
+    }
     d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -841,7 +854,12 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 // before k_col_diag(kb+1), which would otherwise touch a tile it is still
 // updating. The main stream waits for the last rest-update before returning.
 // A non-positive pivot sets status bit 1 (singular saddle system).
-rc_status factor_loop_fp64(rc_ctx* ctx) {
+// Right-hand sides: nx blocks of 64 rows stored below the factor (rows Vp ..,
+// B^T) are carried along as extra matrix rows: the panel TRSM of every panel
+// (the last one included) and the column updates of k_col_rows cover them,
+// which leaves Y^T = (L^-1 B)^T in their place, the forward substitution of
+// L L^T X = B done inside the factorisation.
+rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     const int Vp = ctx->nblk * NB;
     const int np = Vp / PB;
     if ((int)ctx->dep_ev.size() < 3 * np + 2) {
@@ -884,14 +902,16 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
                            ctx->d_Linv128, ctx->d_Linv, ctx->d_status);
         RC_CHECK_LAUNCH();
         const int nt = np - kb - 1;
-        if (nt == 0) break;
-        // panel kb below its diagonal block: needs column kb below the
-        // diagonal tile (k_col_rows of panel kb-1, second stream)
+        if (nt == 0 && nx == 0) break;
+        // panel kb below its diagonal block (and in the right-hand-side
+        // rows): needs column kb below the diagonal tile (k_col_rows of
+        // panel kb-1, second stream)
         if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Cv[kb - 1], 0));
-        hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
-                           (const double*)ctx->d_Linv128);
+        hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt + nx), dim3(256), 0, ctx->stream, ctx->d_F, Vp,
+                           kb, (const double*)ctx->d_Linv128);
         RC_CHECK_LAUNCH();
         RC_HIP(hipEventRecord(T[kb], ctx->stream));
+        if (nt == 0) break;
         // next diagonal tile: after the rest-update of panel kb-1 has left it
         if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[kb - 1], 0));
         hipLaunchKernelGGL(k_col_diag, dim3(3), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb);
@@ -899,9 +919,9 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
         // second stream: the rest of column kb+1, then the rest of the
         // trailing matrix, both with panel kb
         RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
-        if (nt > 1) {
-            hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1)), dim3(256), 0, ctx->stream2, ctx->d_F,
-                               Vp, kb);
+        if (2 * (nt - 1) + nx * nt > 0) {
+            hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1) + nx * nt), dim3(256), 0, ctx->stream2,
+                               ctx->d_F, Vp, kb, nt);
             RC_CHECK_LAUNCH();
         }
         RC_HIP(hipEventRecord(Cv[kb], ctx->stream2));
@@ -926,8 +946,10 @@ rc_status factor_loop_fp64(rc_ctx* ctx) {
 }  // namespace
 
 // Factor A_hat = A + mu 11^T into ctx->d_F (FP64). mu = trace(A)/V^2 is kept
-// in d_scal[1]. A non-positive pivot sets status bit 1 (singular).
-rc_status rc_factor_fp64(rc_ctx* ctx) {
+// in d_scal[1]. A non-positive pivot sets status bit 1 (singular). nx 64-row
+// blocks of right-hand sides B^T below the factor (rows Vp .. Vp + 64 nx - 1)
+// are overwritten with (L^-1 B)^T.
+rc_status rc_factor_fp64(rc_ctx* ctx, int nx) {
     const int Vp = ctx->nblk * NB;
     RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
     hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_A, ctx->V,
@@ -936,13 +958,18 @@ rc_status rc_factor_fp64(rc_ctx* ctx) {
     hipLaunchKernelGGL(k_factor_prep<double>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
                        ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, 0.0, ctx->d_F);
     RC_CHECK_LAUNCH();
-    if (ctx->kprof) return factor_loop_fp64(ctx);   // per-launch timing events: eager launches
+    if (ctx->kprof) return factor_loop_fp64(ctx, nx);   // per-launch timing events: eager launches
     // The panel loop (3 * V/128 launches on two streams plus their events,
-    // identical for every step of a context) is captured once as a HIP graph
-    // and replayed, so its launches are submitted in one go.
+    // identical for every step of a context with the same number of
+    // right-hand-side blocks) is captured as a HIP graph and replayed, so its
+    // launches are submitted in one go.
+    if (ctx->fgraph_exec && ctx->fgraph_nx != nx) {
+        RC_HIP(hipGraphExecDestroy(ctx->fgraph_exec));
+        ctx->fgraph_exec = nullptr;
+    }
     if (!ctx->fgraph_exec) {
         RC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        rc_status s = factor_loop_fp64(ctx);
+        rc_status s = factor_loop_fp64(ctx, nx);
         hipGraph_t g = nullptr;
         const hipError_t he = hipStreamEndCapture(ctx->stream, &g);
         if (s != RC_OK) {
@@ -960,6 +987,7 @@ rc_status rc_factor_fp64(rc_ctx* ctx) {
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(hi);
             return RC_HIP_ERROR;
         }
+        ctx->fgraph_nx = nx;
     }
     RC_HIP(hipGraphLaunch(ctx->fgraph_exec, ctx->stream));
     return RC_OK;
@@ -1024,10 +1052,21 @@ static rc_status solve_impl(rc_ctx* ctx, const T* F, const T* Linv, double* d_B,
     return RC_OK;
 }
 
-// Solve A_hat X = B for nrhs columns of length Vp (column-major, ld = Vp) in
-// place, with the FP64 factor.
-rc_status rc_solve_fp64(rc_ctx* ctx, double* d_B, int nrhs) {
-    return solve_impl<double>(ctx, ctx->d_F, ctx->d_Linv, d_B, nrhs, nullptr);
+// L^T X = Y in place for nrhs columns of length Vp (column-major, ld = Vp)
+// with the FP64 factor; Y = L^-1 B comes out of the factorisation's
+// right-hand-side rows.
+rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
+    const int Vp = ctx->nblk * NB;
+    for (int c0 = 0; c0 < nrhs; c0 += RHS_CHUNK) {
+        const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
+        const int32_t eb = ++ctx->epoch;
+        hipLaunchKernelGGL((k_trsm<double, false>), dim3(ctx->nblk), dim3(256), 0, ctx->stream,
+                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv,
+                           d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
+                           ctx->d_status, (const int32_t*)nullptr);
+        RC_CHECK_LAUNCH();
+    }
+    return RC_OK;
 }
 
 // Solve S Y = B in place with the FP32 factor (caller applies the scaling).

@@ -40,6 +40,7 @@ struct rc_ctx {
     hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
     std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies
     hipGraphExec_t fgraph_exec = nullptr;  // captured FP64 panel loop
+    int fgraph_nx = -1;             // right-hand-side row blocks it was captured for
     std::string err;
 
     // topology (device)
@@ -73,7 +74,8 @@ struct rc_ctx {
     double* d_rsp = nullptr;      // row-sum partials [nvt][2][E]
     double* d_rs = nullptr;       // row sums W_K, Q_K (2E)
     double* d_A = nullptr;        // V x V FP64
-    double* d_F = nullptr;        // V x V FP64 factor (lower) of A + mu 11^T
+    double* d_F = nullptr;        // Vp x Vp FP64 factor (lower) of A + mu 11^T, then
+                                  // RC_XROWS right-hand-side rows (B^T, then (L^-1 B)^T)
     float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
     double* d_Linv = nullptr;     // inverted diagonal blocks [nblk][NB][NB]
     double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]

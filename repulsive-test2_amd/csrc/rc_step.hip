@@ -219,7 +219,7 @@ __global__ void k_build_rhs(const double* __restrict__ dE, const double* __restr
     const int col = blockIdx.y;
     if (v >= Vp) return;
     double val = 0.0;
-    if (v < V) {
+    if (v < V && col < 3 + 3 * k) {
         if (col < 3)
             val = dE[(size_t)col * Vp + v];
         else
@@ -856,9 +856,11 @@ rc_status rc_launch_penalty_energy(rc_ctx* ctx, const DevPen& pen, double* d_out
     return RC_OK;
 }
 
-rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B) {
+// B = [dE, C^T blocks] as ncols >= 3 + 3k columns of length Vp (column
+// stride Vp); columns past 3 + 3k are zero.
+rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B, int ncols) {
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_build_rhs, dim3((Vp + 255) / 256, 3 + 3 * k), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_build_rhs, dim3((Vp + 255) / 256, ncols), dim3(256), 0, ctx->stream,
                        dE, ctx->d_C, ctx->V, Vp, k, B);
     RC_CHECK_LAUNCH();
     return RC_OK;
