@@ -16,8 +16,8 @@
 //   k_potrf128    one workgroup factors the 128x128 diagonal block in LDS
 //                 (16-column sub-panels, MFMA updates) and inverts it
 //   k_panel_trsm  L_ik = A_ik L_kk^-T as a GEMM with the stored inverse
-//   k_syrk128     A_ij -= L_ik L_jk^T on 128x128 lower tiles, K = 128,
-//                 v_mfma_f64_16x16x4_f64, XCD-grouped tile order, on a second
+//   k_syrk128     A_ij -= L_ik L_jk^T on 128x64 lower blocks, K = 128,
+//                 v_mfma_f64_16x16x4_f64, XCD-grouped block order, on a second
 //                 stream; the next panel's column is updated separately
 //                 (k_col_diag for its diagonal tile on the critical path,
 //                 k_col_rows below it on the second stream), so the next
@@ -421,11 +421,21 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
 // buffered with the next chunk's global loads in flight during the MFMAs.
 // MFMA operand layout: A[lane%16][lane/16], B[lane/16][lane%16];
 // result element r of a lane is D[4r + lane/16][lane%16].
-template <int MT, int KC, int NC = PB>
+// hook(ch) runs at the top of chunk ch, after the next chunk's global loads
+// are issued. An update C -= A B^T loads its C values into registers there at
+// chunk 0 (in flight during the whole product) and stores C - acc after the
+// loop: a read-modify-write after the loop would expose the load latency,
+// and accumulating onto a preloaded -C would round every partial sum at the
+// magnitude of C instead of the panel's contribution.
+struct NoHook {
+    __device__ void operator()(int) const {}
+};
+
+template <int MT, int KC, int NC = PB, class Hook = NoHook>
 __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int lda,
                                             const double* __restrict__ B, int ldb,
                                             double (*As)[KC][LDT], double (*Bs)[KC][LDT],
-                                            d4 (&acc)[MT / 32][NC / 32]) {
+                                            d4 (&acc)[MT / 32][NC / 32], Hook hook = Hook()) {
     constexpr int TA = MT / 32, TB = NC / 32;
     constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk
     constexpr int NBv = NC * KC / 256;  // B doubles per thread per chunk
@@ -463,6 +473,7 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
     for (int ch = 0; ch < PB / KC; ++ch) {
         const int buf = ch & 1;
         if (ch + 1 < PB / KC) gload((ch + 1) * KC);
+        hook(ch);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             double av[TA], bv[TB];
@@ -510,44 +521,67 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
                     acc[a][b][r];
 }
 
-// Trailing update A_ij -= L_ik L_jk^T over 128x128 lower tiles of the
-// trailing matrix (nt trailing panels). skip_first_col = 0: all nt (nt+1)/2
-// tiles; 1: the (nt-1) nt / 2 tiles right of the first trailing column (that
-// column, the next panel, is updated by k_col_diag / k_col_rows). Tiles are
-// numbered row by row; workgroup b runs on XCD b % 8, so each XCD is given a
-// contiguous range of tile numbers (tiles of one row share their A panel in
-// that XCD's L2).
+// Trailing update A_ij -= L_ik L_jk^T right of the first trailing column
+// (that column, the next panel, is updated by k_col_diag / k_col_rows), over
+// 128 x 64 blocks of the lower triangle: row tile ti (128 rows) and column
+// half hj (64 columns), 0 <= hj <= 2 ti + 1, numbered p = ti (ti + 1) + hj,
+// ti, hj counted from the second trailing 128-column tile. Per workgroup:
+// 4 waves of 64 x 32, K = 128 (the panel) on v_mfma_f64_16x16x4_f64, the
+// block's 32 C values per lane prefetched into registers at the start.
+// Workgroup b runs on XCD b % 8, so each XCD is given a contiguous range of
+// block numbers (blocks of one row share their A panel in that XCD's L2).
+__device__ __forceinline__ void decode_half(int p, int& ti, int& hj) {
+    int t = (int)((__builtin_sqrt(4.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) <= p) ++t;
+    while (t * (t + 1) > p) --t;
+    ti = t;
+    hj = p - t * (t + 1);
+}
+
 __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
-                                                 int ntiles, int skip_first_col) {
+                                                 int nblocks) {
     __shared__ double As[2][KC_SYRK][LDT];
     __shared__ double Bs[2][KC_SYRK][LDT];
-    const int per = (ntiles + 7) >> 3;
+    const int per = (nblocks + 7) >> 3;
     const int p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (p >= ntiles) return;
-    int ti, tj;
-    decode_lower(p, ti, tj);
-    ti += skip_first_col;
-    tj += skip_first_col;
+    if (p >= nblocks) return;
+    int ti, hj;
+    decode_half(p, ti, hj);
     const size_t k0 = (size_t)kb * PB;
-    const size_t ri = k0 + PB + (size_t)ti * PB, rj = k0 + PB + (size_t)tj * PB;
-    d4 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<128, KC_SYRK>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
+    const size_t ri = k0 + 2 * PB + (size_t)ti * PB, rj = k0 + 2 * PB + (size_t)hj * 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
-    const bool diag = (ti == tj);
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
+    double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
+    d4 acc[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    double cv[4][2][4];
+    gemm_nt_f64<128, KC_SYRK, 64>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc,
+                                  [&](int ch) {
+        if (ch == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
+        }
+    });
+    const bool diag = (hj >> 1) == ti;
+    const int jo = (hj & 1) * 64;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
-                const int j = wc + 16 * b + (lane & 15);
-                if (!diag || j <= i) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
+                const int j = jo + wc + 16 * b + (lane & 15);
+                if (!diag || j <= i)
+                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
             }
 }
 
@@ -574,22 +608,33 @@ __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int
         ri = (size_t)Vp + 64 * (size_t)(q / nt);
         rj = k0 + PB + (size_t)(q % nt) * PB;
     }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
     d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    double cv[2][4][4];
+    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc, [&](int ch) {
+        if (ch == 0) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
+        }
+    });
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                F[(ri + wr + 16 * a + 4 * r + (lane >> 4)) * Vp + rj + wc + 16 * b + (lane & 15)] -=
-                    acc[a][b][r];
+                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
 }
 
 // Update of the next panel's 128 x 128 diagonal tile with panel kb, on the
@@ -601,14 +646,27 @@ __global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int
     const int hi = blockIdx.x == 0 ? 0 : 1, hj = blockIdx.x == 2 ? 1 : 0;
     const size_t k0 = (size_t)kb * PB;
     const size_t ri = k0 + PB + 64 * hi, rj = k0 + PB + 64 * hj;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
     d4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<64, KC_TRSM, 64>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    double cv[2][2][4];
+    gemm_nt_f64<64, KC_TRSM, 64>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc,
+                                 [&](int ch) {
+        if (ch == 0) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
+        }
+    });
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -617,7 +675,8 @@ __global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
                 const int j = wc + 16 * b + (lane & 15);
-                if (hi != hj || j <= i) F[(ri + i) * Vp + rj + j] -= acc[a][b][r];
+                if (hi != hj || j <= i)
+                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
             }
 }
 
@@ -925,12 +984,13 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
             RC_CHECK_LAUNCH();
         }
         RC_HIP(hipEventRecord(Cv[kb], ctx->stream2));
-        const int ntiles = (nt - 1) * nt / 2;
-        if (ntiles > 0) {
+        // 128 x 64 blocks of the lower triangle right of column kb+1
+        const int nblocks = (nt - 1) * nt;
+        if (nblocks > 0) {
             const double mr = (double)(nt - 1) * PB;
             hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
-            hipLaunchKernelGGL(k_syrk128, dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, ntiles, 1);
+            hipLaunchKernelGGL(k_syrk128, dim3(8 * ((nblocks + 7) / 8)), dim3(256), 0, ctx->stream2,
+                               ctx->d_F, Vp, kb, nblocks);
             RC_CHECK_LAUNCH();
             if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream2));
         }

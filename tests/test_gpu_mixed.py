@@ -70,6 +70,30 @@ def test_mixed_vs_fp64_step(V, maker):
     assert rmx["stats"]["residual"] <= 1e-10
 
 
+def test_fp64_factor_forward_error_v8192():
+    """Accuracy guard for the FP64 factorisation at full size: the FP64 step
+    against the PCG-refined solution (FP64 operator, relres 1e-12, independent
+    of the FP64 factor). kappa(A_hat) ~ 1e9 here, so the FP64 forward error
+    reflects how the trailing updates round: accumulating each panel's
+    contribution from zero and subtracting it once gives ~6e-9 (energy) and
+    ~2.5e-8 (direction); rounding every partial sum at the magnitude of the
+    trailing entries instead gives ~3e-7 / ~1.2e-6."""
+    v, e = curves.fourier_curve(8192, seed=1)
+    x0 = core.barycenter_target(v, e)
+    L0 = core.total_length(v, e)
+    cs = [core.barycenter_block(x0), core.total_length_block(L0)]
+    pen = core.PenaltyConfig(field_weight=10.0, field_X=(0.5, 1.0, 0.0))
+    r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen, collect_field=True))
+    rmx = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
+        penalties=pen, collect_field=True, precision="mixed"))
+    de = abs(rmx["energy"] - r64["energy"]) / abs(rmx["energy"])
+    dd = rel(r64["descentField"], rmx["descentField"])
+    log("fp64 vs refined", de, dd, rmx["diagnostics"]["refineResidual"])
+    assert rmx["diagnostics"]["refineResidual"] <= 1e-12
+    assert de <= 3e-8
+    assert dd <= 1e-7
+
+
 def test_mixed_step_matches_oracle_trefoil():
     v, e = curves.trefoil_jittered(128, 0.01, seed=1)
     x0 = rco.barycenter_target(v, e)
