@@ -227,12 +227,29 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
     }
 }
 
-__global__ void k_rowsum_reduce(const double* __restrict__ rsp, int nvt, int E,
-                                double* __restrict__ rs) {
+// Sums in tile order t = 0 .. nvt-1 (fixed order); the loads of 16 tiles are
+// issued ahead of their adds, and 64-thread blocks spread the edges over
+// E/64 workgroups.
+__global__ __launch_bounds__(64) void k_rowsum_reduce(const double* __restrict__ rsp, int nvt,
+                                                      int E, double* __restrict__ rs) {
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= E) return;
     double sw = 0.0, sy = 0.0;
-    for (int t = 0; t < nvt; ++t) {
+    int t = 0;
+    for (; t + 16 <= nvt; t += 16) {
+        double a[16], b[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            a[q] = rsp[((size_t)(t + q) * 2) * E + K];
+            b[q] = rsp[((size_t)(t + q) * 2 + 1) * E + K];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            sw += a[q];
+            sy += b[q];
+        }
+    }
+    for (; t < nvt; ++t) {
         sw += rsp[((size_t)t * 2) * E + K];
         sy += rsp[((size_t)t * 2 + 1) * E + K];
     }
@@ -279,7 +296,7 @@ rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p) {
                            ctx->d_vLocal, ctx->d_A, ctx->d_rsp);
     }
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 63) / 64), dim3(64), 0, ctx->stream,
                        ctx->d_rsp, ctx->nvt, ctx->E, ctx->d_rs);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gram_band, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,

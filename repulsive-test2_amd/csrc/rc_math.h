@@ -21,44 +21,47 @@ __device__ __forceinline__ double rcp(double x) {
     return y;
 }
 
-// sqrt(x) for x >= 0 (returns 0 for x == 0). Heron-corrected.
+// sqrt(x) for x >= 0 (0 for x == 0, NaN for x < 0 or NaN). Heron-corrected.
+// Branch-free: x <= 0 runs the same sequence on 1 and is replaced by a
+// select at the end, so the pair kernels' several roots interleave.
 __device__ __forceinline__ double sqrt_(double x) {
-    if (!(x > 0.0)) return (x == 0.0) ? 0.0 : __builtin_sqrt(x);
-    double y = __builtin_amdgcn_rsq(x);
+    const bool pos = x > 0.0;
+    const double xs = pos ? x : 1.0;
+    double y = __builtin_amdgcn_rsq(xs);
     double h = 0.5 * y;
-    double g = x * y;
+    double g = xs * y;
     double r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
     r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
-    double d = fma(-g, g, x);
-    return fma(d, h, g);
+    double d = fma(-g, g, xs);
+    const double sq = fma(d, h, g);
+    return pos ? sq : ((x == 0.0) ? 0.0 : __builtin_nan(""));
 }
 
-// sqrt and 1/sqrt together (x > 0 required for inv; inv = 0 when x == 0).
+// sqrt and 1/sqrt together (inv = 0 when x <= 0; s as sqrt_).
 __device__ __forceinline__ void sqrt_rsqrt(double x, double& s, double& inv) {
-    if (!(x > 0.0)) {
-        s = (x == 0.0) ? 0.0 : __builtin_sqrt(x);
-        inv = 0.0;
-        return;
-    }
-    double y = __builtin_amdgcn_rsq(x);
+    const bool pos = x > 0.0;
+    const double xs = pos ? x : 1.0;
+    double y = __builtin_amdgcn_rsq(xs);
     double h = 0.5 * y;
-    double g = x * y;
+    double g = xs * y;
     double r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
     r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
-    double d = fma(-g, g, x);
-    s = fma(d, h, g);
-    inv = 2.0 * h;
+    double d = fma(-g, g, xs);
+    const double sq = fma(d, h, g);
+    double iv = 2.0 * h;
     // one more Newton step on the reciprocal for full accuracy
-    double e = fma(-s, inv, 1.0);
-    inv = fma(inv, e, inv);
+    double e = fma(-sq, iv, 1.0);
+    iv = fma(iv, e, iv);
+    s = pos ? sq : ((x == 0.0) ? 0.0 : __builtin_nan(""));
+    inv = pos ? iv : 0.0;
 }
 
 // x^(-1/3), x > 0.

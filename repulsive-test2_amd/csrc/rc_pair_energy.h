@@ -10,6 +10,7 @@
 // with the per-endpoint form to rounding).
 #pragma once
 
+#include "rc_internal.h"
 #include "rc_math.h"
 
 namespace rcpair {
@@ -53,6 +54,59 @@ __device__ __forceinline__ double pair_energy(const P& pw, double eps, const dou
     const double sFI = nI[0] * (invB[0][0] + invB[1][0]) + nI[1] * (invB[0][1] + invB[1][1]);
     const double sFJ = nJ[0] * (invB[0][0] + invB[0][1]) + nJ[1] * (invB[1][0] + invB[1][1]);
     return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
+}
+
+// One wave's J tile in LDS (structure of arrays, RC_TILE edges).
+struct ETile {
+    double P[6][RC_TILE];
+    double ell[RC_TILE], lp[RC_TILE];
+    int32_t i1[RC_TILE], i2[RC_TILE];
+};
+
+// Energy of the unordered tile pair (a, b), a <= b, for this wave: lane t
+// owns edge I = 64a + t and walks the edges J of tile b (J > I on the
+// diagonal tile) in order; disjoint pairs only. The J tile is staged in the
+// wave's LDS tile T with one coalesced load per field, then read as LDS
+// broadcasts. Returns the lane's sum (the caller reduces over the wave).
+template <class P>
+__device__ __forceinline__ double tile_pair_energy(const EdgeSoA& ed, int E, const P& pw,
+                                                   double eps, int a, int b, int lane, ETile& T) {
+    {
+        const int J = b * RC_TILE + lane;
+        const int Jc = J < E ? J : E - 1;
+        T.P[0][lane] = ed.ax[Jc]; T.P[1][lane] = ed.ay[Jc]; T.P[2][lane] = ed.az[Jc];
+        T.P[3][lane] = ed.bx[Jc]; T.P[4][lane] = ed.by[Jc]; T.P[5][lane] = ed.bz[Jc];
+        T.ell[lane] = ed.ell[Jc];
+        T.lp[lane] = ed.lpow[Jc];
+        T.i1[lane] = ed.i1[Jc];
+        T.i2[lane] = ed.i2[Jc];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int I = a * RC_TILE + lane;
+    const bool Iok = I < E;
+    const int Ic = Iok ? I : 0;
+    const double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
+    const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
+    const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
+    const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
+    const bool diag = (a == b);
+    const int ns = min(RC_TILE, E - b * RC_TILE);
+    double sum = 0.0;
+#pragma unroll 2
+    for (int s = 0; s < ns; ++s) {
+        const double PJ[6] = {T.P[0][s], T.P[1][s], T.P[2][s], T.P[3][s], T.P[4][s], T.P[5][s]};
+        const int j1 = T.i1[s], j2 = T.i2[s];
+        const bool valid = Iok && (!diag || s > lane) && i1 != j1 && i1 != j2 && i2 != j1 &&
+                           i2 != j2;
+        const double e = pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, PJ[3] - PJ[0],
+                                     PJ[4] - PJ[1], PJ[5] - PJ[2], ellI, T.ell[s], lpI, T.lp[s]);
+        sum += valid ? e : 0.0;
+    }
+    // the tile is rewritten by the wave's next tile pair
+    __builtin_amdgcn_wave_barrier();
+    return sum;
 }
 
 }  // namespace rcpair

@@ -82,6 +82,7 @@ __global__ void k_edge_prep(const double* __restrict__ pos, int V, const int32_t
 template <class P>
 __global__ __launch_bounds__(256) void k_energy(EdgeSoA ed, int E, int npairs, P pw, double eps,
                                                 double* __restrict__ part) {
+    __shared__ rcpair::ETile jt[4];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int base = blockIdx.x * 4; base < npairs; base += gridDim.x * 4) {
@@ -89,27 +90,7 @@ __global__ __launch_bounds__(256) void k_energy(EdgeSoA ed, int E, int npairs, P
         if (p >= npairs) break;
         int a, b;
         tri_decode(p, a, b);
-        const int I = a * RC_TILE + lane;
-        const bool Iok = I < E;
-        const int Ic = Iok ? I : 0;
-        double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
-        const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
-        const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
-        const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
-        const bool diag = (a == b);
-        double sum = 0.0;
-        for (int s = 0; s < RC_TILE; ++s) {
-            const int J = b * RC_TILE + s;
-            if (J >= E) break;
-            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
-            const int j1 = ed.i1[J], j2 = ed.i2[J];
-            const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
-                               i2 != j2;
-            const double eJx = PJ[3] - PJ[0], eJy = PJ[4] - PJ[1], eJz = PJ[5] - PJ[2];
-            const double e = rcpair::pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, eJx, eJy, eJz, ellI,
-                                         ed.ell[J], lpI, ed.lpow[J]);
-            sum += valid ? e : 0.0;
-        }
+        double sum = rcpair::tile_pair_energy(ed, E, pw, eps, a, b, lane, jt[wave]);
         sum = wave_sum(sum);
         if (lane == 0) part[p] = sum;
     }

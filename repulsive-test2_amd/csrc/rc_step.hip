@@ -755,6 +755,7 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
                                                    LSCtl* ctl) {
     __shared__ double wsums[4];
     __shared__ int last;
+    __shared__ rcpair::ETile jt[4];
     if (ctl->skip_energy) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -764,26 +765,7 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
         if (p >= npairs) break;
         int a, b;
         tri_decode_e(p, a, b);
-        const int I = a * RC_TILE + lane;
-        const bool Iok = I < E;
-        const int Ic = Iok ? I : 0;
-        const double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
-        const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
-        const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
-        const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
-        const bool diag = (a == b);
-        for (int s = 0; s < RC_TILE; ++s) {
-            const int J = b * RC_TILE + s;
-            if (J >= E) break;
-            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
-            const int j1 = ed.i1[J], j2 = ed.i2[J];
-            const bool valid = Iok && (!diag || J > I) && i1 != j1 && i1 != j2 && i2 != j1 &&
-                               i2 != j2;
-            const double e = rcpair::pair_energy(pw, eps, PI, PJ, eIx, eIy, eIz, PJ[3] - PJ[0],
-                                            PJ[4] - PJ[1], PJ[5] - PJ[2], ellI, ed.ell[J], lpI,
-                                            ed.lpow[J]);
-            sum += valid ? e : 0.0;
-        }
+        sum += rcpair::tile_pair_energy(ed, E, pw, eps, a, b, lane, jt[wave]);
     }
     sum = wsum(sum);
     if (lane == 0) wsums[wave] = sum;
