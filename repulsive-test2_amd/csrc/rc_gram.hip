@@ -133,6 +133,11 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
     __shared__ double Xt[RC_RMAX][RC_RMAX + 1];
     __shared__ double Yt[RC_RMAX][RC_RMAX + 1];
     __shared__ double Wt[RC_RMAX][RC_RMAX + 1];
+    // per tile: incidences of its vertices (offsets relative to the tile's
+    // first vertex, entries local edge index | slot << 16)
+    __shared__ int begU[RC_VTILE + 1], begW[RC_VTILE + 1];
+    __shared__ int incU[2 * RC_RMAX], incW[2 * RC_RMAX];
+    const int t = threadIdx.x;
     for (int p = blockIdx.x; p < ngram; p += gridDim.x) {
         int U, W;
         tri_decode(p, U, W);  // U <= W
@@ -140,22 +145,34 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
         int nU, nV;
         load_tile_edges(GU, rOff, rEdge, U, ed, owner, nU);
         load_tile_edges(GV, rOff, rEdge, W, ed, owner, nV);
+        {
+            // a tile's incidences number at most 2 RC_RMAX (each of its <=
+            // RC_RMAX edges has two endpoints)
+            const int u0 = vOff[U * RC_VTILE], w0 = vOff[W * RC_VTILE];
+            if (t <= RC_VTILE) {
+                begU[t] = vOff[min(U * RC_VTILE + t, V)] - u0;
+                begW[t] = vOff[min(W * RC_VTILE + t, V)] - w0;
+            }
+            const int nu = vOff[min(U * RC_VTILE + RC_VTILE, V)] - u0;
+            const int nw = vOff[min(W * RC_VTILE + RC_VTILE, V)] - w0;
+            if (t < nu) incU[t] = vLocal[u0 + t] | (vSlot[u0 + t] << 16);
+            if (t >= 128 && t - 128 < nw) incW[t - 128] = vLocal[w0 + t - 128] | (vSlot[w0 + t - 128] << 16);
+        }
         __syncthreads();
         // pair phase
-        for (int q = threadIdx.x; q < nU * nV; q += blockDim.x) {
+        for (int q = t; q < nU * nV; q += blockDim.x) {
             const int li = q / nV, lj = q - (q / nV) * nV;
             const int I = GU.id[li], J = GV.id[lj];
             const bool dis = GU.i1[li] != GV.i1[lj] && GU.i1[li] != GV.i2[lj] &&
                              GU.i2[li] != GV.i1[lj] && GU.i2[li] != GV.i2[lj];
             double w2 = 0.0, X = 0.0, Y = 0.0;
             if (dis) {
-                // canonical role order (lower edge id first) so equal pairs give equal bits
-                if (I <= J)
-                    pair_weights(pw, eps, GU.P[li], GU.u[li], GU.ell[li], GV.P[lj], GV.u[lj],
-                                 GV.ell[lj], w2, X, Y);
-                else
-                    pair_weights(pw, eps, GV.P[lj], GV.u[lj], GV.ell[lj], GU.P[li], GU.u[li],
-                                 GU.ell[li], w2, X, Y);
+                // canonical role order (lower edge id first) so equal pairs give
+                // equal bits, chosen by operand address (no divergent branch)
+                const bool ord = I <= J;
+                pair_weights(pw, eps, ord ? GU.P[li] : GV.P[lj], ord ? GU.u[li] : GV.u[lj],
+                             ord ? GU.ell[li] : GV.ell[lj], ord ? GV.P[lj] : GU.P[li],
+                             ord ? GV.u[lj] : GU.u[li], ord ? GV.ell[lj] : GU.ell[li], w2, X, Y);
             }
             Xt[li][lj] = X;
             Yt[li][lj] = Y;
@@ -164,30 +181,32 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
         __syncthreads();
         // row-sum partials: slot [t][K] for K owned by one side, summed over
         // the edges owned by the other side (each ordered pair counted once).
-        if ((int)threadIdx.x < nU) {
-            const int li = threadIdx.x;
-            if (GU.own[li] == U) {
-                double sw = 0.0, sy = 0.0;
-                for (int lj = 0; lj < nV; ++lj)
-                    if (GV.own[lj] == W) {
-                        sw += Wt[li][lj];
-                        sy += Yt[li][lj];
-                    }
-                rsp[((size_t)W * 2) * E + GU.id[li]] = sw;
-                rsp[((size_t)W * 2 + 1) * E + GU.id[li]] = sy;
+        // Four consecutive lanes per row, each over every fourth entry, then
+        // combined in fixed order ((0 + 1) + (2 + 3)).
+        for (int side = 0; side < (diag ? 1 : 2); ++side) {
+            const int nr = side ? nV : nU, nc = side ? nU : nV;
+            const int row = t >> 2, part = t & 3;
+            double sw = 0.0, sy = 0.0;
+            bool live = false;
+            if (row < nr) {
+                live = side ? (GV.own[row] == W) : (GU.own[row] == U);
+                for (int c = part; c < nc; c += 4) {
+                    const bool on = side ? (GU.own[c] == U) : (GV.own[c] == W);
+                    const double wv = side ? Wt[c][row] : Wt[row][c];
+                    const double yv = side ? Yt[c][row] : Yt[row][c];
+                    sw += on ? wv : 0.0;
+                    sy += on ? yv : 0.0;
+                }
             }
-        }
-        if (!diag && (int)threadIdx.x >= 64 && (int)threadIdx.x - 64 < nV) {
-            const int lj = threadIdx.x - 64;
-            if (GV.own[lj] == W) {
-                double sw = 0.0, sy = 0.0;
-                for (int li = 0; li < nU; ++li)
-                    if (GU.own[li] == U) {
-                        sw += Wt[li][lj];
-                        sy += Yt[li][lj];
-                    }
-                rsp[((size_t)U * 2) * E + GV.id[lj]] = sw;
-                rsp[((size_t)U * 2 + 1) * E + GV.id[lj]] = sy;
+            sw += __shfl_xor(sw, 1, 64);
+            sy += __shfl_xor(sy, 1, 64);
+            sw += __shfl_xor(sw, 2, 64);
+            sy += __shfl_xor(sy, 2, 64);
+            if (row < nr && part == 0 && live) {
+                const int K = side ? GV.id[row] : GU.id[row];
+                const int other = side ? U : W;
+                rsp[((size_t)other * 2) * E + K] = sw;
+                rsp[((size_t)other * 2 + 1) * E + K] = sy;
             }
         }
         // gather phase: val(u, v), u in U row tile, v in W column tile,
@@ -205,15 +224,17 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
                 int u = U * RC_VTILE + r, v = W * RC_VTILE + c;
                 if (u >= V || v >= V) continue;
                 bool swap = diag && v < u;   // canonical (min, max) order
-                const int lo = swap ? v : u, hi = swap ? u : v;
+                // tile-local indices: lo in the U tile, hi in the W tile
+                const int rl = swap ? c : r, rh = swap ? r : c;
                 double val = 0.0;
-                for (int q1 = vOff[lo]; q1 < vOff[lo + 1]; ++q1) {
-                    const int li = vLocal[q1];
-                    const double si = vSlot[q1] ? -1.0 : 1.0;
-                    for (int q2 = vOff[hi]; q2 < vOff[hi + 1]; ++q2) {
-                        const int lj = vLocal[q2];
-                        const double sj = vSlot[q2] ? -1.0 : 1.0;
-                        // lo always indexes the U-side edge list in off-diagonal tiles
+                for (int q1 = begU[rl]; q1 < begU[rl + 1]; ++q1) {
+                    const int e1 = incU[q1];
+                    const int li = e1 & 0xffff;
+                    const double si = (e1 >> 16) ? -1.0 : 1.0;
+                    for (int q2 = begW[rh]; q2 < begW[rh + 1]; ++q2) {
+                        const int e2 = incW[q2];
+                        const int lj = e2 & 0xffff;
+                        const double sj = (e2 >> 16) ? -1.0 : 1.0;
                         val -= si * sj * Xt[li][lj] + Yt[li][lj];
                     }
                 }
