@@ -112,7 +112,6 @@ extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->dep_ev) (void)hipEventDestroy(e);
-    if (ctx->fgraph_exec) (void)hipGraphExecDestroy(ctx->fgraph_exec);
     free_all(ctx);
     delete ctx;
 }

@@ -39,8 +39,6 @@ struct rc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
     std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies
-    hipGraphExec_t fgraph_exec = nullptr;  // captured FP64 panel loop
-    int fgraph_nx = -1;             // right-hand-side row blocks it was captured for
     std::string err;
 
     // topology (device)
