@@ -90,7 +90,7 @@ static void free_all(rc_ctx* ctx) {
                     ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
                     ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_A,
                     ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
-                    ctx->d_flags, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
+                    ctx->d_flags, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
                     ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
                     ctx->d_dE, ctx->d_gt, ctx->d_Y};
@@ -262,6 +262,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_Linv, (size_t)ctx->nblk * RC_NB * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, ctx->nblk);
+    ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
     ALLOC(ctx->d_status, 4);
     ctx->nrhs_cap = 3 + 3 * RC_KMAX;
     ALLOC(ctx->d_rhs, (size_t)ctx->nrhs_cap * Vp);
@@ -620,7 +621,7 @@ extern "C" rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xy
     RC_HIP(hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     RC_HIP(hipStreamSynchronize(ctx->stream));
     if (st[0] & 2) {
-        ctx->err = "triangular solve dependency wait timed out";
+        ctx->err = "device dependency wait timed out (triangular solve or factorisation)";
         return RC_HIP_ERROR;
     }
     if (lambda)
@@ -757,7 +758,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     if (tm) RC_HIP(hipEventRecord(ev[8], ctx->stream));
     RC_HIP(hipStreamSynchronize(ctx->stream));
     if (hst[0] & 2) {
-        ctx->err = "triangular solve dependency wait timed out";
+        ctx->err = "device dependency wait timed out (triangular solve or factorisation)";
         return RC_HIP_ERROR;
     }
     const double* d = ctx->d_host_stage;

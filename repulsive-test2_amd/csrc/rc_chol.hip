@@ -495,12 +495,48 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
     }
 }
 
+__device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t epoch, int32_t* status) {
+    if (threadIdx.x == 0) {
+        int spins = 0;
+        // poll with relaxed loads (an acquire load would invalidate the L2
+        // on every spin), then one acquire fence once the flag is seen
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1 << 24)) {
+                atomicOr(status, 2);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void publish(int32_t* flag, int32_t epoch) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Completion counter of a producer kernel: every workgroup releases its
+// stores device-wide, then one thread adds 1 (consumers poll with wait_flag
+// for the number of producer workgroups).
+__device__ __forceinline__ void signal_done(int32_t* cnt) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
-// workgroup, in place.
+// workgroup, in place. wcnt (nullable): the previous panel's column update
+// (k_col_rows, other stream) must have counted `target` workgroups first.
 __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int Vp, int kb,
-                                                    const double* __restrict__ Linv128) {
+                                                    const double* __restrict__ Linv128,
+                                                    const int32_t* __restrict__ wcnt,
+                                                    int32_t target, int32_t* __restrict__ status) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
+    if (wcnt) wait_flag(wcnt, target, status);
     const size_t k0 = (size_t)kb * PB;
     const size_t r0 = k0 + PB + (size_t)blockIdx.x * 64;
     d4 acc[2][4];
@@ -539,7 +575,7 @@ __device__ __forceinline__ void decode_half(int p, int& ti, int& hj) {
 }
 
 __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
-                                                 int nblocks) {
+                                                 int nblocks, int32_t* __restrict__ diag_done) {
     __shared__ double As[2][KC_SYRK][LDT];
     __shared__ double Bs[2][KC_SYRK][LDT];
     const int per = (nblocks + 7) >> 3;
@@ -583,6 +619,9 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
                 if (!diag || j <= i)
                     C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
             }
+    // blocks 0 and 1 (scheduled first) are the diagonal tile of the panel
+    // after next, which k_col_diag of the next panel waits for
+    if (p < 2) signal_done(diag_done);
 }
 
 // Update of the next panel's column (the first trailing column) with panel
@@ -593,7 +632,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
 // right-hand-side rows below the factor (rows Vp ..): 64-row block x, all nt
 // trailing columns tj, workgroup 2(nt-1) + x nt + tj.
 __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb,
-                                                     int nt) {
+                                                     int nt, int32_t* __restrict__ done) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
     const size_t k0 = (size_t)kb * PB;
@@ -635,14 +674,20 @@ __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
+    signal_done(done);
 }
 
 // Update of the next panel's 128 x 128 diagonal tile with panel kb, on the
 // factorisation's critical path: its three lower 64 x 64 blocks, one
 // workgroup each (block (hi, hj) = (0,0), (1,0), (1,1)).
-__global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int Vp, int kb) {
+__global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int Vp, int kb,
+                                                     const int32_t* __restrict__ wcnt,
+                                                     int32_t* __restrict__ status) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
+    // the previous panel's trailing update must have left this tile (its
+    // two blocks of it counted)
+    if (wcnt) wait_flag(wcnt, 2, status);
     const int hi = blockIdx.x == 0 ? 0 : 1, hj = blockIdx.x == 2 ? 1 : 0;
     const size_t k0 = (size_t)kb * PB;
     const size_t ri = k0 + PB + 64 * hi, rj = k0 + PB + 64 * hj;
@@ -741,29 +786,6 @@ __global__ __launch_bounds__(256) void k_syrk_f16x3(float* __restrict__ F, int V
                 const int col = wc + 16 * b + lr;
                 if (!diag || col <= row) F[(size_t)(ri + row) * Vp + rj + col] -= acc[a][b][r];
             }
-}
-
-__device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t epoch, int32_t* status) {
-    if (threadIdx.x == 0) {
-        int spins = 0;
-        // poll with relaxed loads (an acquire load would invalidate the L2
-        // on every spin), then one acquire fence once the flag is seen
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1 << 24)) {
-                atomicOr(status, 2);
-                break;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void publish(int32_t* flag, int32_t epoch) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Sync-free blocked substitution on nrhs <= RHS_CHUNK columns of B
@@ -907,11 +929,17 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 // update of the next diagonal tile (k_col_diag), the critical path. Second
 // stream: the rest of column kb+1 (k_col_rows) and then the rest of the
 // trailing matrix (k_syrk128), both with panel kb, overlapping the next
-// panel's diagonal factorisation. Ordering by events: T (panel kb final)
-// before the second stream's updates of kb; C (column kb+1 below its
-// diagonal tile done) before panel TRSM(kb+1); B (rest-update kb done)
-// before k_col_diag(kb+1), which would otherwise touch a tile it is still
-// updating. The main stream waits for the last rest-update before returning.
+// panel's diagonal factorisation. Ordering: the event T (panel kb final)
+// before the second stream's updates of kb; device counters (d_fcnt, cleared
+// per factorisation) for the main stream's waits, which a stream event would
+// turn into a command-processor barrier of ~5-10 us on the critical path:
+// panel TRSM(kb+1) polls for all k_col_rows(kb) workgroups, k_col_diag(kb+1)
+// for the two k_syrk128(kb) blocks of its tile (scheduled first). Both
+// consumers are small and their producers were launched earlier on the other
+// stream, so a poll always ends; a bound on it sets status bit 2. (Waiting
+// for the whole previous update with a stream event while it is large was
+// measured slower on the same box: 87.6-88.6 against 90.9-91.2 steps/s.)
+// The main stream waits for the last rest-update before returning.
 // A non-positive pivot sets status bit 1 (singular saddle system).
 // Right-hand sides: nx blocks of 64 rows stored below the factor (rows Vp ..,
 // B^T) are carried along as extra matrix rows: the panel TRSM of every panel
@@ -921,8 +949,8 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     const int Vp = ctx->nblk * NB;
     const int np = Vp / PB;
-    if ((int)ctx->dep_ev.size() < 3 * np + 2) {
-        while ((int)ctx->dep_ev.size() < 3 * np + 2) {
+    if ((int)ctx->dep_ev.size() < np + 2) {
+        while ((int)ctx->dep_ev.size() < np + 2) {
             // completion signals only: each kernel's own end-of-kernel release
             // makes its writes visible device-wide, so no extra fence here
             hipEvent_t e;
@@ -930,11 +958,12 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
             ctx->dep_ev.push_back(e);
         }
     }
-    hipEvent_t* T = ctx->dep_ev.data();            // panel kb final
-    hipEvent_t* Bv = ctx->dep_ev.data() + np;      // rest-update kb done
-    hipEvent_t* Cv = ctx->dep_ev.data() + 2 * np;  // column kb+1 below its diagonal tile done
-    hipEvent_t start = ctx->dep_ev[3 * np];
-    hipEvent_t fin = ctx->dep_ev[3 * np + 1];
+    hipEvent_t* T = ctx->dep_ev.data();   // panel kb final
+    hipEvent_t start = ctx->dep_ev[np];
+    hipEvent_t fin = ctx->dep_ev[np + 1];
+    int32_t* colcnt = ctx->d_fcnt;        // [kb]: k_col_rows(kb) workgroups done
+    int32_t* diagcnt = ctx->d_fcnt + np;  // [kb]: k_syrk128(kb) blocks of tile kb+2 done
+    RC_HIP(hipMemsetAsync(ctx->d_fcnt, 0, sizeof(int32_t) * 2 * (size_t)np, ctx->stream));
     // the second stream starts after the factor's input is prepared
     RC_HIP(hipEventRecord(start, ctx->stream));
     RC_HIP(hipStreamWaitEvent(ctx->stream2, start, 0));
@@ -964,37 +993,38 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
         if (nt == 0 && nx == 0) break;
         // panel kb below its diagonal block (and in the right-hand-side
         // rows): needs column kb below the diagonal tile (k_col_rows of
-        // panel kb-1, second stream)
-        if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Cv[kb - 1], 0));
+        // panel kb-1, second stream: 2 (nt - 1) + nx nt workgroups of it
+        // with nt = np - kb)
+        const int prev_cols = (kb > 0) ? 2 * (np - kb - 1) + nx * (np - kb) : 0;
         hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt + nx), dim3(256), 0, ctx->stream, ctx->d_F, Vp,
-                           kb, (const double*)ctx->d_Linv128);
+                           kb, (const double*)ctx->d_Linv128,
+                           (const int32_t*)(prev_cols > 0 ? colcnt + kb - 1 : nullptr), prev_cols,
+                           ctx->d_status);
         RC_CHECK_LAUNCH();
         RC_HIP(hipEventRecord(T[kb], ctx->stream));
         if (nt == 0) break;
         // next diagonal tile: after the rest-update of panel kb-1 has left it
-        if (kb > 0) RC_HIP(hipStreamWaitEvent(ctx->stream, Bv[kb - 1], 0));
-        hipLaunchKernelGGL(k_col_diag, dim3(3), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb);
+        hipLaunchKernelGGL(k_col_diag, dim3(3), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
+                           (const int32_t*)(kb > 0 ? diagcnt + kb - 1 : nullptr), ctx->d_status);
         RC_CHECK_LAUNCH();
         // second stream: the rest of column kb+1, then the rest of the
         // trailing matrix, both with panel kb
         RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
         if (2 * (nt - 1) + nx * nt > 0) {
             hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1) + nx * nt), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nt);
+                               ctx->d_F, Vp, kb, nt, colcnt + kb);
             RC_CHECK_LAUNCH();
         }
-        RC_HIP(hipEventRecord(Cv[kb], ctx->stream2));
         // 128 x 64 blocks of the lower triangle right of column kb+1
         const int nblocks = (nt - 1) * nt;
         if (nblocks > 0) {
             const double mr = (double)(nt - 1) * PB;
             hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
             hipLaunchKernelGGL(k_syrk128, dim3(8 * ((nblocks + 7) / 8)), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nblocks);
+                               ctx->d_F, Vp, kb, nblocks, diagcnt + kb);
             RC_CHECK_LAUNCH();
             if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream2));
         }
-        RC_HIP(hipEventRecord(Bv[kb], ctx->stream2));
     }
     // the main stream continues after all second-stream work
     RC_HIP(hipEventRecord(fin, ctx->stream2));

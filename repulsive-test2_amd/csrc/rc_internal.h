@@ -81,6 +81,7 @@ struct rc_ctx {
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
     double mixed_shift = 0.0;     // diagonal shift of the last FP32 factorisation
     int32_t* d_flags = nullptr;   // triangular-solve block flags
+    int32_t* d_fcnt = nullptr;    // factorisation completion counters [2][nblk]
     int32_t epoch = 0;
     int32_t* d_status = nullptr;  // device status words (factor breakdown, spin timeouts)
 
