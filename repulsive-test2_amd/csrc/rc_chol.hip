@@ -904,6 +904,131 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
     publish(flags + r, epoch);
 }
 
+// Backward substitution L^T X = Y (FP64 factor) on nrhs <= RHS_CHUNK columns
+// of Y (column-major, leading dimension ldb), in place; block b solves the
+// 64-row block r = nblk-1-b. 8 waves, wave w owns the 16 x 16 output tile
+// (rows 16 (w & 3), columns 16 (w >> 2)).
+//   X_r = L_rr^-T (Y_r - sum_{c > r} L_cr^T X_c)
+//       = Z_r - P_r^T X_{r+1},  Z_r = L_rr^-T (Y_r - sum_{c >= r+2} L_cr^T X_c),
+//                               P_r = L_{r+1,r} L_rr^-1,
+// so only one 64 x 64 x nrhs product follows the wait for block r+1: Z_r is
+// formed while block r+1 is still solving, P_r at the start. P_r is computed
+// as MFMA results (P tile (nb, mb): P[16nb + 4q + lk][16mb + lr] in register
+// q), which is exactly the A-operand layout of -P_r^T for row block mb and
+// k-block nb, so it stays in registers.
+__global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, int Vp,
+                                                  const double* __restrict__ Linv, double* B,
+                                                  int ldb, int nrhs, int nblk,
+                                                  int32_t* __restrict__ flags, int32_t epoch,
+                                                  int32_t* __restrict__ status) {
+    __shared__ double Lt[NB][NB + 1];         // A operand L_cr^T (Lt[m][i]); staging at start
+    __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (X_c, the accumulator for Z)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int r = nblk - 1 - (int)blockIdx.x;
+    const size_t rb = (size_t)r * NB;
+    const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
+    const double* Lrr = Linv + rb * NB;       // L_rr^-1, row-major 64 x 64
+    // factor tile rows tr, 8 consecutive columns from tc
+    const int tr = t >> 3, tc = (t & 7) * 8;
+    double v[8];
+    auto gload = [&](int cb) {                // L_{cb,r} rows tr, columns tc..tc+7 of block r
+        const double* p = F + ((size_t)cb * NB + tr) * Vp + rb + tc;
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            const double2 x = *reinterpret_cast<const double2*>(p + q);
+            v[q] = x.x;
+            v[q + 1] = x.y;
+        }
+    };
+    auto sstore_t = [&]() {                   // Lt[m][i] = L[i][m]
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Lt[tc + q][tr] = v[q];
+    };
+    auto load_ys = [&](int cb) {              // Ys[m][c] = X_cb[m][c]
+#pragma unroll
+        for (int q = 0; q < NB * RHS_CHUNK / 512; ++q) {
+            const int e = t + 512 * q, m = e % NB, c = e / NB;
+            const int cc = c < nrhs ? c : nrhs - 1;
+            const double y = B[(size_t)cc * ldb + (size_t)cb * NB + m];
+            Ys[m][c] = (c < nrhs) ? y : 0.0;
+        }
+    };
+    // this wave's rows of L_rr^-T as A operands: A[m = wr + lr][k = 4kk + lk]
+    double li[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(4 * kk + lk) * NB + wr + lr];
+    // -P_r^T rows wr.. as A operands for k-blocks nb: pt[nb][q]
+    double pt[4][4];
+    const int ndep = nblk - 1 - r;
+    if (ndep > 0) {
+        // stage L_{r+1,r} row-major in Lt (Lt[n][i]), then P = L_{r+1,r} L_rr^-1
+        gload(r + 1);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Lt[tr][tc + q] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    Lt[16 * nb + lr][4 * kk + lk], Lrr[(size_t)(4 * kk + lk) * NB + wr + lr], acc,
+                    0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pt[nb][q] = -acc[q];
+        }
+        __syncthreads();
+    }
+    d4 acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = wc + lr;
+        acc[q] = (c < nrhs) ? B[(size_t)c * ldb + rb + wr + 4 * q + lk] : 0.0;
+    }
+    // contributions of blocks c = nblk-1 .. r+2
+    if (ndep > 1) gload(nblk - 1);
+    for (int s = 0; s + 1 < ndep; ++s) {
+        const int cb = nblk - 1 - s;
+        wait_flag(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
+        load_ys(cb);
+        sstore_t();
+        if (s + 2 < ndep) gload(nblk - 2 - s);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
+                                                       Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
+        __syncthreads();
+    }
+    // Z = L_rr^-T acc (acc through Ys as the B operand)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Ys[wr + 4 * q + lk][wc + lr] = acc[q];
+    __syncthreads();
+    d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
+    if (ndep > 0) {
+        // X_r = Z - P^T X_{r+1}
+        wait_flag(flags + r + 1, epoch, status);
+        load_ys(r + 1);
+        __syncthreads();
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                x = __builtin_amdgcn_mfma_f64_16x16x4f64(pt[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr],
+                                                         x, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = wc + lr;
+        if (c < nrhs) B[(size_t)c * ldb + rb + wr + 4 * q + lk] = x[q];
+    }
+    publish(flags + r, epoch);
+}
+
 // Mixed form: 64-column panels on the FP32 factor.
 rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
     const int Vp = ctx->nblk * NB;
@@ -1121,10 +1246,10 @@ rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
     for (int c0 = 0; c0 < nrhs; c0 += RHS_CHUNK) {
         const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
         const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL((k_trsm<double, false>), dim3(ctx->nblk), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_trsm_bwd, dim3(ctx->nblk), dim3(512), 0, ctx->stream,
                            (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv,
                            d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
-                           ctx->d_status, (const int32_t*)nullptr);
+                           ctx->d_status);
         RC_CHECK_LAUNCH();
     }
     return RC_OK;
@@ -1148,6 +1273,10 @@ int rc_trsm_max_coresident(void) {
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_trsm<float, false>, 256, 0) != hipSuccess)
         return 0;
+    int per5 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per5, k_trsm_bwd, 512, 0) != hipSuccess)
+        return 0;
+    if (per5 < per4) per4 = per5;
     int m = per;
     if (per2 < m) m = per2;
     if (per3 < m) m = per3;
