@@ -88,7 +88,8 @@ static void free_all(rc_ctx* ctx) {
     void* ptrs[] = {ctx->d_i1, ctx->d_i2, ctx->d_vOff, ctx->d_vEdge, ctx->d_vSlot, ctx->d_vLocal,
                     ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_interior, ctx->d_pos,
                     ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
-                    ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_A,
+                    ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_Adiag,
+                    ctx->d_symv,
                     ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
                     ctx->d_flags, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
@@ -255,7 +256,8 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_scal, 16);
     ALLOC(ctx->d_rsp, (size_t)ctx->nvt * 2 * ctx->E);
     ALLOC(ctx->d_rs, 2 * (size_t)ctx->E);
-    ALLOC(ctx->d_A, (size_t)V * V);
+    ALLOC(ctx->d_Adiag, (size_t)Vp);
+    ALLOC(ctx->d_symv, (size_t)ctx->nblk * ctx->nblk * 3 * RC_NB);
     // factor plus the right-hand-side rows the factorisation carries along
     // (3 + 3k <= 3 + 3 RC_KMAX rows, whole 64-row blocks)
     ALLOC(ctx->d_F, (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
@@ -294,6 +296,12 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ctx->host_stage_bytes = 4096;
     he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
+    // the factor storage starts finite (its padding outside V x V is read
+    // by the factorisation's tiles)
+    if (he == hipSuccess)
+        he = hipMemset(ctx->d_F, 0,
+                       sizeof(double) * (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
+    if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
     if (he != hipSuccess) {
         ctx->err = std::string("hipMemset: ") + hipGetErrorString(he);
         return fail(RC_HIP_ERROR);
@@ -373,8 +381,11 @@ extern "C" rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_para
     if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
     if (s == RC_OK) s = rc_launch_gram(ctx, p);
     if (s != RC_OK) return s;
-    RC_HIP(hipMemcpyAsync(A, ctx->d_A, sizeof(double) * (size_t)ctx->V * ctx->V,
-                          hipMemcpyDeviceToHost, ctx->stream));
+    // A sits in the factor storage (leading dimension Vp)
+    const size_t ld = sizeof(double) * (size_t)ctx->nblk * RC_NB;
+    RC_HIP(hipMemcpy2DAsync(A, sizeof(double) * (size_t)ctx->V, ctx->d_F, ld,
+                            sizeof(double) * (size_t)ctx->V, ctx->V, hipMemcpyDeviceToHost,
+                            ctx->stream));
     RC_HIP(hipStreamSynchronize(ctx->stream));
     return RC_OK;
 }

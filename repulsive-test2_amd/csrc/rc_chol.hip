@@ -45,10 +45,17 @@ namespace {
 constexpr int NB = RC_NB;
 constexpr int RHS_CHUNK = 32;
 
-__global__ void k_trace_mu(const double* __restrict__ A, int V, double* __restrict__ mu) {
+// mu = trace(A) / V^2; the diagonal of A is kept in diag (the residual reads
+// A from the upper triangle of F and this diagonal once F holds the factor).
+__global__ void k_trace_mu(const double* __restrict__ A, int V, int ldA, double* __restrict__ mu,
+                           double* __restrict__ diag) {
     __shared__ double sh[1024];
     double s = 0.0;
-    for (int i = threadIdx.x; i < V; i += 1024) s += A[(size_t)i * V + i];
+    for (int i = threadIdx.x; i < V; i += 1024) {
+        const double a = A[(size_t)i * ldA + i];
+        diag[i] = a;
+        s += a;
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
@@ -59,17 +66,17 @@ __global__ void k_trace_mu(const double* __restrict__ A, int V, double* __restri
 }
 
 // dinv[i] = 1/sqrt(A_ii + mu) (1 on padding)
-__global__ void k_diag_scale(const double* __restrict__ A, int V, int Vp,
+__global__ void k_diag_scale(const double* __restrict__ A, int V, int ldA, int Vp,
                              const double* __restrict__ mu, double* __restrict__ dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Vp) return;
-    dinv[i] = (i < V) ? 1.0 / __builtin_sqrt(A[(size_t)i * V + i] + mu[0]) : 1.0;
+    dinv[i] = (i < V) ? 1.0 / __builtin_sqrt(A[(size_t)i * ldA + i] + mu[0]) : 1.0;
 }
 
-// F (lower) = A + mu (FP64 form) or dinv_i (A + mu) dinv_j + shift delta_ij
-// (FP32 form); identity padding.
+// F32 (lower) = dinv_i (A + mu) dinv_j + shift delta_ij (mixed form; A in
+// the FP64 buffer, leading dimension ldA); identity padding.
 template <typename T>
-__global__ void k_factor_prep(const double* __restrict__ A, int V, int Vp,
+__global__ void k_factor_prep(const double* __restrict__ A, int V, int ldA, int Vp,
                               const double* __restrict__ mu, const double* __restrict__ dinv,
                               double shift, T* __restrict__ F) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,13 +84,23 @@ __global__ void k_factor_prep(const double* __restrict__ A, int V, int Vp,
     if (j >= Vp) return;
     double v;
     if (i < V && j < V) {
-        v = (j <= i) ? A[(size_t)i * V + j] + mu[0] : 0.0;
+        v = (j <= i) ? A[(size_t)i * ldA + j] + mu[0] : 0.0;
         if (dinv) v = v * dinv[i] * dinv[j];
         if (i == j) v += shift;
     } else {
         v = (i == j) ? 1.0 : 0.0;
     }
     F[(size_t)i * Vp + j] = (T)v;
+}
+
+// FP64 factor storage, padding rows V <= i < Vp: identity (written before
+// every factorisation, so that a breakdown of an earlier one cannot leave
+// non-finite values there). Their factor is the identity again.
+__global__ void k_pad_identity(double* __restrict__ F, int V, int Vp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = V + (int)blockIdx.y;
+    if (j >= Vp) return;
+    F[(size_t)i * Vp + j] = (i == j) ? 1.0 : 0.0;
 }
 
 // Factor the 64x64 diagonal block held in D (lower) in place and write its
@@ -243,6 +260,7 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
 __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp, int kb,
                                                   double* __restrict__ Linv128,
                                                   double* __restrict__ Linv64,
+                                                  const double* __restrict__ mu, int V,
                                                   int32_t* __restrict__ status) {
     __shared__ double Am[PB][LDA];
     __shared__ double Dv[PB / SB][SB][SB + 1];
@@ -254,7 +272,10 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
     {
         // wave w loads rows w, w + 4, ...: lane l the column pair 2l, 2l + 1
         // (one 1 KiB row per wave instruction), 32 loads in flight
+        // (the first panel reads A itself: A_hat = A + mu on the V x V part)
         const int m = 2 * lane;
+        const double mu0 = (kb == 0) ? mu[0] : 0.0;
+        const bool sh0 = kb == 0 && m < V, sh1 = kb == 0 && m + 1 < V;
         double2 v[32];
 #pragma unroll
         for (int q = 0; q < 32; ++q)
@@ -262,8 +283,11 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
 #pragma unroll
         for (int q = 0; q < 32; ++q) {
             const int i = w + 4 * q;
-            Am[i][m] = (m <= i) ? v[q].x : 0.0;
-            Am[i][m + 1] = (m + 1 <= i) ? v[q].y : 0.0;
+            const bool si = i < V;
+            const double x0 = (sh0 && si) ? v[q].x + mu0 : v[q].x;
+            const double x1 = (sh1 && si) ? v[q].y + mu0 : v[q].y;
+            Am[i][m] = (m <= i) ? x0 : 0.0;
+            Am[i][m + 1] = (m + 1 <= i) ? x1 : 0.0;
         }
     }
     __syncthreads();
@@ -390,16 +414,18 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
     }
     if (t == 0 && bad) atomicOr(status, 1);
     // wave w writes rows w, w + 4, ...: lane l the column pair m = 2l, 2l + 1.
-    // F gets whole rows of the block (the part above the diagonal is never
-    // read); X = L^-1 is D_p on diagonal 16x16 blocks, the transposed upper
-    // blocks of Am below them and zero above.
+    // F gets the lower triangle of the block (the part above the diagonal
+    // keeps A for the residual); X = L^-1 is D_p on diagonal 16x16 blocks,
+    // the transposed upper blocks of Am below them and zero above.
     {
         double* X = Linv128 + k0 * PB;
         const int m = 2 * lane, bm = m >> 4, h = m >> 6;
 #pragma unroll 4
         for (int q = 0; q < 32; ++q) {
             const int i = w + 4 * q, bi = i >> 4;
-            *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) = make_double2(Am[i][m], Am[i][m + 1]);
+            double* dst = F + (k0 + i) * Vp + k0 + m;
+            if (m + 1 <= i) *reinterpret_cast<double2*>(dst) = make_double2(Am[i][m], Am[i][m + 1]);
+            else if (m == i) dst[0] = Am[i][m];
             double2 xv;
             if (bm > bi) xv = make_double2(0.0, 0.0);
             else if (bm == bi) xv = make_double2(Dv[bi][i & 15][m & 15], Dv[bi][i & 15][(m & 15) + 1]);
@@ -427,15 +453,21 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
 // loop: a read-modify-write after the loop would expose the load latency,
 // and accumulating onto a preloaded -C would round every partial sum at the
 // magnitude of C instead of the panel's contribution.
+// aload(x, row, col) maps each A operand value (tile-local row and column)
+// on its way into LDS.
 struct NoHook {
     __device__ void operator()(int) const {}
 };
+struct AsIs {
+    __device__ double operator()(double x, int, int) const { return x; }
+};
 
-template <int MT, int KC, int NC = PB, class Hook = NoHook>
+template <int MT, int KC, int NC = PB, class Hook = NoHook, class ALoad = AsIs>
 __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int lda,
                                             const double* __restrict__ B, int ldb,
                                             double (*As)[KC][LDT], double (*Bs)[KC][LDT],
-                                            d4 (&acc)[MT / 32][NC / 32], Hook hook = Hook()) {
+                                            d4 (&acc)[MT / 32][NC / 32], Hook hook = Hook(),
+                                            ALoad aload = ALoad()) {
     constexpr int TA = MT / 32, TB = NC / 32;
     constexpr int NA = MT * KC / 256;   // A doubles per thread per chunk
     constexpr int NBv = NC * KC / 256;  // B doubles per thread per chunk
@@ -460,14 +492,14 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
             rb[q + 1] = v.y;
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int k) {
 #pragma unroll
-        for (int q = 0; q < NA; ++q) As[buf][acol + q][arow] = ra[q];
+        for (int q = 0; q < NA; ++q) As[buf][acol + q][arow] = aload(ra[q], arow, k + acol + q);
 #pragma unroll
         for (int q = 0; q < NBv; ++q) Bs[buf][bcol + q][brow] = rb[q];
     };
     gload(0);
-    sstore(0);
+    sstore(0, 0);
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
     for (int ch = 0; ch < PB / KC; ++ch) {
@@ -490,7 +522,7 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
         // keep the next chunk's LDS stores (and their wait on the global
         // loads) after this chunk's MFMAs, so the loads overlap the MFMAs
         __builtin_amdgcn_sched_barrier(0);
-        if (ch + 1 < PB / KC) sstore(buf ^ 1);
+        if (ch + 1 < PB / KC) sstore(buf ^ 1, (ch + 1) * KC);
         __syncthreads();
     }
 }
@@ -530,10 +562,20 @@ __device__ __forceinline__ void signal_done(int32_t* cnt) {
 // Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
 // workgroup, in place. wcnt (nullable): the previous panel's column update
 // (k_col_rows, other stream) must have counted `target` workgroups first.
+// A_hat = A + mu on operand values of the V x V part (first panel).
+struct ShiftA {
+    double mu;
+    int rows, cols;   // tile-local bounds of the V x V part
+    __device__ double operator()(double x, int r, int c) const {
+        return (r < rows && c < cols) ? x + mu : x;
+    }
+};
+
 __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int Vp, int kb,
                                                     const double* __restrict__ Linv128,
                                                     const int32_t* __restrict__ wcnt,
-                                                    int32_t target, int32_t* __restrict__ status) {
+                                                    int32_t target, const double* __restrict__ mu,
+                                                    int V, int32_t* __restrict__ status) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
     if (wcnt) wait_flag(wcnt, target, status);
@@ -544,7 +586,11 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    gemm_nt_f64<64, KC_TRSM>(F + r0 * Vp + k0, Vp, Linv128 + k0 * PB, PB, As, Bs, acc);
+    if (kb == 0)   // the first panel reads A itself
+        gemm_nt_f64<64, KC_TRSM>(F + r0 * Vp, Vp, Linv128, PB, As, Bs, acc, NoHook{},
+                                 ShiftA{mu[0], V - (int)r0, V});
+    else
+        gemm_nt_f64<64, KC_TRSM>(F + r0 * Vp + k0, Vp, Linv128 + k0 * PB, PB, As, Bs, acc);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
 #pragma unroll
@@ -575,7 +621,8 @@ __device__ __forceinline__ void decode_half(int p, int& ti, int& hj) {
 }
 
 __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
-                                                 int nblocks, int32_t* __restrict__ diag_done) {
+                                                 int nblocks, const double* __restrict__ mu, int V,
+                                                 int32_t* __restrict__ diag_done) {
     __shared__ double As[2][KC_SYRK][LDT];
     __shared__ double Bs[2][KC_SYRK][LDT];
     const int per = (nblocks + 7) >> 3;
@@ -608,6 +655,9 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
     });
     const bool diag = (hj >> 1) == ti;
     const int jo = (hj & 1) * 64;
+    // the first panel's update reads A itself: A_hat = A + mu on the V x V part
+    const double mu0 = (kb == 0) ? mu[0] : 0.0;
+    const int vi = (kb == 0) ? V - (int)ri : 0, vj = (kb == 0) ? V - (int)rj : 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -615,9 +665,10 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
-                const int j = jo + wc + 16 * b + (lane & 15);
+                const int jl = wc + 16 * b + (lane & 15), j = jo + jl;
+                const double c = (i < vi && jl < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
                 if (!diag || j <= i)
-                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
+                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
             }
     // blocks 0 and 1 (scheduled first) are the diagonal tile of the panel
     // after next, which k_col_diag of the next panel waits for
@@ -632,7 +683,8 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
 // right-hand-side rows below the factor (rows Vp ..): 64-row block x, all nt
 // trailing columns tj, workgroup 2(nt-1) + x nt + tj.
 __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb,
-                                                     int nt, int32_t* __restrict__ done) {
+                                                     int nt, const double* __restrict__ mu, int V,
+                                                     int32_t* __restrict__ done) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
     const size_t k0 = (size_t)kb * PB;
@@ -667,13 +719,19 @@ __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int
                         cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
         }
     });
+    // first panel: A_hat = A + mu on the V x V part (not the right-hand sides)
+    const double mu0 = (kb == 0) ? mu[0] : 0.0;
+    const int vi = (kb == 0) ? V - (int)ri : 0, vj = (kb == 0) ? V - (int)rj : 0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
+            for (int r = 0; r < 4; ++r) {
+                const int i = wr + 16 * a + 4 * r + (lane >> 4), j = wc + 16 * b + (lane & 15);
+                const double c = (i < vi && j < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
+                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
+            }
     signal_done(done);
 }
 
@@ -682,6 +740,7 @@ __global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int
 // workgroup each (block (hi, hj) = (0,0), (1,0), (1,1)).
 __global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int Vp, int kb,
                                                      const int32_t* __restrict__ wcnt,
+                                                     const double* __restrict__ mu, int V,
                                                      int32_t* __restrict__ status) {
     __shared__ double As[2][KC_TRSM][LDT];
     __shared__ double Bs[2][KC_TRSM][LDT];
@@ -712,6 +771,9 @@ __global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int
                         cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
         }
     });
+    // first panel: A_hat = A + mu on the V x V part
+    const double mu0 = (kb == 0) ? mu[0] : 0.0;
+    const int vi = (kb == 0) ? V - (int)ri : 0, vj = (kb == 0) ? V - (int)rj : 0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -720,8 +782,9 @@ __global__ __launch_bounds__(256, 2) void k_col_diag(double* __restrict__ F, int
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
                 const int j = wc + 16 * b + (lane & 15);
+                const double c = (i < vi && j < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
                 if (hi != hj || j <= i)
-                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = cv[a][b][r] - acc[a][b][r];
+                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
             }
 }
 
@@ -1088,6 +1151,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     hipEvent_t fin = ctx->dep_ev[np + 1];
     int32_t* colcnt = ctx->d_fcnt;        // [kb]: k_col_rows(kb) workgroups done
     int32_t* diagcnt = ctx->d_fcnt + np;  // [kb]: k_syrk128(kb) blocks of tile kb+2 done
+    const double* mu = ctx->d_scal + 1;
     RC_HIP(hipMemsetAsync(ctx->d_fcnt, 0, sizeof(int32_t) * 2 * (size_t)np, ctx->stream));
     // the second stream starts after the factor's input is prepared
     RC_HIP(hipEventRecord(start, ctx->stream));
@@ -1112,7 +1176,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     };
     for (int kb = 0; kb < np; ++kb) {
         hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
-                           ctx->d_Linv128, ctx->d_Linv, ctx->d_status);
+                           ctx->d_Linv128, ctx->d_Linv, mu, ctx->V, ctx->d_status);
         RC_CHECK_LAUNCH();
         const int nt = np - kb - 1;
         if (nt == 0 && nx == 0) break;
@@ -1124,20 +1188,21 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
         hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt + nx), dim3(256), 0, ctx->stream, ctx->d_F, Vp,
                            kb, (const double*)ctx->d_Linv128,
                            (const int32_t*)(prev_cols > 0 ? colcnt + kb - 1 : nullptr), prev_cols,
-                           ctx->d_status);
+                           mu, ctx->V, ctx->d_status);
         RC_CHECK_LAUNCH();
         RC_HIP(hipEventRecord(T[kb], ctx->stream));
         if (nt == 0) break;
         // next diagonal tile: after the rest-update of panel kb-1 has left it
         hipLaunchKernelGGL(k_col_diag, dim3(3), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
-                           (const int32_t*)(kb > 0 ? diagcnt + kb - 1 : nullptr), ctx->d_status);
+                           (const int32_t*)(kb > 0 ? diagcnt + kb - 1 : nullptr), mu, ctx->V,
+                           ctx->d_status);
         RC_CHECK_LAUNCH();
         // second stream: the rest of column kb+1, then the rest of the
         // trailing matrix, both with panel kb
         RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
         if (2 * (nt - 1) + nx * nt > 0) {
             hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1) + nx * nt), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nt, colcnt + kb);
+                               ctx->d_F, Vp, kb, nt, mu, ctx->V, colcnt + kb);
             RC_CHECK_LAUNCH();
         }
         // 128 x 64 blocks of the lower triangle right of column kb+1
@@ -1146,7 +1211,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
             const double mr = (double)(nt - 1) * PB;
             hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
             hipLaunchKernelGGL(k_syrk128, dim3(8 * ((nblocks + 7) / 8)), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nblocks, diagcnt + kb);
+                               ctx->d_F, Vp, kb, nblocks, mu, ctx->V, diagcnt + kb);
             RC_CHECK_LAUNCH();
             if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream2));
         }
@@ -1159,19 +1224,25 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
 
 }  // namespace
 
-// Factor A_hat = A + mu 11^T into ctx->d_F (FP64). mu = trace(A)/V^2 is kept
-// in d_scal[1]. A non-positive pivot sets status bit 1 (singular). nx 64-row
-// blocks of right-hand sides B^T below the factor (rows Vp .. Vp + 64 nx - 1)
-// are overwritten with (L^-1 B)^T.
+// Factor A_hat = A + mu 11^T in place in ctx->d_F (FP64), which holds A (the
+// Gram assembly's output, leading dimension Vp). mu = trace(A)/V^2 is kept
+// in d_scal[1], the diagonal of A in d_Adiag. The factor replaces the lower
+// triangle; the strict upper triangle keeps A. A non-positive pivot sets
+// status bit 1 (singular). nx 64-row blocks of right-hand sides B^T below the
+// factor (rows Vp .. Vp + 64 nx - 1) are overwritten with (L^-1 B)^T.
 rc_status rc_factor_fp64(rc_ctx* ctx, int nx) {
     const int Vp = ctx->nblk * NB;
     RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
-    hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_A, ctx->V,
-                       ctx->d_scal + 1);
+    if (Vp > ctx->V) {
+        hipLaunchKernelGGL(k_pad_identity, dim3((Vp + 255) / 256, Vp - ctx->V), dim3(256), 0,
+                           ctx->stream, ctx->d_F, ctx->V, Vp);
+        RC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, (const double*)ctx->d_F,
+                       ctx->V, Vp, ctx->d_scal + 1, ctx->d_Adiag);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_factor_prep<double>, dim3((Vp + 255) / 256, Vp), dim3(256), 0, ctx->stream,
-                       ctx->d_A, ctx->V, Vp, ctx->d_scal + 1, (const double*)nullptr, 0.0, ctx->d_F);
-    RC_CHECK_LAUNCH();
+    // no separate pass forms A_hat: the first panel's kernels add mu where
+    // they read A
     // The panel loop is launched eagerly: the host enqueues it ~3 ms ahead of
     // the GPU. (Replaying it as a captured HIP graph measured 0.25 ms slower
     // per step: hipGraphLaunch holds the host for the whole factorisation, so
@@ -1194,17 +1265,17 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
         RC_HIP(hipMalloc((void**)&ctx->d_Linv32, sizeof(float) * (size_t)ctx->nblk * NB * NB));
         RC_HIP(hipMalloc((void**)&ctx->d_dscale, sizeof(double) * (size_t)Vp));
     }
-    hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_A, ctx->V,
-                       ctx->d_scal + 1);
+    hipLaunchKernelGGL(k_trace_mu, dim3(1), dim3(1024), 0, ctx->stream, (const double*)ctx->d_F,
+                       ctx->V, Vp, ctx->d_scal + 1, ctx->d_Adiag);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_diag_scale, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_A,
-                       ctx->V, Vp, ctx->d_scal + 1, ctx->d_dscale);
+    hipLaunchKernelGGL(k_diag_scale, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double*)ctx->d_F, ctx->V, Vp, Vp, ctx->d_scal + 1, ctx->d_dscale);
     RC_CHECK_LAUNCH();
     double shift = 1.0 / 1048576.0;
     for (int attempt = 0; attempt < 4; ++attempt, shift *= 8.0) {
         RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
         hipLaunchKernelGGL(k_factor_prep<float>, dim3((Vp + 255) / 256, Vp), dim3(256), 0,
-                           ctx->stream, ctx->d_A, ctx->V, Vp, ctx->d_scal + 1,
+                           ctx->stream, (const double*)ctx->d_F, ctx->V, Vp, Vp, ctx->d_scal + 1,
                            (const double*)ctx->d_dscale, shift, ctx->d_F32);
         RC_CHECK_LAUNCH();
         rc_status s = factor_loop_mixed(ctx, ctx->d_F32, ctx->d_Linv32);

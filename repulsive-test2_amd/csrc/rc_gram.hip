@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
                                                     const int32_t* __restrict__ vOff,
                                                     const int32_t* __restrict__ vSlot,
                                                     const int32_t* __restrict__ vLocal,
-                                                    double* __restrict__ A,
+                                                    double* __restrict__ A, int ldA,
                                                     double* __restrict__ rsp) {
     __shared__ GEdge GU, GV;
     __shared__ double Xt[RC_RMAX][RC_RMAX + 1];
@@ -239,9 +239,9 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
                     }
                 }
                 if (pass == 0)
-                    A[(size_t)u * V + v] = val;
+                    A[(size_t)u * ldA + v] = val;
                 else
-                    A[(size_t)v * V + u] = val;
+                    A[(size_t)v * ldA + u] = val;
             }
         }
         __syncthreads();
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(64) void k_rowsum_reduce(const double* __restrict__
 // same value from both of its vertices' threads (exact symmetry).
 __global__ void k_gram_band(EdgeSoA ed, const double* __restrict__ rs, int V, int E,
                             const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
-                            double* __restrict__ A) {
+                            double* __restrict__ A, int ldA) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= V) return;
     double dsum = 0.0;
@@ -294,9 +294,9 @@ __global__ void k_gram_band(EdgeSoA ed, const double* __restrict__ rs, int V, in
         const double qk = rs[E + K];
         dsum += wk + qk;
         const int v = (ed.i1[K] == u) ? ed.i2[K] : ed.i1[K];
-        A[(size_t)u * V + v] += qk - wk;
+        A[(size_t)u * ldA + v] += qk - wk;
     }
-    A[(size_t)u * V + u] += dsum;
+    A[(size_t)u * ldA + u] += dsum;
 }
 
 }  // namespace
@@ -307,21 +307,22 @@ rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p) {
         hipLaunchKernelGGL(k_gram_tiles<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
                            Pow36{}, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram, ctx->d_rOff,
                            ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot, ctx->d_vLocal,
-                           ctx->d_A, ctx->d_rsp);
+                           ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
     } else {
         const double s = (p->beta - 1) / p->alpha;
         PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
         hipLaunchKernelGGL(k_gram_tiles<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream,
                            ctx->ed, pw, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram,
                            ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot,
-                           ctx->d_vLocal, ctx->d_A, ctx->d_rsp);
+                           ctx->d_vLocal, ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
     }
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 63) / 64), dim3(64), 0, ctx->stream,
                        ctx->d_rsp, ctx->nvt, ctx->E, ctx->d_rs);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_gram_band, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
-                       ctx->d_rs, ctx->V, ctx->E, ctx->d_vOff, ctx->d_vEdge, ctx->d_A);
+                       ctx->d_rs, ctx->V, ctx->E, ctx->d_vOff, ctx->d_vEdge, ctx->d_F,
+                       ctx->nblk * RC_NB);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }

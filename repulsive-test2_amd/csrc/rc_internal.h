@@ -71,9 +71,12 @@ struct rc_ctx {
     // Gram assembly
     double* d_rsp = nullptr;      // row-sum partials [nvt][2][E]
     double* d_rs = nullptr;       // row sums W_K, Q_K (2E)
-    double* d_A = nullptr;        // V x V FP64
-    double* d_F = nullptr;        // Vp x Vp FP64 factor (lower) of A + mu 11^T, then
-                                  // RC_XROWS right-hand-side rows (B^T, then (L^-1 B)^T)
+    double* d_F = nullptr;        // Vp x Vp FP64: A (Gram assembly output), then its
+                                  // factor (lower) of A + mu 11^T with A kept in the
+                                  // strict upper triangle; below, the right-hand-side
+                                  // rows (B^T, then (L^-1 B)^T)
+    double* d_Adiag = nullptr;    // diagonal of A (Vp), saved before the factorisation
+    double* d_symv = nullptr;     // residual product partials [nblk][nblk][3][NB]
     float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
     double* d_Linv = nullptr;     // inverted diagonal blocks [nblk][NB][NB]
     double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_ahat_gemm(const double* __restrict__ A,
     if (skip && *skip) return;
     __shared__ double sh[4][GEMM_CHUNK];
     const int u = blockIdx.x;
-    const double* row = A + (size_t)u * V;
+    const double* row = A + (size_t)u * Vp;   // A in the FP64 factor storage (ld Vp)
     double acc[GEMM_CHUNK];
 #pragma unroll
     for (int c = 0; c < GEMM_CHUNK; ++c) acc[c] = 0.0;
@@ -219,7 +219,7 @@ static rc_status ahat_mult(rc_ctx* ctx, const double* X, double* Q, double* cs, 
     RC_CHECK_LAUNCH();
     for (int c0 = 0; c0 < nrhs; c0 += GEMM_CHUNK) {
         const int nc = (nrhs - c0) < GEMM_CHUNK ? (nrhs - c0) : GEMM_CHUNK;
-        hipLaunchKernelGGL(k_ahat_gemm, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_A, ctx->V,
+        hipLaunchKernelGGL(k_ahat_gemm, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_F, ctx->V,
                            Vp, ctx->d_scal + 1, cs + c0, X + (size_t)c0 * Vp, nc,
                            Q + (size_t)c0 * Vp, skip);
         RC_CHECK_LAUNCH();

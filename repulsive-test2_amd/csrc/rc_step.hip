@@ -363,28 +363,122 @@ __global__ void k_gtilde(const double* __restrict__ X, const double* __restrict_
     gt[(size_t)c * Vp + v] = s;
 }
 
-// y_c = A x_c for the three coordinate blocks (x stride Vp, y stride Vp).
-__global__ __launch_bounds__(256) void k_gemv3(const double* __restrict__ A, int V, int Vp,
-                                               const double* __restrict__ x,
-                                               double* __restrict__ y) {
-    __shared__ double sh[32];
-    const int u = blockIdx.x;
-    const double* row = A + (size_t)u * V;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int j = threadIdx.x; j < V; j += 256) {
-        const double a = row[j];
-        s0 = fma(a, x[j], s0);
-        s1 = fma(a, x[Vp + j], s1);
-        s2 = fma(a, x[2 * (size_t)Vp + j], s2);
+// y_c = A x_c for the three coordinate blocks (x, y stride Vp), from the
+// strict upper triangle of the FP64 factor storage (which keeps A there:
+// the factorisation writes only the lower triangle) and the saved diagonal
+// d. One workgroup per 64 x 64 tile (I, J), I <= J, of the upper triangle,
+// each tile read once: its row sums are block I's partial from J and its
+// column sums (A is symmetric) block J's partial from I; a diagonal tile's
+// strict upper part gives both to one partial. part[I][J][c][i] is added up
+// by k_symv_reduce in a fixed order.
+__device__ __forceinline__ void upper_tile(int p, int& I, int& J) {
+    int j = (int)((__builtin_sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((j + 1) * (j + 2) / 2 <= p) ++j;
+    while (j * (j + 1) / 2 > p) --j;
+    J = j;
+    I = p - j * (j + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_symv_tiles(const double* __restrict__ F, int V, int Vp,
+                                                    int nblk, const double* __restrict__ x,
+                                                    double* __restrict__ part) {
+    __shared__ double rsh[RC_NB][3];
+    __shared__ double csh[4][RC_NB][3];
+    int I, J;
+    upper_tile(blockIdx.x, I, J);
+    const int t = threadIdx.x, w = t >> 6;
+    const int ri = t >> 4, cj = t & 15;   // tile rows 4 ri .. 4 ri + 3, columns 4 cj .. 4 cj + 3
+    const int i0 = I * RC_NB + 4 * ri, j0 = J * RC_NB + 4 * cj;
+    double a[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double* p = F + (size_t)(i0 + r) * Vp + j0;
+        const double2 v0 = *reinterpret_cast<const double2*>(p);
+        const double2 v1 = *reinterpret_cast<const double2*>(p + 2);
+        a[r][0] = v0.x;
+        a[r][1] = v0.y;
+        a[r][2] = v1.x;
+        a[r][3] = v1.y;
     }
-    s0 = block_sum(s0, sh);
-    s1 = block_sum(s1, sh);
-    s2 = block_sum(s2, sh);
-    if (threadIdx.x == 0) {
-        y[u] = s0;
-        y[Vp + u] = s1;
-        y[2 * (size_t)Vp + u] = s2;
+    double xi[4][3], xj[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xi[q][c] = x[(size_t)c * Vp + i0 + q];
+            xj[q][c] = x[(size_t)c * Vp + j0 + q];
+        }
+    // only the V x V part, and on a diagonal tile only above the diagonal
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = i0 + r < V && j0 + k < V && (I < J || j0 + k > i0 + r);
+            a[r][k] = on ? a[r][k] : 0.0;
+        }
+    double rs[4][3], cs[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double u = 0.0, v = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                u = fma(a[q][m], xj[m][c], u);
+                v = fma(a[m][q], xi[m][c], v);
+            }
+            rs[q][c] = u;
+            cs[q][c] = v;
+        }
+    // row sums over the 16 column groups (lanes 1, 2, 4, 8 apart), column
+    // sums over the wave's 4 row groups (lanes 16, 32 apart); a butterfly
+    // leaves the same bits in every lane
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) rs[q][c] += __shfl_xor(rs[q][c], off, 64);
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cs[q][c] += __shfl_xor(cs[q][c], off, 64);
+    if (cj == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) rsh[4 * ri + q][c] = rs[q][c];
+    if (((t >> 4) & 3) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) csh[w][4 * cj + q][c] = cs[q][c];
+    __syncthreads();
+    if (t < 3 * RC_NB) {
+        const int c = t >> 6, i = t & 63;
+        const double col = ((csh[0][i][c] + csh[1][i][c]) + csh[2][i][c]) + csh[3][i][c];
+        if (I == J) {
+            part[(((size_t)I * nblk + I) * 3 + c) * RC_NB + i] = rsh[i][c] + col;
+        } else {
+            part[(((size_t)I * nblk + J) * 3 + c) * RC_NB + i] = rsh[i][c];
+            part[(((size_t)J * nblk + I) * 3 + c) * RC_NB + i] = col;
+        }
     }
+}
+
+// y_c[v] = d_v x_c[v] + sum over J (in order) of part[I][J][c][v - 64 I].
+__global__ __launch_bounds__(64) void k_symv_reduce(const double* __restrict__ part,
+                                                    const double* __restrict__ d,
+                                                    const double* __restrict__ x, int V, int Vp,
+                                                    int nblk, double* __restrict__ y) {
+    const int I = blockIdx.x, c = blockIdx.y, i = threadIdx.x, v = I * RC_NB + i;
+    const double* p = part + ((size_t)I * nblk * 3 + c) * RC_NB + i;
+    double s = (v < V) ? d[v] * x[(size_t)c * Vp + v] : 0.0;
+#pragma unroll 8
+    for (int J = 0; J < nblk; ++J) s += p[(size_t)J * 3 * RC_NB];
+    y[(size_t)c * Vp + v] = s;
 }
 
 // ||K z - rhs|| / max(1, ||rhs||) (linsolve.ts:810-849) with z = [g~; lambda],
@@ -881,7 +975,12 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
 rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
                              const double* d_sol, double* d_Ax, double* d_Cg, LSCtl* d_ctl) {
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_gemv3, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_A, ctx->V, Vp, gt,
+    const int nb = ctx->nblk;
+    hipLaunchKernelGGL(k_symv_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, ctx->stream,
+                       (const double*)ctx->d_F, ctx->V, Vp, nb, gt, ctx->d_symv);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_symv_reduce, dim3(nb, 3), dim3(RC_NB), 0, ctx->stream,
+                       (const double*)ctx->d_symv, (const double*)ctx->d_Adiag, gt, ctx->V, Vp, nb,
                        d_Ax);
     RC_CHECK_LAUNCH();
     if (k > 0) {
