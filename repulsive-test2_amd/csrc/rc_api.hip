@@ -90,7 +90,7 @@ static void free_all(rc_ctx* ctx) {
                     ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
                     ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_Adiag,
                     ctx->d_symv,
-                    ctx->d_F, ctx->d_F32, ctx->d_Linv, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
+                    ctx->d_F, ctx->d_F32, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
                     ctx->d_flags, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
                     ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
@@ -261,7 +261,6 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     // factor plus the right-hand-side rows the factorisation carries along
     // (3 + 3k <= 3 + 3 RC_KMAX rows, whole 64-row blocks)
     ALLOC(ctx->d_F, (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
-    ALLOC(ctx->d_Linv, (size_t)ctx->nblk * RC_NB * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, ctx->nblk);
     ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
@@ -302,6 +301,9 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         he = hipMemset(ctx->d_F, 0,
                        sizeof(double) * (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
+    // the inverse's zero blocks above the diagonal are never written
+    if (he == hipSuccess)
+        he = hipMemset(ctx->d_Linv128, 0, sizeof(double) * (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     if (he != hipSuccess) {
         ctx->err = std::string("hipMemset: ") + hipGetErrorString(he);
         return fail(RC_HIP_ERROR);

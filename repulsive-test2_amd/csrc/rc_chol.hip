@@ -35,6 +35,7 @@
 // stored factor; the diagonal solve is a product with the stored L_kk^-1.
 #include "rc_internal.h"
 #include "rc_math.h"
+#include <utility>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -257,9 +258,78 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// 64-bit lane gather through the LDS crossbar (addr = 4 x source lane)
+__device__ __forceinline__ double bperm_f64(double v, int addr) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)u);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Every lane gets the value of lane (g, J) of its own 16-lane row g
+// (DPP row_newbcast: a VALU move, no LDS round trip).
+template <int J>
+__device__ __forceinline__ double row_bcast(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, 0x150 + J, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), 0x150 + J, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <class Fn, int... J>
+__device__ __forceinline__ void static_for(Fn&& fn, std::integer_sequence<int, J...>) {
+    (fn(std::integral_constant<int, J>{}), ...);
+}
+
+// State of the 16 x 16 diagonal-block factorisation on one wave, lane
+// (g, c) = 16 g + c: e = rows 4 g + rr of column c of the trailing block,
+// mirrored from the lower triangle so that it is symmetric (a_Jc = a_cJ
+// serves as the pivot row); x = the same rows of X~ = L~^-1, the inverse of
+// the unit lower factor of A = L~ D L~^T; m[rr] = 1 while row 4 g + rr is
+// below the pivot; lp = the previous pivot's masked column, rp = its 1/a_JJ.
+struct Piv16 {
+    double e[4], x[4], m[4], lp[4];
+    double rp;
+    int ad[4];   // bpermute addresses of lanes (R, c), R = 0..3
+    bool bad;
+};
+
+// Pivot J. The chain is broadcast -> 1/a_JJ (v_rcp_f64 and two Newton
+// steps) -> rank-1 update e -= a_iJ a_Jc / a_JJ on columns c > J (every row
+// takes it: rows <= J are finished and never read again, so the update is a
+// plain FMA). Column c keeps a_ic of its own pivot step from then on, which
+// gives L after the loop (no square root in the loop). Off the chain: row
+// J-1 of X~ (final now) is applied to the rows below it.
+template <int J>
+__device__ __forceinline__ void pivot16(Piv16& st, int g, int c) {
+    constexpr int R = J >> 2, RR = J & 3;
+    double aic[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) aic[rr] = row_bcast<J>(st.e[rr]);
+    const double acj = bperm_f64(st.e[RR], st.ad[R]);
+    double piv = readlane_f64(st.e[RR], R * 16 + J);
+    st.bad |= !(piv > 0.0);
+    piv = (piv > 0.0) ? piv : 1.0;
+    double r = __builtin_amdgcn_rcp(piv);
+    r = fma(r, fma(-piv, r, 1.0), r);
+    r = fma(r, fma(-piv, r, 1.0), r);
+    const double f = ((c > J) ? acj : 0.0) * r;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) st.e[rr] = fma(-aic[rr], f, st.e[rr]);
+    if constexpr (J > 0) {
+        constexpr int RP = (J - 1) / 4;
+        const double t = st.rp * bperm_f64(st.x[(J - 1) & 3], st.ad[RP]);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) st.x[rr] = fma(-st.lp[rr], t, st.x[rr]);
+    }
+    st.m[RR] = (g == R) ? 0.0 : st.m[RR];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) st.lp[rr] = aic[rr] * st.m[rr];
+    st.rp = r;
+}
+
 __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp, int kb,
                                                   double* __restrict__ Linv128,
-                                                  double* __restrict__ Linv64,
                                                   const double* __restrict__ mu, int V,
                                                   int32_t* __restrict__ status) {
     __shared__ double Am[PB][LDA];
@@ -291,58 +361,46 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
         }
     }
     __syncthreads();
-    // (b) lane (g, c) keeps rows i = 4g + rr of column c of the trailing
-    // block, mirrored from the lower triangle so that it is symmetric:
-    // a_jc = a_cj then serves as the pivot row. Pivot j: the pivot by
-    // readlane, the pivot column and row by lane shuffles; rows and columns
-    // <= j take masked (unused) updates so that every update is a plain FMA.
-    // The inverse D_p = L_pp^-1 is formed alongside (row j-1 of it is applied
-    // at pivot j, when it is final). Results stay in registers (selects, no
-    // stores and no branches in the pivot loop) and are written at the end.
+    // (b) the 16 pivots of diagonal block p on one wave (pivot16), then
+    // L_pp and its inverse D_p = L_pp^-1 from the pivots (results stay in
+    // registers during the pivot loop: selects, no stores, no branches).
     auto diag16 = [&](int p) {
         const int o = p * SB;
         const int g = lk, c = lr;
-        double e[4], x[4], lprev[4], lcol[4];
-        bool badl = false;
+        Piv16 st;
+        st.bad = false;
+        st.rp = 0.0;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             const int i = 4 * g + rr;
-            e[rr] = Am[o + (i > c ? i : c)][o + (i > c ? c : i)];
-            x[rr] = (i == c) ? 1.0 : 0.0;
-            lprev[rr] = 0.0;
-            lcol[rr] = 0.0;
+            st.e[rr] = Am[o + (i > c ? i : c)][o + (i > c ? c : i)];
+            st.x[rr] = (i == c) ? 1.0 : 0.0;
+            st.m[rr] = 1.0;
+            st.lp[rr] = 0.0;
+            st.ad[rr] = 4 * (16 * rr + c);
         }
-#pragma unroll
-        for (int j = 0; j < SB; ++j) {
-            double piv = readlane_f64(e[j & 3], (j >> 2) * 16 + j);
-            double aic[4];
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) aic[rr] = __shfl(e[rr], g * 16 + j);
-            const double acj = __shfl(e[j & 3], (j >> 2) * 16 + c);
-            const double xprev = (j > 0) ? __shfl(x[(j - 1) & 3], ((j - 1) >> 2) * 16 + c) : 0.0;
-            badl |= !(piv > 0.0);
-            piv = (piv > 0.0) ? piv : 1.0;
-            double s, is;
-            pivot_sqrt(piv, s, is);
-            const double f = (c > j) ? acj * (is * is) : 0.0;
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) e[rr] = fma(-aic[rr], f, e[rr]);
-            // off the pivot chain: inverse rows, column j of L
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const int i = 4 * g + rr;
-                x[rr] = fma(-lprev[rr], xprev, x[rr]);
-                if (rr == (j & 3)) x[rr] = (g == (j >> 2)) ? x[rr] * is : x[rr];
-                lcol[rr] = (c == j) ? ((i == j) ? s : aic[rr] * is) : lcol[rr];
-                lprev[rr] = (i > j) ? aic[rr] * is : 0.0;
-            }
-        }
+        static_for([&](auto jc) { pivot16<decltype(jc)::value>(st, g, c); },
+                   std::make_integer_sequence<int, SB>{});
+        // the pivots a_kk (e of lane (k >> 2, k), register k & 3): of this
+        // lane's column c and of its rows 4 g + rr
+        const int c3 = c & 3;
+        const double dsel = c3 == 0 ? st.e[0] : c3 == 1 ? st.e[1] : c3 == 2 ? st.e[2] : st.e[3];
+        double pc = bperm_f64(dsel, 4 * (16 * (c >> 2) + c));
+        pc = (pc > 0.0) ? pc : 1.0;
+        double sc, isc;
+        pivot_sqrt(pc, sc, isc);
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            Am[o + 4 * g + rr][o + c] = lcol[rr];
-            Dv[p][4 * g + rr][c] = x[rr];
+            const int i = 4 * g + rr;
+            double pr = bperm_f64(dsel, 4 * (16 * g + i));
+            pr = (pr > 0.0) ? pr : 1.0;
+            double sr, isr;
+            pivot_sqrt(pr, sr, isr);
+            // L = L~ D^1/2: l_ic = a_ic / sqrt(a_cc); L^-1 = D^-1/2 X~
+            Am[o + i][o + c] = (i > c) ? st.e[rr] * isc : ((i == c) ? sc : 0.0);
+            Dv[p][i][c] = st.x[rr] * isr;
         }
-        if (badl) bad = 1;
+        if (st.bad) bad = 1;
     };
     // (d) trailing tile number q (row by row over p < j <= i) of sub-panel p
     auto dtile = [&](int p, int q) {
@@ -413,46 +471,34 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
         __syncthreads();
     }
     if (t == 0 && bad) atomicOr(status, 1);
-    // wave w writes rows w, w + 4, ...: lane l the column pair m = 2l, 2l + 1.
-    // F gets the lower triangle of the block (the part above the diagonal
-    // keeps A for the residual); X = L^-1 is D_p on diagonal 16x16 blocks,
-    // the transposed upper blocks of Am below them and zero above.
+    // wave w writes rows w, w + 4, ...: lane l the column pair m = 2l, 2l + 1,
+    // only what later kernels read: X = L^-1 (D_p on diagonal 16x16 blocks,
+    // the transposed upper blocks of Am below them; the zero blocks above
+    // are never written, the buffer starts zeroed) for the panel TRSM and
+    // the backward substitution, and the block of L below the tile's first
+    // 64 x 64 diagonal block (L_{r+1,r} of the 64-row backward substitution;
+    // the diagonal blocks of L are used only through X, and F's upper
+    // triangle keeps A for the residual).
     {
         double* X = Linv128 + k0 * PB;
-        const int m = 2 * lane, bm = m >> 4, h = m >> 6;
+        const int m = 2 * lane, bm = m >> 4;
 #pragma unroll 4
         for (int q = 0; q < 32; ++q) {
             const int i = w + 4 * q, bi = i >> 4;
-            double* dst = F + (k0 + i) * Vp + k0 + m;
-            if (m + 1 <= i) *reinterpret_cast<double2*>(dst) = make_double2(Am[i][m], Am[i][m + 1]);
-            else if (m == i) dst[0] = Am[i][m];
-            double2 xv;
-            if (bm > bi) xv = make_double2(0.0, 0.0);
-            else if (bm == bi) xv = make_double2(Dv[bi][i & 15][m & 15], Dv[bi][i & 15][(m & 15) + 1]);
-            else xv = make_double2(Am[16 * bm + (m & 15)][16 * bi + (i & 15)],
-                                   Am[16 * bm + (m & 15) + 1][16 * bi + (i & 15)]);
-            *reinterpret_cast<double2*>(X + (size_t)i * PB + m) = xv;
-            if ((i >> 6) == h)
-                *reinterpret_cast<double2*>(
-                    Linv64 + ((size_t)(2 * kb + h) * NB + (i & 63)) * NB + (m & 63)) = xv;
+            if (i >= 64 && m < 64)
+                *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) =
+                    make_double2(Am[i][m], Am[i][m + 1]);
+            if (bm <= bi) {
+                const double2 xv =
+                    (bm == bi) ? make_double2(Dv[bi][i & 15][m & 15], Dv[bi][i & 15][(m & 15) + 1])
+                               : make_double2(Am[16 * bm + (m & 15)][16 * bi + (i & 15)],
+                                              Am[16 * bm + (m & 15) + 1][16 * bi + (i & 15)]);
+                *reinterpret_cast<double2*>(X + (size_t)i * PB + m) = xv;
+            }
         }
     }
 }
 
-// acc += A B^T for an MT x NC block of C (MT, NC = 128 or 64), K = 128:
-// A is MT x K and B is NC x K, both row-major (leading dims lda, ldb).
-// 4 waves in a 2 x 2 grid, each an (MT/2) x (NC/2) sub-tile of (MT/32) x
-// (NC/32) v_mfma_f64_16x16x4_f64 tiles. K is staged through LDS in chunks of
-// KC, k-major (so the 16 rows an MFMA operand reads are contiguous), double
-// buffered with the next chunk's global loads in flight during the MFMAs.
-// MFMA operand layout: A[lane%16][lane/16], B[lane/16][lane%16];
-// result element r of a lane is D[4r + lane/16][lane%16].
-// hook(ch) runs at the top of chunk ch, after the next chunk's global loads
-// are issued. An update C -= A B^T loads its C values into registers there at
-// chunk 0 (in flight during the whole product) and stores C - acc after the
-// loop: a read-modify-write after the loop would expose the load latency,
-// and accumulating onto a preloaded -C would round every partial sum at the
-// magnitude of C instead of the panel's contribution.
 // aload(x, row, col) maps each A operand value (tile-local row and column)
 // on its way into LDS.
 struct NoHook {
@@ -980,7 +1026,7 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
 // q), which is exactly the A-operand layout of -P_r^T for row block mb and
 // k-block nb, so it stays in registers.
 __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, int Vp,
-                                                  const double* __restrict__ Linv, double* B,
+                                                  const double* __restrict__ Linv128, double* B,
                                                   int ldb, int nrhs, int nblk,
                                                   int32_t* __restrict__ flags, int32_t epoch,
                                                   int32_t* __restrict__ status) {
@@ -991,7 +1037,9 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     const int r = nblk - 1 - (int)blockIdx.x;
     const size_t rb = (size_t)r * NB;
     const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
-    const double* Lrr = Linv + rb * NB;       // L_rr^-1, row-major 64 x 64
+    // L_rr^-1: diagonal 64 x 64 block (r & 1) of the 128 x 128 inverse
+    // (row-major, leading dimension PB) of panel r / 2
+    const double* Lrr = Linv128 + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
     // factor tile rows tr, 8 consecutive columns from tc
     const int tr = t >> 3, tc = (t & 7) * 8;
     double v[8];
@@ -1020,7 +1068,7 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     // this wave's rows of L_rr^-T as A operands: A[m = wr + lr][k = 4kk + lk]
     double li[16];
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(4 * kk + lk) * NB + wr + lr];
+    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(4 * kk + lk) * PB + wr + lr];
     // -P_r^T rows wr.. as A operands for k-blocks nb: pt[nb][q]
     double pt[4][4];
     const int ndep = nblk - 1 - r;
@@ -1036,7 +1084,7 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk)
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                    Lt[16 * nb + lr][4 * kk + lk], Lrr[(size_t)(4 * kk + lk) * NB + wr + lr], acc,
+                    Lt[16 * nb + lr][4 * kk + lk], Lrr[(size_t)(4 * kk + lk) * PB + wr + lr], acc,
                     0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) pt[nb][q] = -acc[q];
@@ -1176,7 +1224,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     };
     for (int kb = 0; kb < np; ++kb) {
         hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
-                           ctx->d_Linv128, ctx->d_Linv, mu, ctx->V, ctx->d_status);
+                           ctx->d_Linv128, mu, ctx->V, ctx->d_status);
         RC_CHECK_LAUNCH();
         const int nt = np - kb - 1;
         if (nt == 0 && nx == 0) break;
@@ -1318,7 +1366,7 @@ rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
         const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
         const int32_t eb = ++ctx->epoch;
         hipLaunchKernelGGL(k_trsm_bwd, dim3(ctx->nblk), dim3(512), 0, ctx->stream,
-                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv,
+                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
                            d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
                            ctx->d_status);
         RC_CHECK_LAUNCH();

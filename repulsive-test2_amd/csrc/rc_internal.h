@@ -78,7 +78,6 @@ struct rc_ctx {
     double* d_Adiag = nullptr;    // diagonal of A (Vp), saved before the factorisation
     double* d_symv = nullptr;     // residual product partials [nblk][nblk][3][NB]
     float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
-    double* d_Linv = nullptr;     // inverted diagonal blocks [nblk][NB][NB]
     double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]
     float* d_Linv32 = nullptr;
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
