@@ -328,31 +328,37 @@ __device__ __forceinline__ void pivot16(Piv16& st, int g, int c) {
     st.rp = r;
 }
 
-__global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp, int kb,
+__global__ __launch_bounds__(512) void k_potrf128(double* __restrict__ F, int Vp, int kb,
                                                   double* __restrict__ Linv128,
                                                   const double* __restrict__ mu, int V,
                                                   int32_t* __restrict__ status) {
     __shared__ double Am[PB][LDA];
     __shared__ double Dv[PB / SB][SB][SB + 1];
     __shared__ int bad;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // the wave index as a uniform (scalar) value, so that the per-wave task
+    // control flow below compiles to scalar branches
+    const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int lr = lane & 15, lk = lane >> 4;
     const size_t k0 = (size_t)kb * PB;
     if (t == 0) bad = 0;
     {
-        // wave w loads rows w, w + 4, ...: lane l the column pair 2l, 2l + 1
-        // (one 1 KiB row per wave instruction), 32 loads in flight
+        // wave w loads rows w, w + 8, ...: lane l the column pair 2l, 2l + 1
+        // (one 1 KiB row per wave instruction), 16 loads in flight
         // (the first panel reads A itself: A_hat = A + mu on the V x V part)
         const int m = 2 * lane;
         const double mu0 = (kb == 0) ? mu[0] : 0.0;
         const bool sh0 = kb == 0 && m < V, sh1 = kb == 0 && m + 1 < V;
-        double2 v[32];
+        double2 v[16];
 #pragma unroll
-        for (int q = 0; q < 32; ++q)
-            v[q] = *reinterpret_cast<const double2*>(F + (k0 + w + 4 * q) * Vp + k0 + m);
+        for (int q = 0; q < 16; ++q) {
+            // the lower triangle only (the upper one is masked below)
+            v[q] = make_double2(0.0, 0.0);
+            if (m <= w + 8 * q)
+                v[q] = *reinterpret_cast<const double2*>(F + (k0 + w + 8 * q) * Vp + k0 + m);
+        }
 #pragma unroll
-        for (int q = 0; q < 32; ++q) {
-            const int i = w + 4 * q;
+        for (int q = 0; q < 16; ++q) {
+            const int i = w + 8 * q;
             const bool si = i < V;
             const double x0 = (sh0 && si) ? v[q].x + mu0 : v[q].x;
             const double x1 = (sh1 && si) ? v[q].y + mu0 : v[q].y;
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
     // (b) the 16 pivots of diagonal block p on one wave (pivot16), then
     // L_pp and its inverse D_p = L_pp^-1 from the pivots (results stay in
     // registers during the pivot loop: selects, no stores, no branches).
-    auto diag16 = [&](int p) {
+    auto diag16 = [&](int p) __attribute__((always_inline)) {
         const int o = p * SB;
         const int g = lk, c = lr;
         Piv16 st;
@@ -403,7 +409,7 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
         if (st.bad) bad = 1;
     };
     // (d) trailing tile number q (row by row over p < j <= i) of sub-panel p
-    auto dtile = [&](int p, int q) {
+    auto dtile = [&](int p, int q) __attribute__((always_inline)) {
         const int o = p * SB;
         int a, b;
         decode_lower(q, a, b);
@@ -416,32 +422,80 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
 #pragma unroll
         for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][16 * j + lr] -= acc[r];
     };
-    // X_iq = -D_i sum_{k=q}^{i-1} L_ik X_kq (block row i of L^-1), stored
-    // transposed in Am's upper block (q, i): Am[16q + b][16i + a] = X_iq[a][b]
-    auto xblock = [&](int i, int q) {
-        d4 sacc = {0.0, 0.0, 0.0, 0.0};
-        for (int k = q; k < i; ++k) {
+    // Worker tasks of phase p (waves 1-7, task k for k = w - 1, w + 6, ...),
+    // each a 16x16 tile C of Am updated with one 16x16x16 product (4 MFMAs):
+    //  A  (k < p, q = k): X_pq = -D_p (S_pq + L_{p,p-1} X_{p-1,q}), block
+    //     row p of X = L^-1, final;
+    //  B  (next (7 - p) p, row i > p, q < p): S_iq += L_{i,p-1} X_{p-1,q};
+    //  D  (the rest): trailing tile 1.. of sub-panel p (dtile).
+    // S_iq = sum_{k=q}^{i-1} L_ik X_kq (X_qq = D_q) accumulates in the
+    // place of X_iq, transposed in Am's upper block (q, i):
+    // Am[16q + b][16i + a] = X_iq[a][b] (zero from the load). The tasks of a
+    // phase are independent; two waves per SIMD hide each other's LDS
+    // latency.
+    auto xtask = [&](int p, int i, int q, bool fin) __attribute__((always_inline)) {
+        const int cb = 16 * (p - 1);
+        double a[4], b[4], c[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const double xb = (k == q) ? Dv[q][4 * kk + lk][lr]
-                                           : Am[16 * q + lr][16 * k + 4 * kk + lk];
-                sacc = __builtin_amdgcn_mfma_f64_16x16x4f64(Am[16 * i + lr][16 * k + 4 * kk + lk],
-                                                            xb, sacc, 0, 0, 0);
-            }
+        for (int kk = 0; kk < 4; ++kk) {
+            a[kk] = Am[16 * i + lr][cb + 4 * kk + lk];
+            b[kk] = (q == p - 1) ? Dv[p - 1][4 * kk + lk][lr] : Am[16 * q + lr][cb + 4 * kk + lk];
         }
-        d4 xacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c[r] = Am[16 * q + lr][16 * i + 4 * r + lk];
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
-            xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Dv[i][lr][4 * kk + lk], sacc[kk], xacc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b[kk], acc, 0, 0, 0);
+        if (fin) {
+            // the sum's register r is the B operand of k-step r
+            d4 xacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Am[16 * q + lr][16 * i + 4 * r + lk] = xacc[r];
+            for (int kk = 0; kk < 4; ++kk)
+                xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Dv[p][lr][4 * kk + lk], c[kk] + acc[kk],
+                                                            xacc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Am[16 * q + lr][16 * i + 4 * r + lk] = xacc[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Am[16 * q + lr][16 * i + 4 * r + lk] = c[r] + acc[r];
+        }
+    };
+    // Stores, issued as soon as their part is final so that they overlap
+    // the later phases: block row r of X = L^-1 (D_r on the diagonal, the
+    // transposed upper blocks of Am left of it; the zero blocks right of it
+    // are never written, the buffer starts zeroed) after phase r, for the
+    // panel TRSM and the backward substitution; the block of L below the
+    // tile's first 64 x 64 diagonal block (L_{r+1,r} of the 64-row backward
+    // substitution) after phase 3. The diagonal blocks of L are used only
+    // through X, and F's upper triangle keeps A for the residual.
+    double* X = Linv128 + k0 * PB;
+    auto store_xrow = [&](int r) __attribute__((always_inline)) {
+        const int per_row = 8 * (r + 1);   // double2 per row
+        for (int q = t; q < 16 * per_row; q += 512) {
+            const int a = q / per_row, m = 2 * (q - a * per_row), bm = m >> 4;
+            const double2 xv =
+                (bm == r) ? make_double2(Dv[r][a][m & 15], Dv[r][a][(m & 15) + 1])
+                          : make_double2(Am[16 * bm + (m & 15)][16 * r + a],
+                                         Am[16 * bm + (m & 15) + 1][16 * r + a]);
+            *reinterpret_cast<double2*>(X + (size_t)(16 * r + a) * PB + m) = xv;
+        }
+    };
+    auto store_l10 = [&]() __attribute__((always_inline)) {
+#pragma unroll 4
+        for (int q = t; q < 64 * 32; q += 512) {
+            const int i = 64 + (q >> 5), m = 2 * (q & 31);
+            *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) = make_double2(Am[i][m], Am[i][m + 1]);
+        }
     };
     if (w == 0) diag16(0);
     __syncthreads();
     for (int p = 0; p < PB / SB; ++p) {
         const int o = p * SB;
+        if (p > 0) store_xrow(p - 1);
+        if (p == 4) store_l10();
         // (c) rows below the diagonal block: L_ip = A_ip D_p^T
-        for (int i = p + 1 + w; i < PB / SB; i += 4) {
+        for (int i = p + 1 + w; i < PB / SB; i += 8) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk)
@@ -451,9 +505,9 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
             for (int r = 0; r < 4; ++r) Am[16 * i + 4 * r + lk][o + lr] = acc[r];
         }
         __syncthreads();
-        // wave 0: the next diagonal tile's update, then its pivots; waves 1-3
-        // meanwhile: the other trailing tiles and block row p of L^-1 (all
-        // disjoint from the next diagonal block)
+        // wave 0: the next diagonal tile's update, then its pivots (VALU);
+        // waves 1-7 meanwhile (MFMA, two waves per SIMD): the tasks above
+        // (all disjoint from the next diagonal block)
         const int nt = PB / SB - 1 - p;
         if (w == 0) {
             if (nt > 0) {
@@ -465,38 +519,22 @@ __global__ __launch_bounds__(256) void k_potrf128(double* __restrict__ F, int Vp
                 diag16(p + 1);
             }
         } else {
-            for (int q = w; q < nt * (nt + 1) / 2; q += 3) dtile(p, q);
-            for (int q = w - 1; q < p; q += 3) xblock(p, q);
+            const int nA = p, nB = nt * p, ndt = nt > 0 ? nt * (nt + 1) / 2 - 1 : 0;
+            for (int k = w - 1; k < nA + nB + ndt; k += 7) {
+                if (k < nA) {
+                    xtask(p, p, k, true);
+                } else if (k < nA + nB) {
+                    const int bi = k - nA, i = p + 1 + bi / p;
+                    xtask(p, i, bi - (i - p - 1) * p, false);
+                } else {
+                    dtile(p, k - nA - nB + 1);
+                }
+            }
         }
         __syncthreads();
     }
     if (t == 0 && bad) atomicOr(status, 1);
-    // wave w writes rows w, w + 4, ...: lane l the column pair m = 2l, 2l + 1,
-    // only what later kernels read: X = L^-1 (D_p on diagonal 16x16 blocks,
-    // the transposed upper blocks of Am below them; the zero blocks above
-    // are never written, the buffer starts zeroed) for the panel TRSM and
-    // the backward substitution, and the block of L below the tile's first
-    // 64 x 64 diagonal block (L_{r+1,r} of the 64-row backward substitution;
-    // the diagonal blocks of L are used only through X, and F's upper
-    // triangle keeps A for the residual).
-    {
-        double* X = Linv128 + k0 * PB;
-        const int m = 2 * lane, bm = m >> 4;
-#pragma unroll 4
-        for (int q = 0; q < 32; ++q) {
-            const int i = w + 4 * q, bi = i >> 4;
-            if (i >= 64 && m < 64)
-                *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) =
-                    make_double2(Am[i][m], Am[i][m + 1]);
-            if (bm <= bi) {
-                const double2 xv =
-                    (bm == bi) ? make_double2(Dv[bi][i & 15][m & 15], Dv[bi][i & 15][(m & 15) + 1])
-                               : make_double2(Am[16 * bm + (m & 15)][16 * bi + (i & 15)],
-                                              Am[16 * bm + (m & 15) + 1][16 * bi + (i & 15)]);
-                *reinterpret_cast<double2*>(X + (size_t)i * PB + m) = xv;
-            }
-        }
-    }
+    store_xrow(PB / SB - 1);
 }
 
 // aload(x, row, col) maps each A operand value (tile-local row and column)
@@ -1223,7 +1261,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
         return ev;
     };
     for (int kb = 0; kb < np; ++kb) {
-        hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(256), 0, ctx->stream, ctx->d_F, Vp, kb,
+        hipLaunchKernelGGL(k_potrf128, dim3(1), dim3(512), 0, ctx->stream, ctx->d_F, Vp, kb,
                            ctx->d_Linv128, mu, ctx->V, ctx->d_status);
         RC_CHECK_LAUNCH();
         const int nt = np - kb - 1;
