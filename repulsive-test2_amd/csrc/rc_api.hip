@@ -113,6 +113,8 @@ extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->dep_ev) (void)hipEventDestroy(e);
+    if (ctx->asm_go) (void)hipEventDestroy(ctx->asm_go);
+    if (ctx->asm_done) (void)hipEventDestroy(ctx->asm_done);
     free_all(ctx);
     delete ctx;
 }
@@ -541,14 +543,21 @@ __global__ void k_blocks_to_aos(const double* __restrict__ in, int V, int S,
 // Gradient-solve phase shared by rc_solve_constrained_gradient and the step:
 // assembly, constraints, factor, solves, Schur system, g~ and residual.
 // dE (stride Vp) must be resident in ctx->d_dE; edge table at gamma0.
+// gram_done (nullable): the Gram assembly was launched on the second stream
+// and records this event when done; otherwise it is launched here.
 static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, int precision,
                              const rc_step_opts* o, hipEvent_t* ev_asm0, hipEvent_t* ev_asm1,
-                             hipEvent_t* ev_fac1) {
+                             hipEvent_t* ev_fac1, hipEvent_t gram_done = nullptr) {
     const int k = ds.k;
     RC_HIP(hipMemcpyAsync(ctx->d_set, &ds, sizeof(DevSet), hipMemcpyHostToDevice, ctx->stream));
-    if (ev_asm0) RC_HIP(hipEventRecord(*ev_asm0, ctx->stream));
-    rc_status s = rc_launch_gram(ctx, p);
-    if (s != RC_OK) return s;
+    rc_status s = RC_OK;
+    if (gram_done) {
+        RC_HIP(hipStreamWaitEvent(ctx->stream, gram_done, 0));
+    } else {
+        if (ev_asm0) RC_HIP(hipEventRecord(*ev_asm0, ctx->stream));
+        s = rc_launch_gram(ctx, p);
+        if (s != RC_OK) return s;
+    }
     if (ev_asm1) RC_HIP(hipEventRecord(*ev_asm1, ctx->stream));
     if (k > 0) {
         s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
@@ -686,6 +695,20 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
 
     s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
     if (s != RC_OK) return s;
+    // The Gram assembly needs only the edge table: it runs on the second
+    // stream beside the energy and the adjoint gradient (all earlier users of
+    // the factor storage are ordered before asm_go on this stream).
+    if (!ctx->asm_go) {
+        RC_HIP(hipEventCreateWithFlags(&ctx->asm_go, hipEventDisableTiming | hipEventDisableSystemFence));
+        RC_HIP(hipEventCreateWithFlags(&ctx->asm_done, hipEventDisableTiming | hipEventDisableSystemFence));
+    }
+    RC_HIP(hipEventRecord(ctx->asm_go, ctx->stream));
+    RC_HIP(hipStreamWaitEvent(ctx->stream2, ctx->asm_go, 0));
+    if (tm) RC_HIP(hipEventRecord(ev[3], ctx->stream2));
+    s = rc_launch_gram(ctx, p, ctx->stream2);
+    if (s != RC_OK) return s;
+    if (tm) RC_HIP(hipEventRecord(ev[9], ctx->stream2));
+    RC_HIP(hipEventRecord(ctx->asm_done, ctx->stream2));
     // E0 = objective at gamma0 unless the caller supplies it (optimizer.ts:117-132)
     if (!o->has_energy_before) {
         s = rc_launch_energy(ctx, p, ctx->d_scal + 2);
@@ -712,8 +735,8 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     hc.max_iter = o->max_iter;
     RC_HIP(hipMemcpyAsync(ctx->d_ctl, &hc, sizeof(LSCtl), hipMemcpyHostToDevice, ctx->stream));
 
-    s = solve_phase(ctx, p, ds, o->precision, o, tm ? &ev[3] : nullptr, tm ? &ev[4] : nullptr,
-                    tm ? &ev[5] : nullptr);
+    s = solve_phase(ctx, p, ds, o->precision, o, nullptr, tm ? &ev[4] : nullptr,
+                    tm ? &ev[5] : nullptr, ctx->asm_done);
     if (s != RC_OK) return s;
     if (tm) RC_HIP(hipEventRecord(ev[6], ctx->stream));
     s = rc_launch_ls_setup(ctx, ctx->d_gt, ctx->d_dE, ctx->d_dir, ctx->d_scal + 2,
@@ -792,13 +815,14 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     stats->refine_iters = is[5];
     if (tm) {
         float ms[9] = {};
-        for (int i = 1; i <= 8; ++i) RC_HIP(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
+        for (int i = 1; i <= 2; ++i) RC_HIP(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
         std::memset(timings, 0, sizeof(*timings));
         timings->energy = ms[1];
         timings->dE = ms[2];
-        // ev[2]->ev[3]: control upload; ev[3]->ev[4] assembly
+        // ev[3]->ev[9]: assembly (second stream, beside energy and dE);
+        // ev[4]: the main stream has the assembled matrix
         float asm_ms = 0, fac_ms = 0, sad_ms = 0, ls_ms = 0, step_ms = 0;
-        RC_HIP(hipEventElapsedTime(&asm_ms, ev[3], ev[4]));
+        RC_HIP(hipEventElapsedTime(&asm_ms, ev[3], ev[9]));
         RC_HIP(hipEventElapsedTime(&fac_ms, ev[4], ev[5]));
         RC_HIP(hipEventElapsedTime(&sad_ms, ev[4], ev[6]));
         RC_HIP(hipEventElapsedTime(&ls_ms, ev[6], ev[7]));

@@ -301,28 +301,30 @@ __global__ void k_gram_band(EdgeSoA ed, const double* __restrict__ rs, int V, in
 
 }  // namespace
 
-rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p) {
+rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p, hipStream_t st) {
     const int grid = ctx->ngram < 8192 ? ctx->ngram : 8192;
     if (p->alpha == 3.0 && p->beta == 6.0) {
-        hipLaunchKernelGGL(k_gram_tiles<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+        hipLaunchKernelGGL(k_gram_tiles<Pow36>, dim3(grid), dim3(256), 0, st, ctx->ed,
                            Pow36{}, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram, ctx->d_rOff,
                            ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot, ctx->d_vLocal,
                            ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
     } else {
         const double s = (p->beta - 1) / p->alpha;
         PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
-        hipLaunchKernelGGL(k_gram_tiles<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_gram_tiles<PowGeneric>, dim3(grid), dim3(256), 0, st,
                            ctx->ed, pw, p->epsilon, ctx->V, ctx->E, ctx->nvt, ctx->ngram,
                            ctx->d_rOff, ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot,
                            ctx->d_vLocal, ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
     }
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 63) / 64), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 63) / 64), dim3(64), 0, st,
                        ctx->d_rsp, ctx->nvt, ctx->E, ctx->d_rs);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_gram_band, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
+    hipLaunchKernelGGL(k_gram_band, dim3((ctx->V + 255) / 256), dim3(256), 0, st, ctx->ed,
                        ctx->d_rs, ctx->V, ctx->E, ctx->d_vOff, ctx->d_vEdge, ctx->d_F,
                        ctx->nblk * RC_NB);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
+
+rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p) { return rc_launch_gram(ctx, p, ctx->stream); }

@@ -39,6 +39,8 @@ struct rc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
     std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies
+    hipEvent_t asm_go = nullptr;    // step: edge table ready (second stream may assemble)
+    hipEvent_t asm_done = nullptr;  // step: Gram assembly done
     std::string err;
 
     // topology (device)
@@ -159,5 +161,6 @@ rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p
 rc_status rc_launch_energy(rc_ctx* ctx, const rc_params* p, double* d_out);
 rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa);
 rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p);
+rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p, hipStream_t st);
 rc_status rc_upload_positions_aos(rc_ctx* ctx, const double* xyz, double* d_dst);
 rc_status rc_download_positions_aos(rc_ctx* ctx, const double* d_src, double* xyz);
