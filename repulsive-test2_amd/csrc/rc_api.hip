@@ -23,8 +23,9 @@ rc_status rc_launch_penalty_energy(rc_ctx* ctx, const DevPen& pen, double* d_out
 rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B, int ncols);
 rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k, double* gt,
                           double* d_dots, double* d_bsum, double* d_sol, double* d_Minv);
-rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
-                             const double* d_sol, double* d_Ax, double* d_Cg, LSCtl* d_ctl);
+rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, const double* dE,
+                             int k, const double* d_sol, double* d_Ax, double* d_Cg,
+                             double* d_out);
 rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
                              const double* e0tpe, const double* e0pen, int has_e0,
                              double e0_given, double tau0, LSCtl* d_ctl);
@@ -115,6 +116,8 @@ extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     for (hipEvent_t e : ctx->dep_ev) (void)hipEventDestroy(e);
     if (ctx->asm_go) (void)hipEventDestroy(ctx->asm_go);
     if (ctx->asm_done) (void)hipEventDestroy(ctx->asm_done);
+    if (ctx->res_go) (void)hipEventDestroy(ctx->res_go);
+    if (ctx->res_done) (void)hipEventDestroy(ctx->res_done);
     free_all(ctx);
     delete ctx;
 }
@@ -545,9 +548,13 @@ __global__ void k_blocks_to_aos(const double* __restrict__ in, int V, int S,
 // dE (stride Vp) must be resident in ctx->d_dE; edge table at gamma0.
 // gram_done (nullable): the Gram assembly was launched on the second stream
 // and records this event when done; otherwise it is launched here.
+// resid_done (nullable): run the residual on the second stream and record
+// this event there when done (the caller waits for it before reading the
+// residual); otherwise it runs on the main stream.
 static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, int precision,
                              const rc_step_opts* o, hipEvent_t* ev_asm0, hipEvent_t* ev_asm1,
-                             hipEvent_t* ev_fac1, hipEvent_t gram_done = nullptr) {
+                             hipEvent_t* ev_fac1, hipEvent_t gram_done = nullptr,
+                             hipEvent_t resid_go = nullptr, hipEvent_t resid_done = nullptr) {
     const int k = ds.k;
     RC_HIP(hipMemcpyAsync(ctx->d_set, &ds, sizeof(DevSet), hipMemcpyHostToDevice, ctx->stream));
     rc_status s = RC_OK;
@@ -599,7 +606,15 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_gt, ctx->d_dots, ctx->d_bsum, ctx->d_ssol,
                         ctx->d_Minv);
     if (s != RC_OK) return s;
-    s = rc_launch_residual(ctx, ctx->d_gt, ctx->d_dE, k, ctx->d_ssol, ctx->d_Ax, ctx->d_Cg, ctx->d_ctl);
+    hipStream_t rst = ctx->stream;
+    if (resid_go && resid_done) {
+        RC_HIP(hipEventRecord(resid_go, ctx->stream));
+        RC_HIP(hipStreamWaitEvent(ctx->stream2, resid_go, 0));
+        rst = ctx->stream2;
+    }
+    s = rc_launch_residual(ctx, rst, ctx->d_gt, ctx->d_dE, k, ctx->d_ssol, ctx->d_Ax, ctx->d_Cg,
+                           ctx->d_scal + RC_SCAL_RESID);
+    if (s == RC_OK && rst == ctx->stream2) RC_HIP(hipEventRecord(resid_done, ctx->stream2));
     return s;
 }
 
@@ -637,8 +652,9 @@ extern "C" rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xy
     std::vector<double> sol(ds.k + 3);
     RC_HIP(hipMemcpyAsync(sol.data(), ctx->d_ssol, sizeof(double) * (ds.k + 3), hipMemcpyDeviceToHost,
                           ctx->stream));
-    LSCtl ctl;
-    RC_HIP(hipMemcpyAsync(&ctl, ctx->d_ctl, sizeof(LSCtl), hipMemcpyDeviceToHost, ctx->stream));
+    double resid = 0.0;
+    RC_HIP(hipMemcpyAsync(&resid, ctx->d_scal + RC_SCAL_RESID, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
     int32_t st[4];
     RC_HIP(hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     RC_HIP(hipStreamSynchronize(ctx->stream));
@@ -648,7 +664,7 @@ extern "C" rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xy
     }
     if (lambda)
         for (int i = 0; i < ds.k; ++i) lambda[i] = sol[i];
-    if (residual) *residual = ctl.residual;
+    if (residual) *residual = resid;
     if (st[0] & (1 | 4)) {
         ctx->err = "saddle system is singular";
         return RC_SINGULAR;
@@ -701,6 +717,8 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     if (!ctx->asm_go) {
         RC_HIP(hipEventCreateWithFlags(&ctx->asm_go, hipEventDisableTiming | hipEventDisableSystemFence));
         RC_HIP(hipEventCreateWithFlags(&ctx->asm_done, hipEventDisableTiming | hipEventDisableSystemFence));
+        RC_HIP(hipEventCreateWithFlags(&ctx->res_go, hipEventDisableTiming | hipEventDisableSystemFence));
+        RC_HIP(hipEventCreateWithFlags(&ctx->res_done, hipEventDisableTiming | hipEventDisableSystemFence));
     }
     RC_HIP(hipEventRecord(ctx->asm_go, ctx->stream));
     RC_HIP(hipStreamWaitEvent(ctx->stream2, ctx->asm_go, 0));
@@ -735,8 +753,10 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     hc.max_iter = o->max_iter;
     RC_HIP(hipMemcpyAsync(ctx->d_ctl, &hc, sizeof(LSCtl), hipMemcpyHostToDevice, ctx->stream));
 
+    // the residual (a diagnostic the line search does not read) runs on the
+    // second stream beside the line search
     s = solve_phase(ctx, p, ds, o->precision, o, nullptr, tm ? &ev[4] : nullptr,
-                    tm ? &ev[5] : nullptr, ctx->asm_done);
+                    tm ? &ev[5] : nullptr, ctx->asm_done, ctx->res_go, ctx->res_done);
     if (s != RC_OK) return s;
     if (tm) RC_HIP(hipEventRecord(ev[6], ctx->stream));
     s = rc_launch_ls_setup(ctx, ctx->d_gt, ctx->d_dE, ctx->d_dir, ctx->d_scal + 2,
@@ -768,6 +788,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         }
     }
     if (tm) RC_HIP(hipEventRecord(ev[7], ctx->stream));
+    RC_HIP(hipStreamWaitEvent(ctx->stream, ctx->res_done, 0));
     s = rc_launch_ls_finalize(ctx, ctx->d_ctl, ctx->d_pos, ctx->d_work, update, ctx->d_stats);
     if (s != RC_OK) return s;
     RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_stats, 16 * sizeof(double),

@@ -14,6 +14,7 @@
 #define RC_VTILE 32         // vertices per ownership tile of the Gram assembly
 #define RC_RMAX 40          // max incident edges of one vertex tile (assembly LDS bound)
 #define RC_NB 64            // Cholesky / triangular-solve block size
+#define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
 
 // Structure-of-arrays per-edge geometry refreshed by k_edge_prep for the
 // current positions. Index = edge id; padded entries (id >= E) are unused.
@@ -41,6 +42,8 @@ struct rc_ctx {
     std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies
     hipEvent_t asm_go = nullptr;    // step: edge table ready (second stream may assemble)
     hipEvent_t asm_done = nullptr;  // step: Gram assembly done
+    hipEvent_t res_go = nullptr;    // step: solution ready (second stream may form the residual)
+    hipEvent_t res_done = nullptr;  // step: residual done
     std::string err;
 
     // topology (device)

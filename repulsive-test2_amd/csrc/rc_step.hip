@@ -482,13 +482,14 @@ __global__ __launch_bounds__(64) void k_symv_reduce(const double* __restrict__ p
 }
 
 // ||K z - rhs|| / max(1, ||rhs||) (linsolve.ts:810-849) with z = [g~; lambda],
-// rhs = [dE; 0]; Ax = A g~ blocks, Cg = C g~.
+// rhs = [dE; 0]; Ax = A g~ blocks, Cg = C g~. Written to *out (its own slot:
+// it may run beside the line search, which owns the control block).
 __global__ __launch_bounds__(CT) void k_residual(const double* __restrict__ Ax,
                                                  const double* __restrict__ C,
                                                  const double* __restrict__ sol,
                                                  const double* __restrict__ dE,
                                                  const double* __restrict__ Cg, int k, int V,
-                                                 int Vp, LSCtl* ctl) {
+                                                 int Vp, double* __restrict__ out) {
     __shared__ double sh[32];
     double r2 = 0.0, b2 = 0.0;
     for (int q = threadIdx.x; q < 3 * V; q += CT) {
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(CT) void k_residual(const double* __restrict__ Ax,
     for (int p = threadIdx.x; p < k; p += CT) r2 += Cg[p] * Cg[p];
     r2 = block_sum(r2, sh);
     b2 = block_sum(b2, sh);
-    if (threadIdx.x == 0) ctl->residual = sqrt(r2) / js_max(1.0, sqrt(b2));
+    if (threadIdx.x == 0) out[0] = sqrt(r2) / js_max(1.0, sqrt(b2));
 }
 
 // ---------------------------------------------------------------------------
@@ -804,7 +805,8 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
 
 __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restrict__ pos,
                                                     const double* __restrict__ cur, int V,
-                                                    int update, double* __restrict__ stats) {
+                                                    int update, const double* __restrict__ resid,
+                                                    double* __restrict__ stats) {
     const int accepted = ctl->accepted;
     if (update && accepted)
         for (int q = threadIdx.x; q < 3 * V; q += CT) pos[q] = cur[q];
@@ -813,7 +815,8 @@ __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restri
         stats[0] = accepted ? ctl->e1 : ctl->e0;
         stats[1] = ctl->e0;
         stats[2] = accepted ? ctl->tau_acc : 0.0;
-        stats[3] = ctl->residual;
+        // NaN for a singular system (set by the line-search setup)
+        stats[3] = (ctl->residual == ctl->residual) ? resid[0] : ctl->residual;
         stats[4] = ctl->gnorm;
         stats[5] = ctl->slope;
         stats[6] = accepted ? ctl->phi_norm : NAN;
@@ -972,24 +975,25 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
     return RC_OK;
 }
 
-rc_status rc_launch_residual(rc_ctx* ctx, const double* gt, const double* dE, int k,
-                             const double* d_sol, double* d_Ax, double* d_Cg, LSCtl* d_ctl) {
+rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, const double* dE,
+                             int k, const double* d_sol, double* d_Ax, double* d_Cg,
+                             double* d_out) {
     const int Vp = ctx->nblk * RC_NB;
     const int nb = ctx->nblk;
-    hipLaunchKernelGGL(k_symv_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_symv_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, st,
                        (const double*)ctx->d_F, ctx->V, Vp, nb, gt, ctx->d_symv);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_symv_reduce, dim3(nb, 3), dim3(RC_NB), 0, ctx->stream,
+    hipLaunchKernelGGL(k_symv_reduce, dim3(nb, 3), dim3(RC_NB), 0, st,
                        (const double*)ctx->d_symv, (const double*)ctx->d_Adiag, gt, ctx->V, Vp, nb,
                        d_Ax);
     RC_CHECK_LAUNCH();
     if (k > 0) {
-        hipLaunchKernelGGL(k_dots, dim3(k), dim3(256), 0, ctx->stream, ctx->d_C, 3 * (size_t)Vp, gt,
+        hipLaunchKernelGGL(k_dots, dim3(k), dim3(256), 0, st, ctx->d_C, 3 * (size_t)Vp, gt,
                            (size_t)0, 1, 3 * (size_t)Vp, d_Cg);
         RC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_residual, dim3(1), dim3(CT), 0, ctx->stream, d_Ax, ctx->d_C, d_sol, dE,
-                       d_Cg, k, ctx->V, Vp, d_ctl);
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(CT), 0, st, d_Ax, ctx->d_C, d_sol, dE, d_Cg, k,
+                       ctx->V, Vp, d_out);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
@@ -1031,7 +1035,7 @@ rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, con
 rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
                                 int update, double* d_stats) {
     hipLaunchKernelGGL(k_ls_finalize, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, pos, cur, ctx->V,
-                       update, d_stats);
+                       update, (const double*)(ctx->d_scal + RC_SCAL_RESID), d_stats);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
