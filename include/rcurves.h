@@ -62,6 +62,10 @@ typedef struct {
 
 enum { RC_PRECISION_FP64 = 0, RC_PRECISION_MIXED = 1 };
 
+/* Differential of the energy: StepOptions.mode 'analytical' | 'finiteDiff'
+ * (optimizer.ts:21,96). */
+enum { RC_MODE_ANALYTICAL = 0, RC_MODE_FINITE_DIFF = 1 };
+
 /* SobolevStepOptions (optimizer.ts:95-175) + LineSearchOptions
  * (lineSearch.ts:333-371) + ProjectConstraintSetOptions (:120-145). */
 typedef struct {
@@ -76,6 +80,9 @@ typedef struct {
     double refine_tol;          /* mixed form: PCG relative residual target, default 1e-12 */
     int32_t refine_max_iter;    /* mixed form: PCG iteration cap, default 200 */
     int32_t collect_timings;    /* fill rc_timings (hipEvent per phase) */
+    int32_t gradient_mode;      /* dE by RC_MODE_ANALYTICAL (default) | RC_MODE_FINITE_DIFF
+                                 * (SobolevStepOptions.mode, optimizer.ts:96,278-282) */
+    double fd_h;                /* forward-difference step h, default 1e-6 (optimizer.ts:18) */
 } rc_step_opts;
 
 /* SobolevStepStats (optimizer.ts:196-202) + line-search diagnostics. */
@@ -127,6 +134,18 @@ rc_status rc_energy(rc_ctx* ctx, const double* xyz, const rc_params* p, double* 
  * AoS double[V][3] out, ascent orientation. Optionally also the energy. */
 rc_status rc_gradient(rc_ctx* ctx, const double* xyz, const rc_params* p, double* grad,
                       double* energy_out);
+/* gradientFiniteDiff (tangentPointEnergy.ts:114-136): forward difference
+ * (E(x + h e_vd) - E(x)) / h for every vertex v and coordinate d, AoS out.
+ * h must be finite and nonzero. */
+rc_status rc_gradient_fd(rc_ctx* ctx, const double* xyz, const rc_params* p, double h,
+                         double* grad);
+/* step (optimizer.ts:40-68): the raw L2 step. dE at xyz_in by mode
+ * (RC_MODE_ANALYTICAL | RC_MODE_FINITE_DIFF, h used by the latter),
+ * xyz_out = xyz_in - step_size * dE per coordinate (product rounded, then the
+ * difference), *energy = the energy at xyz_out. field (nullable) receives dE
+ * (StepOptions.collectField). xyz_in is not modified. */
+rc_status rc_step(rc_ctx* ctx, const double* xyz_in, const rc_params* p, int32_t mode,
+                  double step_size, double h, double* xyz_out, double* energy, double* field);
 /* assembleAFlat (innerProduct.ts:325-350): V x V row-major A = B + B0. */
 rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A);
 
@@ -142,8 +161,8 @@ rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xyz, const do
                                         double* residual);
 int32_t rc_constraint_rows(const rc_ctx* ctx, const rc_block* blocks, int32_t nblocks);
 
-/* sobolevStepSet (optimizer.ts:231-440), mode 'analytical',
- * projectionMode 'frozen'. Positions in/out are host AoS; gtilde_out
+/* sobolevStepSet (optimizer.ts:231-440), projectionMode 'frozen', dE by
+ * o->gradient_mode. Positions in/out are host AoS; gtilde_out
  * (collectField -> descentField) and timings are nullable. Never fails on a
  * singular saddle: that folds into stats->reason like the reference. */
 rc_status rc_sobolev_step(rc_ctx* ctx, const double* xyz_in, const rc_block* blocks,

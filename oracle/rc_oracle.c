@@ -66,6 +66,8 @@ void rco_default_opts(rco_step_opts* o) {
     o->tau0 = 1.0;
     o->tau_min = 1e-12;
     o->penalties = NULL;
+    o->gradient_mode = RCO_MODE_ANALYTICAL;
+    o->fd_h = 1e-6;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -108,6 +110,45 @@ double rco_energy(int V, int E, const double* v, const int32_t* edges,
         }
     }
     return total / 2;
+}
+
+/* gradientFiniteDiff — tangentPointEnergy.ts:114-136. Forward difference of
+ * the full energy: E0 once, then for each vertex v and coordinate d a copy of
+ * the curve with that coordinate moved by +h (one rounding, x + h), E1 of the
+ * copy, g[v][d] = (E1 - E0) / h. O(V E^2): small curves only. */
+void rco_gradient_fd(int V, int E, const double* v, const int32_t* edges,
+                     double alpha, double beta, double eps, double h, double* g) {
+    const double e0 = rco_energy(V, E, v, edges, alpha, beta, eps);
+    double* pert = (double*)malloc(sizeof(double) * 3 * (size_t)(V > 0 ? V : 1));
+    for (int i = 0; i < V; i++) {
+        for (int d = 0; d < 3; d++) {
+            memcpy(pert, v, sizeof(double) * 3 * (size_t)V);
+            pert[3 * i + d] += h;
+            const double e1 = rco_energy(V, E, pert, edges, alpha, beta, eps);
+            g[3 * i + d] = (e1 - e0) / h;
+        }
+    }
+    free(pert);
+}
+
+/* step — optimizer.ts:40-68. The raw L2 step: dE at the input (analytic or
+ * forward difference), next = v - stepSize * dE per coordinate (product
+ * rounded, then the difference), energy at next. field (nullable) = dE. */
+void rco_step(int V, int E, const double* v, const int32_t* edges, double alpha, double beta,
+              double eps, int mode, double step_size, double h, double* out, double* energy,
+              double* field) {
+    double* g = (double*)malloc(sizeof(double) * 3 * (size_t)(V > 0 ? V : 1));
+    if (mode == RCO_MODE_ANALYTICAL)
+        rco_gradient(V, E, v, edges, alpha, beta, eps, g);
+    else
+        rco_gradient_fd(V, E, v, edges, alpha, beta, eps, h, g);
+    for (int i = 0; i < 3 * V; i++) {
+        const double t = step_size * g[i];
+        out[i] = v[i] - t;
+    }
+    *energy = rco_energy(V, E, out, edges, alpha, beta, eps);
+    if (field) memcpy(field, g, sizeof(double) * 3 * (size_t)V);
+    free(g);
 }
 
 /* Same terms, Neumaier-compensated accumulation of the pair energies. */
@@ -1403,7 +1444,11 @@ int rco_sobolev_step(int V, int E, const double* pos, const int32_t* edges,
     st->projection_iterations = -1;
     const int k = rco_constraint_rows(V, E, nblocks, blocks);
     double* dE = (double*)malloc(sizeof(double) * 3 * (size_t)V);
-    rco_gradient(V, E, pos, edges, o->alpha, o->beta, o->epsilon, dE);
+    /* dE by the requested mode (optimizer.ts:278-282) */
+    if (o->gradient_mode == RCO_MODE_FINITE_DIFF)
+        rco_gradient_fd(V, E, pos, edges, o->alpha, o->beta, o->epsilon, o->fd_h, dE);
+    else
+        rco_gradient(V, E, pos, edges, o->alpha, o->beta, o->epsilon, dE);
     if (rco_penalties_active(o->penalties)) {
         double* pg = (double*)malloc(sizeof(double) * 3 * (size_t)V);
         rco_penalty_gradient(V, E, pos, edges, o->penalties, pg);

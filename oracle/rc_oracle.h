@@ -24,6 +24,9 @@ extern "C" {
  * (src/core/sobolev/constraintSet.ts:39-43). */
 enum { RCO_BARYCENTER = 0, RCO_TOTAL_LENGTH = 1, RCO_EDGE_LENGTHS = 2, RCO_POINT = 3 };
 
+/* Differential modes of step() and sobolevStepSet (optimizer.ts:21,96). */
+enum { RCO_MODE_ANALYTICAL = 0, RCO_MODE_FINITE_DIFF = 1 };
+
 /* Failure reasons (src/core/sobolev/lineSearch.ts:374-379, optimizer.ts:186). */
 enum {
     RCO_REASON_NONE = 0,
@@ -63,6 +66,9 @@ typedef struct {
     double tol_abs, tol_rel;    /* projection tolerances (default 1e-4) */
     double c1, shrink, tau0, tau_min;
     const rco_penalties* penalties; /* nullable */
+    int32_t gradient_mode;      /* RCO_MODE_ANALYTICAL (default) | RCO_MODE_FINITE_DIFF */
+    int32_t _pad2;
+    double fd_h;                /* forward-difference step (default 1e-6, optimizer.ts:18) */
 } rco_step_opts;
 
 typedef struct {
@@ -89,6 +95,14 @@ double rco_energy(int V, int E, const double* pos, const int32_t* edges,
                   double alpha, double beta, double eps);
 void rco_gradient(int V, int E, const double* pos, const int32_t* edges,
                   double alpha, double beta, double eps, double* grad);
+/* Forward-difference gradient (tangentPointEnergy.ts:114-136), AoS out. */
+void rco_gradient_fd(int V, int E, const double* pos, const int32_t* edges,
+                     double alpha, double beta, double eps, double h, double* grad);
+/* Raw L2 step (optimizer.ts:40-68): out = pos - step_size * dE, energy at out;
+ * field (nullable) receives dE. */
+void rco_step(int V, int E, const double* pos, const int32_t* edges, double alpha, double beta,
+              double eps, int mode, double step_size, double h, double* out, double* energy,
+              double* field);
 /* Same energy with a compensated (Neumaier) outer sum: separates the
  * reference's left-to-right rounding from device error at large E. */
 double rco_energy_compensated(int V, int E, const double* pos, const int32_t* edges,
@@ -148,7 +162,7 @@ int rco_line_search(int V, int E, const double* pos, const int32_t* edges,
                     int nblocks, const rco_block* blocks,
                     const rco_step_opts* o, double* pos_out, rco_step_stats* st);
 
-/* One full Sobolev step (optimizer.ts:231-440), mode 'analytical'. */
+/* One full Sobolev step (optimizer.ts:231-440); dE by o->gradient_mode. */
 int rco_sobolev_step(int V, int E, const double* pos, const int32_t* edges,
                      int nblocks, const rco_block* blocks, const rco_step_opts* o,
                      double* pos_out, double* gtilde_out, rco_step_stats* st);

@@ -72,6 +72,9 @@ class StepOpts(C.Structure):
         ("tau0", C.c_double),
         ("tau_min", C.c_double),
         ("penalties", C.POINTER(Penalties)),
+        ("gradient_mode", C.c_int32),
+        ("_pad2", C.c_int32),
+        ("fd_h", C.c_double),
     ]
 
 
@@ -121,6 +124,8 @@ def lib():
         L.rco_energy_compensated.restype = d
         L.rco_energy_compensated.argtypes = [i, i, _dp, _ip, d, d, d]
         L.rco_gradient.argtypes = [i, i, _dp, _ip, d, d, d, _dp]
+        L.rco_gradient_fd.argtypes = [i, i, _dp, _ip, d, d, d, d, _dp]
+        L.rco_step.argtypes = [i, i, _dp, _ip, d, d, d, i, d, d, _dp, _dp, _dp]
         for nm in ("rco_assemble_b_high", "rco_assemble_b_low", "rco_assemble_a"):
             getattr(L, nm).argtypes = [i, i, _dp, _ip, d, d, d, _dp]
         L.rco_barycenter_target.argtypes = [i, i, _dp, _ip, _dp]
@@ -179,6 +184,27 @@ def gradient(v, e, alpha=3.0, beta=6.0, eps=1e-10):
     g = np.zeros_like(v)
     lib().rco_gradient(len(v), len(e), _d(v), _i(e), alpha, beta, eps, _d(g))
     return g
+
+
+def gradient_fd(v, e, alpha=3.0, beta=6.0, eps=1e-10, h=1e-6):
+    v, e = _verts(v), _edges(e)
+    g = np.zeros_like(v)
+    lib().rco_gradient_fd(len(v), len(e), _d(v), _i(e), alpha, beta, eps, h, _d(g))
+    return g
+
+
+MODE_ANALYTICAL, MODE_FINITE_DIFF = 0, 1
+
+
+def step(v, e, mode, step_size, alpha=3.0, beta=6.0, eps=1e-10, h=1e-6):
+    """Raw L2 step: (vertices, energy at them, dE at the input)."""
+    v, e = _verts(v), _edges(e)
+    out = np.zeros_like(v)
+    fld = np.zeros_like(v)
+    en = C.c_double(0)
+    lib().rco_step(len(v), len(e), _d(v), _i(e), alpha, beta, eps, mode, step_size, h, _d(out),
+                   C.byref(en), _d(fld))
+    return out, en.value, fld
 
 
 def assemble(kind, v, e, alpha=3.0, beta=6.0, eps=1e-10):

@@ -78,6 +78,8 @@ extern "C" void rc_default_step_opts(rc_step_opts* o) {
     o->refine_tol = 1e-12;
     o->refine_max_iter = 200;
     o->collect_timings = 0;
+    o->gradient_mode = RC_MODE_ANALYTICAL;
+    o->fd_h = 1e-6;
 }
 
 extern "C" const char* rc_last_error(const rc_ctx* ctx) {
@@ -380,6 +382,42 @@ extern "C" rc_status rc_gradient(rc_ctx* ctx, const double* xyz, const rc_params
     if (s != RC_OK) return s;
     if (energy_out) *energy_out = ctx->d_host_stage[0];
     return RC_OK;
+}
+
+static bool fd_h_ok(double h) { return std::isfinite(h) && h != 0.0; }
+
+extern "C" rc_status rc_gradient_fd(rc_ctx* ctx, const double* xyz, const rc_params* p, double h,
+                                    double* grad) {
+    if (!ctx || !xyz || !grad || !params_ok(p) || !fd_h_ok(h)) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s == RC_OK) s = rc_launch_gradient_fd(ctx, p, h, ctx->d_dE);
+    if (s != RC_OK) return s;
+    return rc_download_positions_aos(ctx, ctx->d_dE, grad);
+}
+
+extern "C" rc_status rc_step(rc_ctx* ctx, const double* xyz_in, const rc_params* p, int32_t mode,
+                             double step_size, double h, double* xyz_out, double* energy,
+                             double* field) {
+    if (!ctx || !xyz_in || !xyz_out || !energy || !params_ok(p)) return RC_BAD_ARG;
+    if (mode != RC_MODE_ANALYTICAL && mode != RC_MODE_FINITE_DIFF) return RC_BAD_ARG;
+    if (mode == RC_MODE_FINITE_DIFF && !fd_h_ok(h)) return RC_BAD_ARG;
+    rc_status s = rc_upload_positions_aos(ctx, xyz_in, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
+    if (s == RC_OK)
+        s = mode == RC_MODE_ANALYTICAL ? rc_launch_gradient(ctx, p, ctx->d_dE)
+                                       : rc_launch_gradient_fd(ctx, p, h, ctx->d_dE);
+    if (s == RC_OK) s = rc_launch_l2_update(ctx, ctx->d_pos, ctx->d_dE, step_size, ctx->d_work);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_work, p);
+    if (s == RC_OK) s = rc_launch_energy(ctx, p, ctx->d_scal);
+    if (s != RC_OK) return s;
+    RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_scal, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    *energy = ctx->d_host_stage[0];
+    s = rc_download_positions_aos(ctx, ctx->d_work, xyz_out);
+    if (s == RC_OK && field) s = rc_download_positions_aos(ctx, ctx->d_dE, field);
+    return s;
 }
 
 extern "C" rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A) {
@@ -697,6 +735,11 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         ctx->err = "line-search options out of range";
         return RC_BAD_ARG;
     }
+    if ((o->gradient_mode != RC_MODE_ANALYTICAL && o->gradient_mode != RC_MODE_FINITE_DIFF) ||
+        (o->gradient_mode == RC_MODE_FINITE_DIFF && !fd_h_ok(o->fd_h))) {
+        ctx->err = "gradient mode or finite-difference step out of range";
+        return RC_BAD_ARG;
+    }
     DevSet ds;
     rc_status s = make_set(ctx, blocks, nblocks, ds);
     if (s != RC_OK) return s;
@@ -734,8 +777,10 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         if (s != RC_OK) return s;
     }
     if (tm) RC_HIP(hipEventRecord(ev[1], ctx->stream));
-    // dE = gradient + penalty gradient (optimizer.ts:278-294), stride Vp
-    s = rc_launch_gradient(ctx, p, ctx->d_grad);
+    // dE = gradient (analytic or forward difference) + penalty gradient
+    // (optimizer.ts:278-294), stride Vp
+    s = o->gradient_mode == RC_MODE_FINITE_DIFF ? rc_launch_gradient_fd(ctx, p, o->fd_h, ctx->d_grad)
+                                                : rc_launch_gradient(ctx, p, ctx->d_grad);
     if (s != RC_OK) return s;
     RC_HIP(hipMemsetAsync(ctx->d_dE, 0, sizeof(double) * 3 * Vp, ctx->stream));
     s = rc_launch_penalty_grad(ctx, dp, ctx->d_grad, ctx->V, ctx->d_dE, Vp);

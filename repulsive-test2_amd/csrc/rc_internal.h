@@ -163,6 +163,9 @@ struct rc_ctx {
 rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p);
 rc_status rc_launch_energy(rc_ctx* ctx, const rc_params* p, double* d_out);
 rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa);
+rc_status rc_launch_gradient_fd(rc_ctx* ctx, const rc_params* p, double h, double* d_grad_soa);
+rc_status rc_launch_l2_update(rc_ctx* ctx, const double* pos, const double* grad,
+                              double step_size, double* out);
 rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p);
 rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p, hipStream_t st);
 rc_status rc_upload_positions_aos(rc_ctx* ctx, const double* xyz, double* d_dst);

@@ -56,6 +56,100 @@ __device__ __forceinline__ double pair_energy(const P& pw, double eps, const dou
     return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
 }
 
+// pair_energy split for the forward-difference gradient: moving one endpoint S
+// of I changes e_I (so both I-ordered numerators), the J-ordered numerator of
+// endpoint S and the two distances from endpoint S; the J-ordered numerator
+// and the distances of the other endpoint are reused from the unmoved pair.
+struct PairParts {
+    double nJ[2];
+    double invB[2][2];
+};
+
+template <class P>
+__device__ __forceinline__ double pair_energy_parts(const P& pw, double eps, const double* PI,
+                                                    const double* PJ, double eJx, double eJy,
+                                                    double eJz, double ellI, double ellJ,
+                                                    double lpI, double lpJ, PairParts& q) {
+    const double eIx = PI[3] - PI[0], eIy = PI[4] - PI[1], eIz = PI[5] - PI[2];
+    double nI[2];
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const double dx = PI[0] - PJ[3 * jb];
+        const double dy = PI[1] - PJ[3 * jb + 1];
+        const double dz = PI[2] - PJ[3 * jb + 2];
+        const double cx = eIy * dz - eIz * dy;
+        const double cy = eIz * dx - eIx * dz;
+        const double cz = eIx * dy - eIy * dx;
+        nI[jb] = pw.c_alpha(rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps);
+    }
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia) {
+        const double dx = PI[3 * ia] - PJ[0];
+        const double dy = PI[3 * ia + 1] - PJ[1];
+        const double dz = PI[3 * ia + 2] - PJ[2];
+        const double cx = eJy * dz - eJz * dy;
+        const double cy = eJz * dx - eJx * dz;
+        const double cz = eJx * dy - eJy * dx;
+        q.nJ[ia] = pw.c_alpha(rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps);
+    }
+#pragma unroll
+    for (int ia = 0; ia < 2; ++ia)
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+            const double dx = PI[3 * ia] - PJ[3 * jb];
+            const double dy = PI[3 * ia + 1] - PJ[3 * jb + 1];
+            const double dz = PI[3 * ia + 2] - PJ[3 * jb + 2];
+            const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+            q.invB[ia][jb] = pw.inv_beta(deps, rcm::rcp(deps));
+        }
+    const double sFI = nI[0] * (q.invB[0][0] + q.invB[1][0]) + nI[1] * (q.invB[0][1] + q.invB[1][1]);
+    const double sFJ = q.nJ[0] * (q.invB[0][0] + q.invB[0][1]) + q.nJ[1] * (q.invB[1][0] + q.invB[1][1]);
+    return 0.25 * (lpI * ellJ * sFI + lpJ * ellI * sFJ);
+}
+
+// pair_energy of I with endpoint S moved (positions Q, length ellQ, lpQ =
+// ellQ^(1-alpha)), reusing the unchanged parts q of the unmoved pair.
+template <int S, class P>
+__device__ __forceinline__ double pair_energy_moved(const P& pw, double eps, const double* Q,
+                                                    const double* PJ, double eJx, double eJy,
+                                                    double eJz, double ellQ, double ellJ,
+                                                    double lpQ, double lpJ, const PairParts& q) {
+    const double eIx = Q[3] - Q[0], eIy = Q[4] - Q[1], eIz = Q[5] - Q[2];
+    double nI[2], nJ[2], invB[2][2];
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const double dx = Q[0] - PJ[3 * jb];
+        const double dy = Q[1] - PJ[3 * jb + 1];
+        const double dz = Q[2] - PJ[3 * jb + 2];
+        const double cx = eIy * dz - eIz * dy;
+        const double cy = eIz * dx - eIx * dz;
+        const double cz = eIx * dy - eIy * dx;
+        nI[jb] = pw.c_alpha(rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps);
+    }
+    {
+        const double dx = Q[3 * S] - PJ[0];
+        const double dy = Q[3 * S + 1] - PJ[1];
+        const double dz = Q[3 * S + 2] - PJ[2];
+        const double cx = eJy * dz - eJz * dy;
+        const double cy = eJz * dx - eJx * dz;
+        const double cz = eJx * dy - eJy * dx;
+        nJ[S] = pw.c_alpha(rcm::sqrt_(cx * cx + cy * cy + cz * cz) + eps);
+        nJ[1 - S] = q.nJ[1 - S];
+    }
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const double dx = Q[3 * S] - PJ[3 * jb];
+        const double dy = Q[3 * S + 1] - PJ[3 * jb + 1];
+        const double dz = Q[3 * S + 2] - PJ[3 * jb + 2];
+        const double deps = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+        invB[S][jb] = pw.inv_beta(deps, rcm::rcp(deps));
+        invB[1 - S][jb] = q.invB[1 - S][jb];
+    }
+    const double sFI = nI[0] * (invB[0][0] + invB[1][0]) + nI[1] * (invB[0][1] + invB[1][1]);
+    const double sFJ = nJ[0] * (invB[0][0] + invB[0][1]) + nJ[1] * (invB[1][0] + invB[1][1]);
+    return 0.25 * (lpQ * ellJ * sFI + lpJ * ellQ * sFJ);
+}
+
 // One wave's J tile in LDS (structure of arrays, RC_TILE edges).
 struct ETile {
     double P[6][RC_TILE];

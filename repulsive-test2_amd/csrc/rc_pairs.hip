@@ -362,10 +362,12 @@ __global__ __launch_bounds__(256) void k_gradient(EdgeSoA ed, int E, int Epad, i
 }
 
 // Per-vertex gather of the per-edge-endpoint partials in fixed order, times
-// the reference's final 1/2 (tangentPointEnergy.ts:385-389). Output SoA.
+// the reference's final 1/2 (tangentPointEnergy.ts:385-389), divided by div
+// (1 for the analytic gradient, exact; h for the forward difference). Output SoA.
 __global__ void k_grad_gather(const double* __restrict__ gpart, int ntiles, int Epad, int V,
                               const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
-                              const int32_t* __restrict__ vSlot, double* __restrict__ g) {
+                              const int32_t* __restrict__ vSlot, double div,
+                              double* __restrict__ g) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     double gx = 0, gy = 0, gz = 0;
@@ -379,9 +381,109 @@ __global__ void k_grad_gather(const double* __restrict__ gpart, int ntiles, int 
             gz += gpart[base + 2 * (size_t)Epad];
         }
     }
-    g[v] = 0.5 * gx;
-    g[V + v] = 0.5 * gy;
-    g[2 * V + v] = 0.5 * gz;
+    g[v] = (0.5 * gx) / div;
+    g[V + v] = (0.5 * gy) / div;
+    g[2 * V + v] = (0.5 * gz) / div;
+}
+
+// ---------------------------------------------------------------------------
+// forward-difference gradient
+//
+// gradientFiniteDiff (tangentPointEnergy.ts:114-136) forms, for each vertex v
+// and coordinate d, (E(gamma + h e_vd) - E(gamma)) / h from two full
+// energies. Moving v changes only the pairs {I, J} in which v is an endpoint
+// of I (a J disjoint from I never contains v, and two edges through v are
+// never disjoint), so
+//   E(gamma + h e_vd) - E(gamma) = 1/2 sum_{I with endpoint v} sum_{J disjoint
+//                                  from I} [pe(I', J) - pe(I, J)]
+// with pe the unordered-pair energy (both ordered terms) and I' the edge with
+// that endpoint coordinate moved to x + h (one rounding, as the reference's
+// `+= h`). The per-pair differences are summed instead of differencing two
+// O(E^2) sums, so the result is the same quotient with less cancellation.
+//
+// A wave owns one ordered tile pair (a, b) of the full E x E square; lane t
+// owns edge I = 64a + t and walks tile b's 64 edges from LDS. For each J it
+// evaluates pe(I, J) once and pe(I', J) for the six moves of I's endpoints
+// (m = 3 * endpoint + coordinate, the gradient's slot layout), and writes
+// the six differences summed over tile b to part[(b * 6 + m) * Epad + I].
+// k_grad_gather then sums the slots of each vertex over all b in fixed order.
+
+template <class P>
+__global__ __launch_bounds__(256) void k_fd_delta(EdgeSoA ed, int E, int Epad, int ntiles, P pw,
+                                                  double eps, double alpha, double h,
+                                                  double* __restrict__ part) {
+    __shared__ rcpair::ETile jt[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    rcpair::ETile& T = jt[wave];
+    const int ntask = ntiles * ntiles;
+    for (int base = blockIdx.x * 4; base < ntask; base += gridDim.x * 4) {
+        const int task = __builtin_amdgcn_readfirstlane(base + wave);
+        if (task >= ntask) break;
+        const int a = task / ntiles, b = task - a * ntiles;
+        {
+            const int J = b * RC_TILE + lane;
+            const int Jc = J < E ? J : E - 1;
+            T.P[0][lane] = ed.ax[Jc]; T.P[1][lane] = ed.ay[Jc]; T.P[2][lane] = ed.az[Jc];
+            T.P[3][lane] = ed.bx[Jc]; T.P[4][lane] = ed.by[Jc]; T.P[5][lane] = ed.bz[Jc];
+            T.ell[lane] = ed.ell[Jc];
+            T.lp[lane] = ed.lpow[Jc];
+            T.i1[lane] = ed.i1[Jc];
+            T.i2[lane] = ed.i2[Jc];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int I = a * RC_TILE + lane;
+        const bool Iok = I < E;
+        const int Ic = Iok ? I : 0;
+        const double PI[6] = {ed.ax[Ic], ed.ay[Ic], ed.az[Ic], ed.bx[Ic], ed.by[Ic], ed.bz[Ic]};
+        const double ellI = ed.ell[Ic], lpI = ed.lpow[Ic];
+        const int i1 = ed.i1[Ic], i2 = ed.i2[Ic];
+        // the six moved copies of edge I: moved coordinate, length, l^(1-alpha)
+        double xm[6], ellM[6], lpM[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            double Q[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Q[c] = PI[c];
+            Q[m] = PI[m] + h;
+            xm[m] = Q[m];
+            const double ex = Q[3] - Q[0], ey = Q[4] - Q[1], ez = Q[5] - Q[2];
+            ellM[m] = __builtin_sqrt(ex * ex + ey * ey + ez * ez) + eps;
+            lpM[m] = pow(ellM[m], 1.0 - alpha);
+        }
+        const int ns = min(RC_TILE, E - b * RC_TILE);
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int s = 0; s < ns; ++s) {
+            const double PJ[6] = {T.P[0][s], T.P[1][s], T.P[2][s], T.P[3][s], T.P[4][s], T.P[5][s]};
+            const double eJx = PJ[3] - PJ[0], eJy = PJ[4] - PJ[1], eJz = PJ[5] - PJ[2];
+            const double ellJ = T.ell[s], lpJ = T.lp[s];
+            const int j1 = T.i1[s], j2 = T.i2[s];
+            const bool valid = Iok && i1 != j1 && i1 != j2 && i2 != j1 && i2 != j2;
+            rcpair::PairParts q;
+            const double e0 = rcpair::pair_energy_parts(pw, eps, PI, PJ, eJx, eJy, eJz, ellI, ellJ,
+                                                        lpI, lpJ, q);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                double Q[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) Q[c] = PI[c];
+                Q[m] = xm[m];
+                const double em =
+                    m < 3 ? rcpair::pair_energy_moved<0>(pw, eps, Q, PJ, eJx, eJy, eJz, ellM[m],
+                                                         ellJ, lpM[m], lpJ, q)
+                          : rcpair::pair_energy_moved<1>(pw, eps, Q, PJ, eJx, eJy, eJz, ellM[m],
+                                                         ellJ, lpM[m], lpJ, q);
+                acc[m] += valid ? (em - e0) : 0.0;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) part[((size_t)b * 6 + m) * Epad + I] = acc[m];
+        // the tile is rewritten by the wave's next task
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 __global__ void k_aos_to_soa(const double* __restrict__ aos, int V, double* __restrict__ soa) {
@@ -398,6 +500,20 @@ __global__ void k_soa_to_aos(const double* __restrict__ soa, int V, double* __re
     aos[3 * v] = soa[v];
     aos[3 * v + 1] = soa[V + v];
     aos[3 * v + 2] = soa[2 * V + v];
+}
+
+// step()'s update (optimizer.ts:56-60): out = pos - step * grad with the
+// product rounded before the difference. The library is built with
+// -ffp-contract=fast, under which the backend fuses a multiply into a
+// following add whatever the source pragmas say; the empty asm makes the
+// rounded product an opaque register value, so no fused multiply-add forms.
+__global__ void k_l2_update(const double* __restrict__ pos, const double* __restrict__ grad,
+                            int n, double step, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = step * grad[i];
+    asm volatile("" : "+v"(t));
+    out[i] = pos[i] - t;
 }
 
 bool is36(const rc_params* p) { return p->alpha == 3.0 && p->beta == 6.0; }
@@ -451,7 +567,38 @@ rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
                        ctx->d_gpart, ctx->ntiles, ctx->Epad, ctx->V, ctx->d_vOff, ctx->d_vEdge,
-                       ctx->d_vSlot, d_grad_soa);
+                       ctx->d_vSlot, 1.0, d_grad_soa);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_gradient_fd(rc_ctx* ctx, const rc_params* p, double h, double* d_grad_soa) {
+    const int ntask = ctx->ntiles * ctx->ntiles;
+    const int grid = pair_grid(ntask);
+    if (is36(p)) {
+        hipLaunchKernelGGL(k_fd_delta<Pow36>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->Epad, ctx->ntiles, Pow36{}, p->epsilon, p->alpha, h,
+                           ctx->d_gpart);
+    } else {
+        const double s = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+        hipLaunchKernelGGL(k_fd_delta<PowGeneric>, dim3(grid), dim3(256), 0, ctx->stream, ctx->ed,
+                           ctx->E, ctx->Epad, ctx->ntiles, pw, p->epsilon, p->alpha, h,
+                           ctx->d_gpart);
+    }
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_gpart, ctx->ntiles, ctx->Epad, ctx->V, ctx->d_vOff, ctx->d_vEdge,
+                       ctx->d_vSlot, h, d_grad_soa);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_l2_update(rc_ctx* ctx, const double* pos, const double* grad,
+                              double step_size, double* out) {
+    const int n = 3 * ctx->V;
+    hipLaunchKernelGGL(k_l2_update, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, pos, grad, n,
+                       step_size, out);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }

@@ -17,6 +17,7 @@ RC_OK, RC_SINGULAR, RC_NONFINITE, RC_BAD_ARG, RC_HIP_ERROR, RC_UNSUPPORTED = ran
 STATUS_NAMES = {0: "ok", 1: "singular", 2: "nonfinite", 3: "bad_arg", 4: "hip_error", 5: "unsupported"}
 BLOCK_BARYCENTER, BLOCK_TOTAL_LENGTH, BLOCK_EDGE_LENGTHS, BLOCK_POINT = range(4)
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+MODE_ANALYTICAL, MODE_FINITE_DIFF = 0, 1
 REASONS = {
     0: None,
     1: "zero_or_nonfinite_gradient_norm",
@@ -80,6 +81,8 @@ class StepOpts(C.Structure):
         ("refine_tol", C.c_double),
         ("refine_max_iter", C.c_int32),
         ("collect_timings", C.c_int32),
+        ("gradient_mode", C.c_int32),
+        ("fd_h", C.c_double),
     ]
 
 
@@ -114,7 +117,7 @@ class KernelProfile(C.Structure):
 
 EXPORTS = (
     "rc_default_step_opts", "rc_ctx_create", "rc_ctx_destroy", "rc_last_error", "rc_energy",
-    "rc_gradient", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
+    "rc_gradient", "rc_gradient_fd", "rc_step", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
     "rc_constraint_rows", "rc_sobolev_step", "rc_set_positions", "rc_get_positions",
     "rc_sobolev_step_resident", "rc_kernel_profile_enable", "rc_kernel_profile_read",
 )
@@ -141,6 +144,9 @@ def lib():
     L.rc_last_error.restype = C.c_char_p
     L.rc_energy.argtypes = [vp, _dp, C.POINTER(Params), _dp]
     L.rc_gradient.argtypes = [vp, _dp, C.POINTER(Params), _dp, _dp]
+    L.rc_gradient_fd.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, _dp]
+    L.rc_step.argtypes = [vp, _dp, C.POINTER(Params), C.c_int32, C.c_double, C.c_double, _dp,
+                          _dp, _dp]
     L.rc_assemble_a.argtypes = [vp, _dp, C.POINTER(Params), _dp]
     L.rc_penalties_eval.argtypes = [vp, _dp, C.POINTER(Penalties), _dp, _dp]
     L.rc_solve_constrained_gradient.argtypes = [vp, _dp, _dp, C.POINTER(Block), C.c_int32,
@@ -157,7 +163,8 @@ def lib():
                                            C.POINTER(Timings)]
     L.rc_kernel_profile_enable.argtypes = [vp, C.c_int32]
     L.rc_kernel_profile_read.argtypes = [vp, C.POINTER(KernelProfile)]
-    for name in ("rc_ctx_create", "rc_energy", "rc_gradient", "rc_assemble_a", "rc_penalties_eval",
+    for name in ("rc_ctx_create", "rc_energy", "rc_gradient", "rc_gradient_fd", "rc_step",
+                 "rc_assemble_a", "rc_penalties_eval",
                  "rc_solve_constrained_gradient", "rc_sobolev_step", "rc_set_positions",
                  "rc_get_positions", "rc_sobolev_step_resident", "rc_kernel_profile_enable",
                  "rc_kernel_profile_read"):

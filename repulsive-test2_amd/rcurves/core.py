@@ -105,6 +105,64 @@ def gradient_analytical(vertices, edges, disjoint_pairs, alpha, beta, epsilon,
     return (g, e.value) if with_energy else g
 
 
+def gradient_finite_diff(vertices, edges, disjoint_pairs, alpha, beta, epsilon, h) -> np.ndarray:
+    """gradientFiniteDiff (tangentPointEnergy.ts:114-136): the forward
+    difference (E(x + h e_vd) - E(x)) / h for every vertex and coordinate."""
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    _check_pairs(disjoint_pairs, ctx.E)
+    g = np.zeros_like(v)
+    ctx.check(L.lib().rc_gradient_fd(ctx.handle, L.dptr(v),
+                                     C.byref(_params(alpha, beta, epsilon)), float(h), L.dptr(g)))
+    return g
+
+
+_MODES = {"analytical": L.MODE_ANALYTICAL, "finiteDiff": L.MODE_FINITE_DIFF}
+
+
+def _mode(mode: str) -> int:
+    if mode not in _MODES:
+        raise L.RcError(L.RC_BAD_ARG, f"unknown gradient mode {mode!r}")
+    return _MODES[mode]
+
+
+@dataclass
+class StepOptions:
+    """StepOptions of the raw L2 step (optimizer.ts:20-36)."""
+
+    mode: str
+    step_size: float
+    alpha: Optional[float] = None
+    beta: Optional[float] = None
+    epsilon: Optional[float] = None
+    h: Optional[float] = None
+    collect_field: bool = False
+
+
+def step(vertices, edges, disjoint_pairs, opts: StepOptions) -> dict:
+    """step (optimizer.ts:40-68): one raw gradient-descent step. Pure: returns
+    new arrays {vertices, energy[, field]}; field (the dE taken at the input)
+    only when opts.collect_field."""
+    mode = _mode(opts.mode)
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    _check_pairs(disjoint_pairs, ctx.E)
+    a = DEFAULTS["alpha"] if opts.alpha is None else opts.alpha
+    b = DEFAULTS["beta"] if opts.beta is None else opts.beta
+    eps = DEFAULTS["epsilon"] if opts.epsilon is None else opts.epsilon
+    h = DEFAULTS["h"] if opts.h is None else opts.h
+    out = np.zeros_like(v)
+    fld = np.zeros_like(v) if opts.collect_field else None
+    e = C.c_double()
+    ctx.check(L.lib().rc_step(ctx.handle, L.dptr(v), C.byref(_params(a, b, eps)), mode,
+                              float(opts.step_size), float(h), L.dptr(out), C.byref(e),
+                              L.dptr(fld) if fld is not None else None))
+    res = {"vertices": out, "energy": e.value}
+    if opts.collect_field:
+        res["field"] = fld
+    return res
+
+
 class TangentPointEnergy:
     """Forward/backward pair: forward() evaluates E, backward() the adjoint
     gradient of the same energy (recomputing the pair terms, no E^2 tape)."""
@@ -263,10 +321,11 @@ class SobolevStepOptions:
     """SobolevStepOptions (optimizer.ts:95-175) plus the line-search and
     projection knobs the reference exposes (lineSearch.ts:120-145, 333-371)."""
 
-    mode: str = "analytical"
+    mode: str = "analytical"      # 'analytical' | 'finiteDiff' (dE of the step)
     alpha: float = 3.0
     beta: float = 6.0
     epsilon: float = 1e-10
+    h: float = 1e-6               # forward-difference step (optimizer.ts:18)
     collect_timings: bool = False
     collect_field: bool = False
     energy_before: Optional[float] = None
@@ -285,8 +344,6 @@ class SobolevStepOptions:
     refine_max_iter: int = 200
 
     def to_c(self) -> L.StepOpts:
-        if self.mode != "analytical":
-            raise L.RcError(L.RC_UNSUPPORTED, "mode 'finiteDiff' is not provided by the device path")
         o = L.StepOpts()
         L.lib().rc_default_step_opts(C.byref(o))
         o.params = L.Params(self.alpha, self.beta, self.epsilon)
@@ -298,6 +355,8 @@ class SobolevStepOptions:
         o.c1, o.shrink, o.tau0, o.tau_min = self.c1, self.shrink, self.tau0, self.tau_min
         o.refine_tol, o.refine_max_iter = float(self.refine_tol), int(self.refine_max_iter)
         o.collect_timings = 1 if self.collect_timings else 0
+        o.gradient_mode = _mode(self.mode)
+        o.fd_h = float(self.h)
         return o
 
 

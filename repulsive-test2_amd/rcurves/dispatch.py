@@ -36,10 +36,46 @@ def build_constraint_set(args: dict) -> list:
     return cs
 
 
+def compute_arrow_field(vertices, edges, mode: str, descent_mode: str, x0) -> Optional[np.ndarray]:
+    """computeArrowField (src/core/arrowField.ts:24-74): dE by mode at the
+    reference defaults; descentMode 'sobolev' maps it to the barycenter-only
+    g~ (None when the saddle system is singular), 'raw' returns dE."""
+    d = core.DEFAULTS
+    if mode == "analytical":
+        dE = core.gradient_analytical(vertices, edges, None, d["alpha"], d["beta"], d["epsilon"])
+    else:
+        dE = core.gradient_finite_diff(vertices, edges, None, d["alpha"], d["beta"], d["epsilon"],
+                                       d["h"])
+    if descent_mode == "sobolev":
+        try:
+            gt, _, _ = core.solve_constrained_gradient_set_frozen(
+                vertices, edges, None, d["alpha"], d["beta"], d["epsilon"], dE,
+                [core.barycenter_block(x0)])
+            return gt
+        except core.L.RcError as err:
+            if err.status in (core.L.RC_SINGULAR, core.L.RC_NONFINITE):
+                return None
+            raise
+    return dE
+
+
 def dispatch_descent_step(args: dict) -> dict:
-    """dispatchDescentStep for descentMode 'sobolev' (the device path)."""
+    """dispatchDescentStep (dispatch.ts:185-290): 'sobolev' -> the constrained
+    Sobolev step, 'raw' -> step() with the frame loop's arguments."""
     if args.get("descentMode", "sobolev") != "sobolev":
-        raise core.L.RcError(core.L.RC_UNSUPPORTED, "raw descent mode is not provided by the device path")
+        r = core.step(args["vertices"], args["edges"], args.get("disjointPairs"),
+                      core.StepOptions(mode=args.get("mode", "analytical"),
+                                       step_size=args["stepSize"],
+                                       collect_field=bool(args.get("collectField", False))))
+        return {
+            "vertices": r["vertices"],
+            "energy": r["energy"],
+            "accepted": True,
+            "converged": False,
+            "stats": None,
+            "timings": None,
+            "descentField": r.get("field"),
+        }
     if ("barycenterConstraint" not in args and "lengthConstraint" not in args
             and "lengthMode" not in args):
         cs = [core.barycenter_block(args["x0"])]
@@ -88,11 +124,10 @@ class SolverWorker:
                 if self.topology is None:
                     raise RuntimeError("solverWorker: field received before topology")
                 a = msg["args"]
-                cs = [core.barycenter_block(a["x0"])]
-                r = core.sobolev_step_set(a["vertices"], self.topology["edges"], None, cs,
-                                          core.SobolevStepOptions(collect_field=True))
-                return {"type": "fieldResult", "graphVersion": msg["graphVersion"],
-                        "field": r.get("descentField")}
+                fld = compute_arrow_field(a["vertices"], self.topology["edges"],
+                                          a.get("mode", "analytical"),
+                                          a.get("descentMode", "sobolev"), a["x0"])
+                return {"type": "fieldResult", "graphVersion": msg["graphVersion"], "field": fld}
             if self.topology is None:
                 raise RuntimeError("solverWorker: step received before topology")
             args: dict[str, Any] = dict(msg["args"])

@@ -1,8 +1,8 @@
 /**
  * bun:ffi drop-in for the reference's src/core energy / gradient / Sobolev-step
  * API, backed by librcurves.so (include/rcurves.h). Same signatures as
- * src/core/tangentPointEnergy.ts (calculateEnergy, gradientAnalytical) and
- * src/core/optimizer.ts (sobolevStepSet); Vec3[] / Edge[] are flattened to
+ * src/core/tangentPointEnergy.ts (calculateEnergy, gradientAnalytical,
+ * gradientFiniteDiff) and src/core/optimizer.ts (step, sobolevStepSet); Vec3[] / Edge[] are flattened to
  * Float64Array / Int32Array at the boundary. One rc_ctx per topology, cached
  * like the worker's topology cache (src/worker/solverWorker.ts:32-43).
  *
@@ -23,6 +23,17 @@ const { symbols: rc } = dlopen(LIB, {
     rc_energy: { args: [FFIType.ptr, FFIType.ptr, FFIType.ptr, FFIType.ptr], returns: FFIType.i32 },
     rc_gradient: {
         args: [FFIType.ptr, FFIType.ptr, FFIType.ptr, FFIType.ptr, FFIType.ptr],
+        returns: FFIType.i32,
+    },
+    rc_gradient_fd: {
+        args: [FFIType.ptr, FFIType.ptr, FFIType.ptr, FFIType.f64, FFIType.ptr],
+        returns: FFIType.i32,
+    },
+    rc_step: {
+        args: [
+            FFIType.ptr, FFIType.ptr, FFIType.ptr, FFIType.i32, FFIType.f64, FFIType.f64,
+            FFIType.ptr, FFIType.ptr, FFIType.ptr,
+        ],
         returns: FFIType.i32,
     },
     rc_default_step_opts: { args: [FFIType.ptr], returns: FFIType.void },
@@ -98,6 +109,43 @@ export function gradientAnalytical(
     return unflat(g);
 }
 
+/** gradientFiniteDiff (tangentPointEnergy.ts:114-136). */
+export function gradientFiniteDiff(
+    vertices: Vec3[], edges: Edge[], _disjointPairs: number[][],
+    alpha: number, beta: number, epsilon: number, h: number,
+): Vec3[] {
+    const c = contextFor(edges, vertices.length);
+    const g = new Float64Array(vertices.length * 3);
+    const st = rc.rc_gradient_fd(c, ptr(flatVerts(vertices)), ptr(params(alpha, beta, epsilon)), h, ptr(g));
+    if (st !== 0) throw new Error(`rc_gradient_fd: ${rc.rc_last_error(c)}`);
+    return unflat(g);
+}
+
+export interface StepOptions {
+    mode: 'analytical' | 'finiteDiff';
+    stepSize: number;
+    alpha?: number; beta?: number; epsilon?: number; h?: number;
+    collectField?: boolean;
+}
+
+/** step (optimizer.ts:40-68): the raw L2 step. Pure: new arrays. */
+export function step(
+    vertices: Vec3[], edges: Edge[], _disjointPairs: number[][], opts: StepOptions,
+): { vertices: Vec3[]; energy: number; field?: Vec3[] } {
+    const c = contextFor(edges, vertices.length);
+    const out = new Float64Array(vertices.length * 3);
+    const energy = new Float64Array(1);
+    const field = opts.collectField ? new Float64Array(vertices.length * 3) : null;
+    const st = rc.rc_step(
+        c, ptr(flatVerts(vertices)), ptr(params(opts.alpha ?? 3, opts.beta ?? 6, opts.epsilon ?? 1e-10)),
+        opts.mode === 'analytical' ? 0 : 1, opts.stepSize, opts.h ?? 1e-6, ptr(out), ptr(energy),
+        field ? ptr(field) : null,
+    );
+    if (st !== 0) throw new Error(`rc_step: ${rc.rc_last_error(c)}`);
+    if (field) return { vertices: unflat(out), energy: energy[0], field: unflat(field) };
+    return { vertices: unflat(out), energy: energy[0] };
+}
+
 /** POD block for the ConstraintSet builders (constraintSet.ts:87-301). */
 export type PodBlock =
     | { kind: 'barycenter'; x0: Vec3 }
@@ -130,7 +178,7 @@ function packBlocks(set: PodBlock[]): { buf: ArrayBuffer; keep: Float64Array[] }
 
 export interface SobolevStepOptions {
     mode: 'analytical' | 'finiteDiff';
-    alpha?: number; beta?: number; epsilon?: number;
+    alpha?: number; beta?: number; epsilon?: number; h?: number;
     energyBefore?: number;
     penalties?: { totalLength?: number; lengthDiff?: number; field?: { weight: number; X: Vec3 } };
     projectionMode?: 'frozen' | 'reassemble';
@@ -144,9 +192,8 @@ export function sobolevStepSet(
     vertices: Vec3[], edges: Edge[], _disjointPairs: number[][], set: PodBlock[],
     opts: SobolevStepOptions,
 ) {
-    if (opts.mode !== 'analytical') throw new Error('finiteDiff dE is not provided by the device path');
     const h = contextFor(edges, vertices.length);
-    const o = new ArrayBuffer(112); // sizeof(rc_step_opts)
+    const o = new ArrayBuffer(128); // sizeof(rc_step_opts)
     rc.rc_default_step_opts(ptr(o));
     const dv = new DataView(o);
     dv.setFloat64(0, opts.alpha ?? 3, true);
@@ -159,6 +206,8 @@ export function sobolevStepSet(
     dv.setInt32(28, opts.precision === 'mixed' ? 1 : 0, true);
     dv.setInt32(40, opts.projectionMode === 'reassemble' ? 0 : 1, true);
     dv.setInt32(108, opts.collectTimings ? 1 : 0, true);
+    dv.setInt32(112, opts.mode === 'finiteDiff' ? 1 : 0, true);
+    dv.setFloat64(120, opts.h ?? 1e-6, true);
     const pen = new Float64Array([
         opts.penalties?.totalLength ?? 0, opts.penalties?.lengthDiff ?? 0,
         opts.penalties?.field?.weight ?? 0, ...(opts.penalties?.field?.X ?? [0, 0, 0]),

@@ -38,6 +38,7 @@ def test_default_options_match_reference_defaults():
     assert (o.c1, o.shrink, o.tau0, o.tau_min) == (1e-4, 0.5, 1.0, 1e-12)
     assert (o.tol_abs, o.tol_rel, o.max_iter) == (1e-4, 1e-4, 8)
     assert o.projection_frozen == 1
+    assert (o.gradient_mode, o.fd_h) == (_lib.MODE_ANALYTICAL, 1e-6)  # optimizer.ts:18
 
 
 def test_struct_sizes_match_header_layout():
@@ -45,3 +46,5 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.Block) == 8 + 56 + 8 + 8
     assert ctypes.sizeof(_lib.StepStats) == 8 * 8 + 6 * 4
     assert ctypes.sizeof(_lib.KernelProfile) == 8 + 8 + 4 + 4
+    # rc_step_opts: params, 2 ints, 1 double, 2 ints, 7 doubles, 2 ints, int + pad, double
+    assert ctypes.sizeof(_lib.StepOpts) == 24 + 8 + 8 + 8 + 56 + 8 + 8 + 8

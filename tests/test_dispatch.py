@@ -72,10 +72,12 @@ def test_worker_topology_message_has_no_reply_and_is_cached():
     np.testing.assert_array_equal(w.topology["edges"], e)
 
 
-def test_raw_descent_mode_is_not_provided():
+def test_raw_step_with_unknown_gradient_mode_is_an_error():
+    # the mode is checked on the host before any device work
     v, e = fixture()
-    resp = dispatch.SolverWorker()
-    resp.handle({"type": "topology", "graphVersion": 1, "edges": e})
-    out = resp.handle({"type": "step", "graphVersion": 1,
-                       "args": {"vertices": v, "x0": [0, 0, 0], "descentMode": "raw"}})
-    assert out["type"] == "error" and "raw descent" in out["message"]
+    w = dispatch.SolverWorker()
+    w.handle({"type": "topology", "graphVersion": 1, "edges": e})
+    out = w.handle({"type": "step", "graphVersion": 1,
+                    "args": {"vertices": v, "x0": [0, 0, 0], "descentMode": "raw",
+                             "mode": "centralDiff", "stepSize": 1e-3}})
+    assert out["type"] == "error" and "unknown gradient mode" in out["message"]

@@ -63,6 +63,39 @@ def test_stage1_analytic_gradient_vs_fd_golden(goldens, name):
 
 
 @pytest.mark.parametrize("name", STAGE1)
+def test_stage1_fd_gradient_vs_fd_golden(goldens, name):
+    # gradientFiniteDiff (tangentPointEnergy.ts:114-136) against the goldens'
+    # forward difference at the same h: they agree to the resolution of a
+    # difference of two full energies, one ulp(E) / h per quantum.
+    fx = goldens["fixtures"][name]
+    gd = goldens["goldens"][name]
+    v, e = fx["vertices"], fx["edges"]
+    a, b, eps = fx["alpha"], fx["beta"], fx["epsilon"]
+    h = gd["finite_difference_h"]
+    g = rco.gradient_fd(v, e, a, b, eps, h)
+    q = float(np.spacing(abs(rco.energy(v, e, a, b, eps)))) / h
+    assert np.abs(g - np.asarray(gd["dE"])).max() <= 8 * q
+    # a backward difference or a missing 1/h fails it by orders of magnitude
+    assert np.abs(-rco.gradient_fd(v, e, a, b, eps, -h) - np.asarray(gd["dE"])).max() > 1e3 * q
+
+
+def test_raw_step_composition_oracle():
+    # optimizer.test.ts:21-42: v - stepSize * grad, energy at the new vertices,
+    # input untouched; field is the dE taken
+    v = np.array([[-1, 0, 0.3], [1, 0, 0.3], [0, -1, -0.3], [0, 1, -0.3]], float)
+    e = np.array([[0, 1], [2, 3]], np.int32)
+    keep = v.copy()
+    out, en, fld = rco.step(v, e, rco.MODE_ANALYTICAL, 0.001)
+    g = rco.gradient(v, e)
+    assert np.array_equal(out, v - 0.001 * g)
+    assert en == rco.energy(v - 0.001 * g, e)
+    assert np.array_equal(fld, g) and np.array_equal(v, keep)
+    out, en, fld = rco.step(v, e, rco.MODE_FINITE_DIFF, 0.001)
+    assert np.array_equal(fld, rco.gradient_fd(v, e))
+    assert np.array_equal(out, v - 0.001 * fld)
+
+
+@pytest.mark.parametrize("name", STAGE1)
 def test_stage1_solve_and_line_search(goldens, name):
     fx = goldens["fixtures"][name]
     gd = goldens["goldens"][name]
