@@ -363,27 +363,45 @@ __global__ __launch_bounds__(256) void k_gradient(EdgeSoA ed, int E, int Epad, i
 
 // Per-vertex gather of the per-edge-endpoint partials in fixed order, times
 // the reference's final 1/2 (tangentPointEnergy.ts:385-389), divided by div
-// (1 for the analytic gradient, exact; h for the forward difference). Output SoA.
-__global__ void k_grad_gather(const double* __restrict__ gpart, int ntiles, int Epad, int V,
-                              const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
-                              const int32_t* __restrict__ vSlot, double div,
-                              double* __restrict__ g) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
+// (1 for the analytic gradient, exact; h for the forward difference). Output
+// SoA. A block owns 64 vertices (lane = vertex, so the slot reads of
+// neighbouring vertices coalesce); its GATHER_WAVES waves split the partner
+// tiles into contiguous ranges and the wave sums are added in wave order, so
+// the result does not depend on scheduling.
+constexpr int GATHER_WAVES = 16;
+
+__global__ __launch_bounds__(64 * GATHER_WAVES) void k_grad_gather(
+    const double* __restrict__ gpart, int ntiles, int Epad, int V,
+    const int32_t* __restrict__ vOff, const int32_t* __restrict__ vEdge,
+    const int32_t* __restrict__ vSlot, double div, double* __restrict__ g) {
+    __shared__ double sh[GATHER_WAVES][3][64];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int v = blockIdx.x * 64 + lane;
+    const int t0 = (w * ntiles) / GATHER_WAVES, t1 = ((w + 1) * ntiles) / GATHER_WAVES;
     double gx = 0, gy = 0, gz = 0;
-    for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
-        const int K = vEdge[q];
-        const int sl = vSlot[q] * 3;
-        for (int t = 0; t < ntiles; ++t) {
-            const size_t base = ((size_t)t * 6 + sl) * Epad + K;
-            gx += gpart[base];
-            gy += gpart[base + Epad];
-            gz += gpart[base + 2 * (size_t)Epad];
+    if (v < V) {
+        for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+            const int K = vEdge[q];
+            const int sl = vSlot[q] * 3;
+            for (int t = t0; t < t1; ++t) {
+                const size_t base = ((size_t)t * 6 + sl) * Epad + K;
+                gx += gpart[base];
+                gy += gpart[base + Epad];
+                gz += gpart[base + 2 * (size_t)Epad];
+            }
         }
     }
-    g[v] = (0.5 * gx) / div;
-    g[V + v] = (0.5 * gy) / div;
-    g[2 * V + v] = (0.5 * gz) / div;
+    sh[w][0][lane] = gx;
+    sh[w][1][lane] = gy;
+    sh[w][2][lane] = gz;
+    __syncthreads();
+    if (w < 3 && v < V) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < GATHER_WAVES; ++k) s += sh[k][w][lane];
+        g[(size_t)w * V + v] = (0.5 * s) / div;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -565,7 +583,7 @@ rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa
                            ctx->d_gpart);
     }
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 63) / 64), dim3(64 * GATHER_WAVES), 0, ctx->stream,
                        ctx->d_gpart, ctx->ntiles, ctx->Epad, ctx->V, ctx->d_vOff, ctx->d_vEdge,
                        ctx->d_vSlot, 1.0, d_grad_soa);
     RC_CHECK_LAUNCH();
@@ -587,7 +605,7 @@ rc_status rc_launch_gradient_fd(rc_ctx* ctx, const rc_params* p, double h, doubl
                            ctx->d_gpart);
     }
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_grad_gather, dim3((ctx->V + 63) / 64), dim3(64 * GATHER_WAVES), 0, ctx->stream,
                        ctx->d_gpart, ctx->ntiles, ctx->Epad, ctx->V, ctx->d_vOff, ctx->d_vEdge,
                        ctx->d_vSlot, h, d_grad_soa);
     RC_CHECK_LAUNCH();
