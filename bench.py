@@ -64,7 +64,8 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
     from rcurves import core
     arr, nb, keep = core._blocks_array(cs)
     penc = pen.to_c()
-    opts = core.SobolevStepOptions(precision=precision, collect_timings=True).to_c()
+    opts = core.SobolevStepOptions(precision=precision, collect_timings=True,
+                                   projection_mode="frozen").to_c()
     ctx.check(L.lib().rc_set_positions(ctx.handle, L.dptr(np.ascontiguousarray(v0))))
     st = L.StepStats()
     tm = L.Timings()
@@ -113,9 +114,9 @@ def accuracy(v, e, cs, pen):
     """Mixed vs FP64 at the same curve: step energy and g~ (direction)."""
     from rcurves import core
     r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
-        penalties=pen, collect_field=True))
+        penalties=pen, collect_field=True, projection_mode="frozen"))
     rmx = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
-        penalties=pen, collect_field=True, precision="mixed"))
+        penalties=pen, collect_field=True, precision="mixed", projection_mode="frozen"))
     g64, gmx = r64["descentField"], rmx["descentField"]
     if g64 is None or gmx is None:
         return {"accuracy_error": "singular saddle: " + str(r64["stats"]) + " / " + str(rmx["stats"])}

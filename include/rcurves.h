@@ -73,7 +73,8 @@ typedef struct {
     int32_t has_energy_before;
     int32_t precision;          /* RC_PRECISION_FP64 (plain) | RC_PRECISION_MIXED */
     double energy_before;
-    int32_t projection_frozen;  /* must be 1: 'frozen' (store.ts:359 app default) */
+    int32_t projection_frozen;  /* 1: 'frozen' (store.ts:359 app setting, default here);
+                                 * 0: 'reassemble' (lineSearch.ts:263-276, FP64 form only) */
     int32_t max_iter;           /* projection maxIter, default 8 */
     double tol_abs, tol_rel;    /* projection tolerances, default 1e-4 */
     double c1, shrink, tau0, tau_min; /* Armijo: 1e-4, 0.5, 1, 1e-12 */

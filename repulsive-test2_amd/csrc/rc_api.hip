@@ -35,6 +35,11 @@ rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, con
                              double* d_epart_ls);
 rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
                                 int update, double* d_stats);
+rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const DevSet* d_set,
+                                  const DevPen& pen, const rc_params* p, const double* pos0,
+                                  const double* dir, double* cur, const double* Y,
+                                  const double* Minv);
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
 
 namespace {
 
@@ -293,6 +298,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_epart_ls, 1024);
     ALLOC(ctx->d_refine, 8);
     ALLOC(ctx->d_stage, 3 * (size_t)V);
+    ALLOC(ctx->d_phi, RC_KMAX + 8);
 #undef ALLOC
     he = hipHostMalloc((void**)&ctx->d_host_stage, 4096, hipHostMallocDefault);
     if (he != hipSuccess) {
@@ -720,6 +726,63 @@ static int trial_count(double tau0, double shrink, double tau_min) {
     return n;
 }
 
+// Line search with projectionMode 'reassemble' (lineSearch.ts:263-276): every
+// projection correction rebuilds A and C at the current iterate and
+// refactors the saddle there (the FP64 factorisation with the right-hand-side
+// rows), then a single-workgroup launch applies it and checks the iterate.
+// The host reads one flag per correction; the trials otherwise follow the
+// frozen form (same candidate, energy and Armijo kernels). The factor storage
+// is reused, so the residual of the gradient solve (second stream) must be
+// done first.
+static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const DevSet& ds,
+                                        const DevPen& dp, int ntr, int max_iter) {
+    const int Vp = ctx->nblk * RC_NB;
+    const int k = ds.k;
+    const int nrhs = 3 + 3 * k;
+    const int nx = (nrhs + RC_NB - 1) / RC_NB;
+    double* Frhs = ctx->d_F + (size_t)Vp * Vp;
+    const double* Y = ctx->d_sol + 3 * (size_t)Vp;
+    RC_HIP(hipStreamWaitEvent(ctx->stream, ctx->res_done, 0));
+    int32_t* flag = (int32_t*)ctx->d_host_stage;
+    rc_status s = RC_OK;
+    for (int trial = 0; trial < ntr; ++trial) {
+        s = rc_launch_ls_project_re(ctx, 0, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
+                                    ctx->d_work, Y, ctx->d_Minv);
+        if (s != RC_OK) return s;
+        for (int corr = 0; corr <= max_iter; ++corr) {
+            RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, need_solve),
+                                  sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            RC_HIP(hipStreamSynchronize(ctx->stream));
+            if (!*flag) break;
+            // A(cur), C(cur) and the factored saddle at the iterate
+            s = rc_launch_edge_prep(ctx, ctx->d_work, p);
+            if (s == RC_OK) s = rc_launch_gram(ctx, p);
+            if (s == RC_OK && k > 0) s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
+            if (s == RC_OK) s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
+            if (s == RC_OK) s = rc_factor_fp64(ctx, nx);
+            if (s != RC_OK) return s;
+            RC_HIP(hipMemcpyAsync(ctx->d_sol, Frhs, sizeof(double) * (size_t)nrhs * Vp,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+            s = rc_solve_fp64_backward(ctx, ctx->d_sol, nrhs);
+            // g~ of the iterate is not needed: scratch output
+            if (s == RC_OK)
+                s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_tmp, ctx->d_dots,
+                                    ctx->d_bsum, ctx->d_ssol, ctx->d_Minv);
+            if (s == RC_OK)
+                s = rc_launch_ls_project_re(ctx, 1, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos,
+                                            ctx->d_dir, ctx->d_work, Y, ctx->d_Minv);
+            if (s != RC_OK) return s;
+        }
+        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
+        if (s != RC_OK) return s;
+        RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        RC_HIP(hipStreamSynchronize(ctx->stream));
+        if (*flag) break;
+    }
+    return RC_OK;
+}
+
 // Device-resident step on ctx->d_pos. update: replace d_pos with the accepted
 // curve. Stats (and g~ / positions when requested) are the single D2H copy.
 static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
@@ -727,8 +790,8 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
                            rc_timings* timings, double* gtilde_out, int update,
                            double* xyz_out) {
     if (!ctx || !o || !stats || !params_ok(&o->params)) return RC_BAD_ARG;
-    if (!o->projection_frozen) {
-        ctx->err = "projectionMode 'reassemble' is not available on the device";
+    if (!o->projection_frozen && o->precision != RC_PRECISION_FP64) {
+        ctx->err = "projectionMode 'reassemble' is provided for the FP64 form only";
         return RC_UNSUPPORTED;
     }
     if (!(o->shrink > 0.0 && o->shrink < 1.0) || !(o->tau0 > 0.0) || o->max_iter < 0) {
@@ -815,7 +878,11 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
 
     const int ntr = trial_count(o->tau0, o->shrink, o->tau_min);
     const int batches[3] = {4, 12, 1 << 20};
-    int done_trials = 0;
+    int done_trials = o->projection_frozen ? 0 : ntr;
+    if (!o->projection_frozen) {
+        s = line_search_reassemble(ctx, p, ds, dp, ntr, o->max_iter);
+        if (s != RC_OK) return s;
+    }
     for (int bi = 0; bi < 3 && done_trials < ntr; ++bi) {
         const int nb = std::min(batches[bi], ntr - done_trials);
         for (int i = 0; i < nb; ++i) {

@@ -57,5 +57,7 @@ struct LSCtl {
     int32_t max_iter;
     int32_t refine_iters;
     int32_t proj_iters_acc;
+    int32_t need_solve;   // reassemble mode: the trial waits for a solve at cur
+    int32_t proj_it;      // reassemble mode: corrections applied to the trial so far
     int32_t _pad;
 };

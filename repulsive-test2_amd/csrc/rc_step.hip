@@ -625,152 +625,161 @@ __device__ void edge_prep_block(EdgeSoA& ed, const double* pos, int V, int E, do
     }
 }
 
-// One trial's candidate + frozen projection (lineSearch.ts:196-295,501-548).
-// Single workgroup. Y = X columns 3.. (A_hat^{-1} C^T blocks), Minv = M^{-1}.
-__global__ __launch_bounds__(CT) void k_ls_project(
-    LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0, DevPen pen,
-    const double* __restrict__ pos0, const double* __restrict__ dir, double* __restrict__ cur,
-    EdgeSoA ed, int V, int E, int Vp, const double* __restrict__ Y,
-    const double* __restrict__ Minv, const int32_t* __restrict__ vOff,
-    const int32_t* __restrict__ vEdge, double alpha, double eps) {
-    __shared__ double sh[32];
-    __shared__ double phi[RC_KMAX + 8];
-    __shared__ double solv[RC_KMAX + 8];
-    __shared__ double Rpin[RC_MAXPIN];
-    __shared__ int flag_s;
-    if (ctl->done) {
-        if (threadIdx.x == 0) ctl->skip_energy = 1;
-        return;
-    }
+// Constraint check of the projection iterate cur (lineSearch.ts:221-249):
+// evaluates Phi (stacked in set order into phi) and every block's scaled
+// tolerance. Returns, in every thread, whether all blocks are within
+// tolerance; phin receives |Phi|. Workgroup-collective.
+__device__ __forceinline__ bool project_check(const LSCtl* ctl, const DevSet* __restrict__ set,
+                                              const double* __restrict__ ell0,
+                                              const double* __restrict__ cur, const EdgeSoA& ed,
+                                              int V, int E, const int32_t* __restrict__ vOff,
+                                              double* sh, double* phi, double* Rpin, int* flag_s,
+                                              double& phin) {
     const int t = threadIdx.x;
-    const double tau = ctl->tau;
-    const int k = set->k;
+    double L = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pe2 = 0.0;
+    for (int K = t; K < E; K += CT) {
+        const int a = ed.i1[K], b = ed.i2[K];
+        const double ax = cur[a], ay = cur[V + a], az = cur[2 * V + a];
+        const double bx = cur[b], by = cur[V + b], bz = cur[2 * V + b];
+        const double ex = bx - ax, ey = by - ay, ez = bz - az;
+        const double l = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+        L += l;
+        if (set->has_bary) {
+            m0 += l * (0.5 * (ax + bx) - set->x0[0]);
+            m1 += l * (0.5 * (ay + by) - set->x0[1]);
+            m2 += l * (0.5 * (az + bz) - set->x0[2]);
+        }
+        if (set->len_per_edge) {
+            const double ph = ell0[K] - l;
+            phi[set->len_row0 + K] = ph;
+            pe2 += ph * ph;
+        }
+    }
+    L = block_sum(L, sh);
+    m0 = block_sum(m0, sh);
+    m1 = block_sum(m1, sh);
+    m2 = block_sum(m2, sh);
+    pe2 = block_sum(pe2, sh);
+    double Rb = 0.0;
+    if (set->has_bary)
+        for (int v = t; v < V; v += CT) {
+            if (vOff[v + 1] == vOff[v]) continue;  // endpoints of edges only
+            const double dx = cur[v] - set->x0[0], dy = cur[V + v] - set->x0[1],
+                         dz = cur[2 * V + v] - set->x0[2];
+            Rb = js_max(Rb, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+        }
+    Rb = block_max(Rb, sh);
+    for (int pn = 0; pn < set->npin; ++pn) {
+        double R = 0.0;
+        const double* tg = set->pin_target[pn];
+        for (int v = t; v < V; v += CT) {
+            const double dx = cur[v] - tg[0], dy = cur[V + v] - tg[1], dz = cur[2 * V + v] - tg[2];
+            R = js_max(R, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        R = block_max(R, sh);
+        if (t == 0) Rpin[pn] = R;
+    }
+    if (t == 0) {
+        // stack Phi in set order and test every block (lineSearch.ts:221-249)
+        bool conv = true;
+        double tot = 0.0;
+        for (int b = 0; b < set->nblocks; ++b) {
+            const int kind = set->kind[b], r0 = set->row0[b];
+            double bn2 = 0.0, scale = 1.0;
+            if (kind == RC_BLOCK_BARYCENTER) {
+                phi[r0] = m0; phi[r0 + 1] = m1; phi[r0 + 2] = m2;
+                bn2 = m0 * m0 + m1 * m1 + m2 * m2;
+                scale = js_max(1.0, L * js_max(1.0, Rb));
+            } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
+                phi[r0] = set->L0 - L;
+                bn2 = phi[r0] * phi[r0];
+                scale = js_max(1.0, L);
+            } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
+                bn2 = pe2;
+                scale = js_max(1.0, L);
+            } else {
+                const int pn = set->pin_of_block[b];
+                const int pv = set->pin_vertex[pn];
+                const double* tg = set->pin_target[pn];
+                if (pv >= 0 && pv < V) {
+                    phi[r0] = cur[pv] - tg[0];
+                    phi[r0 + 1] = cur[V + pv] - tg[1];
+                    phi[r0 + 2] = cur[2 * V + pv] - tg[2];
+                } else {
+                    phi[r0] = phi[r0 + 1] = phi[r0 + 2] = NAN;
+                }
+                bn2 = phi[r0] * phi[r0] + phi[r0 + 1] * phi[r0 + 1] + phi[r0 + 2] * phi[r0 + 2];
+                scale = js_max(1.0, Rpin[pn]);
+            }
+            tot += bn2;
+            const double bn = sqrt(bn2);
+            if (conv && bn > js_max(ctl->tol_abs, ctl->tol_rel * scale)) conv = false;
+        }
+        *flag_s = conv ? 1 : 0;
+        sh[31] = sqrt(tot);
+    }
+    __syncthreads();
+    phin = sh[31];
+    const bool conv = *flag_s != 0;
+    __syncthreads();
+    return conv;
+}
+
+// Projection correction (lineSearch.ts:250-288) from the factored saddle:
+// [lambda; t] = M^{-1} [Phi; 0], x = -Y lambda + N t / V, cur += x.
+// Returns, in every thread, whether a component of x is not finite.
+// Workgroup-collective.
+__device__ __forceinline__ bool project_correct(int k, const double* __restrict__ Y,
+                                                const double* __restrict__ Minv, int V, int Vp,
+                                                double* __restrict__ cur, const double* phi,
+                                                double* solv, double* sh) {
+    const int t = threadIdx.x;
     const int n = k + 3;
+    for (int i = t; i < n; i += CT) {
+        double s = 0.0;
+        for (int j = 0; j < k; ++j) s += Minv[i * n + j] * phi[j];
+        solv[i] = s;
+    }
+    __syncthreads();
+    double nf = 0.0;
     for (int q = t; q < 3 * V; q += CT) {
+        const int c = q / V, v = q % V;
+        double x = solv[k + c] / V;
+        for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+        if (!(x - x == 0.0)) nf = 1.0;
+        cur[(size_t)c * V + v] += x;
+    }
+    nf = block_sum(nf, sh);
+    __syncthreads();
+    return nf > 0.0;
+}
+
+// Candidate of the current trial: cur = gamma0 - tau dir (lineSearch.ts:501-503).
+__device__ __forceinline__ void trial_candidate(const double* __restrict__ pos0,
+                                                const double* __restrict__ dir, double tau, int V,
+                                                int Vp, double* __restrict__ cur) {
+    for (int q = threadIdx.x; q < 3 * V; q += CT) {
         const int c = q / V, v = q % V;
         cur[(size_t)c * V + v] = pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v];
     }
     __syncthreads();
-    int ok = 0, it = 0;
-    double phin = INFINITY;
-    const int maxit = ctl->max_iter;
-    for (it = 0; it <= maxit; ++it) {
-        // ---- evaluate Phi and the per-block scales at cur
-        double L = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pe2 = 0.0;
-        for (int K = t; K < E; K += CT) {
-            const int a = ed.i1[K], b = ed.i2[K];
-            const double ax = cur[a], ay = cur[V + a], az = cur[2 * V + a];
-            const double bx = cur[b], by = cur[V + b], bz = cur[2 * V + b];
-            const double ex = bx - ax, ey = by - ay, ez = bz - az;
-            const double l = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
-            L += l;
-            if (set->has_bary) {
-                m0 += l * (0.5 * (ax + bx) - set->x0[0]);
-                m1 += l * (0.5 * (ay + by) - set->x0[1]);
-                m2 += l * (0.5 * (az + bz) - set->x0[2]);
-            }
-            if (set->len_per_edge) {
-                const double ph = ell0[K] - l;
-                phi[set->len_row0 + K] = ph;
-                pe2 += ph * ph;
-            }
-        }
-        L = block_sum(L, sh);
-        m0 = block_sum(m0, sh);
-        m1 = block_sum(m1, sh);
-        m2 = block_sum(m2, sh);
-        pe2 = block_sum(pe2, sh);
-        double Rb = 0.0;
-        if (set->has_bary)
-            for (int v = t; v < V; v += CT) {
-                if (vOff[v + 1] == vOff[v]) continue;  // endpoints of edges only
-                const double dx = cur[v] - set->x0[0], dy = cur[V + v] - set->x0[1],
-                             dz = cur[2 * V + v] - set->x0[2];
-                Rb = js_max(Rb, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
-            }
-        Rb = block_max(Rb, sh);
-        for (int pn = 0; pn < set->npin; ++pn) {
-            double R = 0.0;
-            const double* tg = set->pin_target[pn];
-            for (int v = t; v < V; v += CT) {
-                const double dx = cur[v] - tg[0], dy = cur[V + v] - tg[1], dz = cur[2 * V + v] - tg[2];
-                R = js_max(R, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
-            }
-            R = block_max(R, sh);
-            if (t == 0) Rpin[pn] = R;
-        }
-        if (t == 0) {
-            // stack Phi in set order and test every block (lineSearch.ts:221-249)
-            bool conv = true;
-            double tot = 0.0;
-            for (int b = 0; b < set->nblocks; ++b) {
-                const int kind = set->kind[b], r0 = set->row0[b];
-                double bn2 = 0.0, scale = 1.0;
-                if (kind == RC_BLOCK_BARYCENTER) {
-                    phi[r0] = m0; phi[r0 + 1] = m1; phi[r0 + 2] = m2;
-                    bn2 = m0 * m0 + m1 * m1 + m2 * m2;
-                    scale = js_max(1.0, L * js_max(1.0, Rb));
-                } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
-                    phi[r0] = set->L0 - L;
-                    bn2 = phi[r0] * phi[r0];
-                    scale = js_max(1.0, L);
-                } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
-                    bn2 = pe2;
-                    scale = js_max(1.0, L);
-                } else {
-                    const int pn = set->pin_of_block[b];
-                    const int pv = set->pin_vertex[pn];
-                    const double* tg = set->pin_target[pn];
-                    if (pv >= 0 && pv < V) {
-                        phi[r0] = cur[pv] - tg[0];
-                        phi[r0 + 1] = cur[V + pv] - tg[1];
-                        phi[r0 + 2] = cur[2 * V + pv] - tg[2];
-                    } else {
-                        phi[r0] = phi[r0 + 1] = phi[r0 + 2] = NAN;
-                    }
-                    bn2 = phi[r0] * phi[r0] + phi[r0 + 1] * phi[r0 + 1] + phi[r0 + 2] * phi[r0 + 2];
-                    scale = js_max(1.0, Rpin[pn]);
-                }
-                tot += bn2;
-                const double bn = sqrt(bn2);
-                if (conv && bn > js_max(ctl->tol_abs, ctl->tol_rel * scale)) conv = false;
-            }
-            phin = sqrt(tot);
-            flag_s = conv ? 1 : 0;
-            sh[31] = phin;
-        }
-        __syncthreads();
-        phin = sh[31];
-        if (flag_s) {
-            ok = 1;
-            break;
-        }
-        if (it == maxit) break;
-        // ---- frozen correction: [lambda; t] = M^{-1} [Phi; 0], x = -Y lambda + N t / V
-        for (int i = t; i < n; i += CT) {
-            double s = 0.0;
-            for (int j = 0; j < k; ++j) s += Minv[i * n + j] * phi[j];
-            solv[i] = s;
-        }
-        __syncthreads();
-        double nf = 0.0;
-        for (int q = t; q < 3 * V; q += CT) {
-            const int c = q / V, v = q % V;
-            double x = solv[k + c] / V;
-            for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-            if (!(x - x == 0.0)) nf = 1.0;
-            cur[(size_t)c * V + v] += x;
-        }
-        nf = block_sum(nf, sh);
-        if (nf > 0.0) break;  // non-finite step -> ok = false (lineSearch.ts:283-288)
-        __syncthreads();
-    }
-    // finite check of the candidate (lineSearch.ts:525-527)
+}
+
+// End of a trial's projection: on success (and a finite candidate,
+// lineSearch.ts:525-527) the edge table and the penalty energy at cur feed
+// the energy kernel; otherwise the trial fails with 'projection_failed' and
+// tau shrinks. Workgroup-collective.
+__device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, double phin,
+                                               const double* __restrict__ cur, EdgeSoA& ed, int V,
+                                               int E, const DevPen& pen,
+                                               const int32_t* __restrict__ vOff,
+                                               const int32_t* __restrict__ vEdge, double alpha,
+                                               double eps, double* sh) {
+    const int t = threadIdx.x;
     double nfc = 0.0;
     for (int q = t; q < 3 * V; q += CT)
         if (!(cur[q] - cur[q] == 0.0)) nfc = 1.0;
     nfc = block_sum(nfc, sh);
-    if (it > maxit) it = maxit;
     if (ok && nfc == 0.0) {
         edge_prep_block(ed, cur, V, E, alpha, eps);
         __syncthreads();
@@ -788,11 +797,110 @@ __global__ __launch_bounds__(CT) void k_ls_project(
         ctl->skip_energy = 1;
         ctl->trials += 1;
         ctl->last_reason = RC_REASON_PROJECTION_FAILED;
-        ctl->tau = tau * ctl->shrink;
+        ctl->tau = ctl->tau * ctl->shrink;
         if (!(ctl->tau >= ctl->tau_min)) {
             ctl->done = 1;
             ctl->reason = ctl->last_reason;
         }
+    }
+}
+
+// One trial's candidate + frozen projection (lineSearch.ts:196-295,501-548).
+// Single workgroup. Y = X columns 3.. (A_hat^{-1} C^T blocks), Minv = M^{-1}.
+__global__ __launch_bounds__(CT) void k_ls_project(
+    LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0, DevPen pen,
+    const double* __restrict__ pos0, const double* __restrict__ dir, double* __restrict__ cur,
+    EdgeSoA ed, int V, int E, int Vp, const double* __restrict__ Y,
+    const double* __restrict__ Minv, const int32_t* __restrict__ vOff,
+    const int32_t* __restrict__ vEdge, double alpha, double eps) {
+    __shared__ double sh[32];
+    __shared__ double phi[RC_KMAX + 8];
+    __shared__ double solv[RC_KMAX + 8];
+    __shared__ double Rpin[RC_MAXPIN];
+    __shared__ int flag_s;
+    if (ctl->done) {
+        if (threadIdx.x == 0) ctl->skip_energy = 1;
+        return;
+    }
+    trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
+    int ok = 0, it = 0;
+    double phin = INFINITY;
+    const int maxit = ctl->max_iter;
+    for (it = 0; it <= maxit; ++it) {
+        if (project_check(ctl, set, ell0, cur, ed, V, E, vOff, sh, phi, Rpin, &flag_s, phin)) {
+            ok = 1;
+            break;
+        }
+        if (it == maxit) break;
+        // frozen correction with Y and M^{-1} of gamma0
+        if (project_correct(set->k, Y, Minv, V, Vp, cur, phi, solv, sh))
+            break;  // non-finite step -> ok = false (lineSearch.ts:283-288)
+    }
+    if (it > maxit) it = maxit;
+    project_finish(ctl, ok, it, phin, cur, ed, V, E, pen, vOff, vEdge, alpha, eps, sh);
+}
+
+// Reassemble-mode projection (lineSearch.ts:263-276): every correction uses
+// A and C rebuilt and refactored at the current iterate, which the host does
+// between launches. phase 0 starts the trial (candidate, first check);
+// phase 1 applies the correction with the Y and M^{-1} just factored at cur
+// (a singular system there fails the trial, as the reference's solve throws)
+// and checks again. Not converged with corrections left: need_solve = 1 and
+// Phi is kept in phi_g for the next launch. Single workgroup.
+__global__ __launch_bounds__(CT) void k_ls_project_re(
+    int phase, LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
+    DevPen pen, const double* __restrict__ pos0, const double* __restrict__ dir,
+    double* __restrict__ cur, EdgeSoA ed, int V, int E, int Vp, const double* __restrict__ Y,
+    const double* __restrict__ Minv, const int32_t* __restrict__ vOff,
+    const int32_t* __restrict__ vEdge, double alpha, double eps, double* __restrict__ phi_g,
+    const int32_t* __restrict__ status) {
+    __shared__ double sh[32];
+    __shared__ double phi[RC_KMAX + 8];
+    __shared__ double solv[RC_KMAX + 8];
+    __shared__ double Rpin[RC_MAXPIN];
+    __shared__ int flag_s;
+    const int t = threadIdx.x;
+    if (ctl->done) {
+        if (t == 0) {
+            ctl->skip_energy = 1;
+            ctl->need_solve = 0;
+        }
+        return;
+    }
+    const int k = set->k;
+    const int maxit = ctl->max_iter;
+    int it = 0;
+    if (phase == 0) {
+        trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
+    } else {
+        it = ctl->proj_it;
+        bool bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
+        if (!bad) {
+            for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
+            __syncthreads();
+            bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
+        }
+        if (bad) {
+            if (t == 0) ctl->need_solve = 0;
+            project_finish(ctl, 0, it, INFINITY, cur, ed, V, E, pen, vOff, vEdge, alpha, eps, sh);
+            return;
+        }
+        ++it;
+    }
+    double phin = INFINITY;
+    const bool conv = project_check(ctl, set, ell0, cur, ed, V, E, vOff, sh, phi, Rpin, &flag_s,
+                                    phin);
+    if (conv || it == maxit) {
+        if (t == 0) ctl->need_solve = 0;
+        project_finish(ctl, conv ? 1 : 0, it, phin, cur, ed, V, E, pen, vOff, vEdge, alpha, eps,
+                       sh);
+        return;
+    }
+    for (int j = t; j < k; j += CT) phi_g[j] = phi[j];
+    if (t == 0) {
+        ctl->proj_it = it;
+        ctl->need_solve = 1;
+        ctl->skip_energy = 1;
     }
 }
 
@@ -1008,6 +1116,8 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
     return RC_OK;
 }
 
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
+
 rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
                              const rc_params* p, const double* pos0, const double* dir,
                              double* cur, const double* Y, const double* Minv,
@@ -1017,6 +1127,26 @@ rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, con
                        pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv, ctx->d_vOff,
                        ctx->d_vEdge, p->alpha, p->epsilon);
     RC_CHECK_LAUNCH();
+    return rc_launch_ls_energy(ctx, d_ctl, p, d_epart_ls);
+}
+
+rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const DevSet* d_set,
+                                  const DevPen& pen, const rc_params* p, const double* pos0,
+                                  const double* dir, double* cur, const double* Y,
+                                  const double* Minv) {
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_ls_project_re, dim3(1), dim3(CT), 0, ctx->stream, phase, d_ctl, d_set,
+                       ctx->d_ell0, pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
+                       ctx->d_vOff, ctx->d_vEdge, p->alpha, p->epsilon, ctx->d_phi,
+                       (const int32_t*)ctx->d_status);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+// Energy of the projected trial and the Armijo decision (skipped on the
+// device when the trial's projection failed or the search is done).
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p,
+                              double* d_epart_ls) {
     int grid = (ctx->npairs + 3) / 4;
     if (grid > 1024) grid = 1024;
     if (p->alpha == 3.0 && p->beta == 6.0) {

@@ -330,7 +330,7 @@ class SobolevStepOptions:
     collect_field: bool = False
     energy_before: Optional[float] = None
     penalties: Optional[PenaltyConfig] = None
-    projection_mode: str = "frozen"
+    projection_mode: str = "reassemble"  # optimizer.ts:144-160 default; the app sets 'frozen' 
     factor_mode: str = "ldlt"     # accepted; the device always factors A + mu 11^T (Cholesky)
     precision: str = "fp64"       # 'fp64' (plain) | 'mixed'
     tol_abs: float = 1e-4
@@ -350,6 +350,8 @@ class SobolevStepOptions:
         o.has_energy_before = 0 if self.energy_before is None else 1
         o.energy_before = 0.0 if self.energy_before is None else float(self.energy_before)
         o.precision = L.PRECISION_MIXED if self.precision == "mixed" else L.PRECISION_FP64
+        if self.projection_mode not in ("frozen", "reassemble"):
+            raise L.RcError(L.RC_BAD_ARG, f"unknown projection mode {self.projection_mode!r}")
         o.projection_frozen = 1 if self.projection_mode == "frozen" else 0
         o.max_iter, o.tol_abs, o.tol_rel = int(self.max_iter), float(self.tol_abs), float(self.tol_rel)
         o.c1, o.shrink, o.tau0, o.tau_min = self.c1, self.shrink, self.tau0, self.tau_min

@@ -92,7 +92,7 @@ def dispatch_descent_step(args: dict) -> dict:
         collect_field=bool(args.get("collectField", False)),
         energy_before=args.get("energyBefore"),
         penalties=pen,
-        projection_mode=args.get("projectionMode", "frozen"),
+        projection_mode=args.get("projectionMode") or "reassemble",
         precision=args.get("precision", "fp64"),
     )
     r = core.sobolev_step_set(args["vertices"], args["edges"], args.get("disjointPairs"), cs, opts)

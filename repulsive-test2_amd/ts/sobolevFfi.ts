@@ -204,7 +204,7 @@ export function sobolevStepSet(
         dv.setFloat64(32, opts.energyBefore, true);
     }
     dv.setInt32(28, opts.precision === 'mixed' ? 1 : 0, true);
-    dv.setInt32(40, opts.projectionMode === 'reassemble' ? 0 : 1, true);
+    dv.setInt32(40, opts.projectionMode === 'frozen' ? 1 : 0, true); // default 'reassemble' (optimizer.ts:144-160)
     dv.setInt32(108, opts.collectTimings ? 1 : 0, true);
     dv.setInt32(112, opts.mode === 'finiteDiff' ? 1 : 0, true);
     dv.setFloat64(120, opts.h ?? 1e-6, true);

@@ -51,7 +51,7 @@ def run_pair(v, e, pins, pen_w, steps, grow=1.0):
             cs.append(core.point_block(int(p), t))
             ocs.point(int(p), t)
         r = core.sobolev_step_set(vg, e, None, cs, core.SobolevStepOptions(
-            penalties=pen, energy_before=eg))
+            penalties=pen, energy_before=eg, projection_mode="frozen"))
         o = (rco.default_opts(penalties=open_) if eo is None else
              rco.default_opts(penalties=open_, has_energy_before=1, energy_before=eo))
         out, _, st = rco.sobolev_step(vo, e, ocs, o)
@@ -96,7 +96,8 @@ def test_c5_full_size_invariants():
         cs = [core.barycenter_block(x0), core.total_length_block(L0)]
         cs += [core.point_block(int(p), t) for p, t in zip(pins, targets)]
         r = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen,
-                                                                          energy_before=eg))
+                                                                          energy_before=eg,
+                                                                          projection_mode="frozen"))
         st = r["stats"]
         log("C5 step", step, st, r["energy"])
         assert r["accepted"] and np.isfinite(r["energy"])

@@ -57,9 +57,10 @@ def test_mixed_vs_fp64_step(V, maker):
     L0 = core.total_length(v, e)
     cs = [core.barycenter_block(x0), core.total_length_block(L0)]
     pen = core.PenaltyConfig(field_weight=10.0, field_X=(0.5, 1.0, 0.0))
-    r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen, collect_field=True))
+    r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen, collect_field=True,
+                                                                             projection_mode="frozen"))
     rmx = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
-        penalties=pen, collect_field=True, precision="mixed"))
+        penalties=pen, collect_field=True, precision="mixed", projection_mode="frozen"))
     de = abs(rmx["energy"] - r64["energy"]) / abs(r64["energy"])
     dd = rel(rmx["descentField"], r64["descentField"])
     log(maker, V, "energy", de, "direction", dd, "tau", r64["stats"]["tau"], rmx["stats"]["tau"],
@@ -83,9 +84,10 @@ def test_fp64_factor_forward_error_v8192():
     L0 = core.total_length(v, e)
     cs = [core.barycenter_block(x0), core.total_length_block(L0)]
     pen = core.PenaltyConfig(field_weight=10.0, field_X=(0.5, 1.0, 0.0))
-    r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen, collect_field=True))
+    r64 = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(penalties=pen, collect_field=True,
+                                                                             projection_mode="frozen"))
     rmx = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
-        penalties=pen, collect_field=True, precision="mixed"))
+        penalties=pen, collect_field=True, precision="mixed", projection_mode="frozen"))
     de = abs(rmx["energy"] - r64["energy"]) / abs(rmx["energy"])
     dd = rel(r64["descentField"], rmx["descentField"])
     log("fp64 vs refined", de, dd, rmx["diagnostics"]["refineResidual"])
@@ -99,7 +101,7 @@ def test_mixed_step_matches_oracle_trefoil():
     x0 = rco.barycenter_target(v, e)
     L0 = rco.total_length(v, e)
     r = core.sobolev_step_set(v, e, None, [core.barycenter_block(x0), core.total_length_block(L0)],
-                              core.SobolevStepOptions(precision="mixed"))
+                              core.SobolevStepOptions(precision="mixed", projection_mode="frozen"))
     out, _, st = rco.sobolev_step(v, e, rco.ConstraintSet().barycenter(x0).total_length(L0),
                                   rco.default_opts())
     assert r["stats"]["tau"] == st["tau"]

@@ -176,7 +176,7 @@ def test_step_vs_oracle_step(goldens, gname, fname):
     gd = goldens["goldens"][gname]
     cs, ocs = build_set(gd)
     pen, open_ = pen_objects(gd)
-    opts = core.SobolevStepOptions(penalties=pen, collect_field=True)
+    opts = core.SobolevStepOptions(penalties=pen, collect_field=True, projection_mode="frozen")
     r = core.sobolev_step_set(fx["vertices"], fx["edges"], None, cs, opts)
     kw = {"penalties": open_} if open_ is not None else {}
     out, gto, st = rco.sobolev_step(fx["vertices"], fx["edges"], ocs, rco.default_opts(**kw))
@@ -200,7 +200,7 @@ def test_trajectory_c1_trefoil_20_steps():
     vg, vo = v.copy(), v.copy()
     eg = eo = None
     for step in range(20):
-        r = core.sobolev_step_set(vg, e, None, cs, core.SobolevStepOptions(energy_before=eg))
+        r = core.sobolev_step_set(vg, e, None, cs, core.SobolevStepOptions(energy_before=eg, projection_mode="frozen"))
         o = rco.default_opts() if eo is None else rco.default_opts(has_energy_before=1, energy_before=eo)
         out, _, st = rco.sobolev_step(vo, e, ocs, o)
         log("step", step, r["stats"]["tau"], st["tau"], r["energy"], st["energy"])
@@ -241,3 +241,97 @@ def test_isolated_vertex_is_singular_system():
     assert r["stats"]["reason"] == "singular_system"
     assert r["stats"]["projectionIterations"] is None
     assert np.array_equal(r["vertices"], v)
+
+
+REASSEMBLE_GOLDENS = [
+    ("crossing-length", "crossing"), ("linked-rings-length", "linked-rings"),
+    ("crossing-point", "crossing"), ("linked-rings-point", "linked-rings"),
+    ("crossing-edgelengths", "crossing"), ("junction-y-edgelengths", "junction-y"),
+    ("crossing-bary-pen-length", "crossing"), ("helix-bary-pen-field", "helix"),
+    ("junction-y-bary-pen-diff", "junction-y"), ("crossing-length-pen-combo", "crossing"),
+]
+
+
+@pytest.mark.parametrize("gname,fname", REASSEMBLE_GOLDENS + [(n, n) for n in STAGE1])
+def test_step_reassemble_vs_oracle_step(goldens, gname, fname):
+    """projectionMode 'reassemble' (lineSearch.ts:263-276, the function default
+    of sobolevStepSet and the mode of these goldens): A and C rebuilt and the
+    saddle refactored at every projection iterate. Device vs the oracle's
+    reassemble step, whose line search the goldens pin."""
+    log("step reassemble", gname)
+    fx = goldens["fixtures"][fname]
+    gd = goldens["goldens"][gname]
+    if "block_kinds" not in gd:
+        gd = dict(gd, block_kinds=["barycenter"])
+    cs, ocs = build_set(gd)
+    pen, open_ = pen_objects(gd)
+    r = core.sobolev_step_set(fx["vertices"], fx["edges"], None, cs,
+                              core.SobolevStepOptions(penalties=pen, collect_field=True))
+    kw = {"penalties": open_} if open_ is not None else {}
+    out, gto, st = rco.sobolev_step(fx["vertices"], fx["edges"], ocs,
+                                    rco.default_opts(projection_frozen=0, **kw))
+    assert r["accepted"] == st["accepted"]
+    assert r["stats"]["tau"] == st["tau"]
+    assert r["stats"]["projectionIterations"] == st["projection_iterations"]
+    assert abs(r["energy"] - st["energy"]) <= 1e-12 * abs(st["energy"])
+    assert rel(r["vertices"], out) <= 1e-9
+    assert rel(r["descentField"], gto) <= 1e-9
+
+
+def test_reassemble_and_frozen_differ_where_the_reference_says():
+    # optimizer.ts:144-160: on junction-heavy fixtures the frozen projection's
+    # tau = 1 trial can fail where reassembling projects it (the goldens:
+    # junction-y-edgelengths tau 1.0 reassembled, 0.5 frozen); the device
+    # follows the oracle in each mode
+    import json
+    import os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "reference_goldens.json")))
+    fx = G["fixtures"]["junction-y"]
+    gd = G["goldens"]["junction-y-edgelengths"]
+    cs, ocs = build_set(gd)
+    taus = {}
+    for mode, frozen in (("reassemble", 0), ("frozen", 1)):
+        r = core.sobolev_step_set(fx["vertices"], fx["edges"], None, cs,
+                                  core.SobolevStepOptions(projection_mode=mode))
+        _, _, st = rco.sobolev_step(fx["vertices"], fx["edges"], ocs,
+                                    rco.default_opts(projection_frozen=frozen))
+        log(mode, r["stats"], st["tau"], st["projection_iterations"])
+        assert r["stats"]["tau"] == st["tau"]
+        assert r["stats"]["projectionIterations"] == st["projection_iterations"]
+        taus[mode] = r["stats"]["tau"]
+    assert taus["reassemble"] >= taus["frozen"]
+
+
+@pytest.mark.parametrize("V", [128, 320])
+def test_step_reassemble_multiblock_vs_oracle(V):
+    """Several 64-row factor blocks, barycenter + length + 2 pins, jittered
+    torus knot: tau and projection iterations exact, positions 1e-9."""
+    v, e = curves.torus_knot(3, 5, V, jitter=0.02, seed=3)
+    x0 = rco.barycenter_target(v, e)
+    # targets off the curve: two reassembled corrections (frozen: two as well)
+    L0 = rco.total_length(v, e) * 1.1
+    pins = curves.seeded_pins(V, 2, seed=2)
+    cs = [core.barycenter_block(x0), core.total_length_block(L0)]
+    ocs = rco.ConstraintSet().barycenter(x0).total_length(L0)
+    for p in pins:
+        cs.append(core.point_block(int(p), v[p] + 0.01))
+        ocs.point(int(p), v[p] + 0.01)
+    log("reassemble multiblock", V)
+    r = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(collect_field=True))
+    out, gto, st = rco.sobolev_step(v, e, ocs, rco.default_opts(projection_frozen=0))
+    log(r["stats"], st["tau"], st["projection_iterations"])
+    assert st["projection_iterations"] == 2
+    assert r["accepted"] == st["accepted"]
+    assert r["stats"]["tau"] == st["tau"]
+    assert r["stats"]["projectionIterations"] == st["projection_iterations"]
+    assert abs(r["energy"] - st["energy"]) <= 1e-11 * abs(st["energy"])
+    assert rel(r["vertices"], out) <= 1e-9
+
+
+def test_reassemble_mixed_precision_is_unsupported():
+    v, e = curves.trefoil_jittered(64, 0.01, seed=1)
+    x0 = rco.barycenter_target(v, e)
+    with pytest.raises(core.L.RcError, match="reassemble"):
+        core.sobolev_step_set(v, e, None, [core.barycenter_block(x0)],
+                              core.SobolevStepOptions(precision="mixed"))
