@@ -214,6 +214,52 @@ static rc_status build_topology(rc_ctx* ctx, const int32_t* edges) {
     return RC_OK;
 }
 
+// Buffers sized by the number of constraint rows k: the factor storage with
+// the 3 + 3k right-hand-side rows it carries below A, the solve columns, C,
+// the Schur system and its workspace. Allocated for RC_KSMALL rows at context
+// creation and grown (never shrunk) when a constraint set needs more; called
+// between steps only (nothing in flight uses them).
+static rc_status ensure_k_capacity(rc_ctx* ctx, int k) {
+    if (k <= ctx->kcap) return RC_OK;
+    const int kc = std::max(k, RC_KSMALL);
+    const int Vp = ctx->nblk * RC_NB;
+    const size_t nrhs = 3 + 3 * (size_t)kc;
+    const size_t frows = (size_t)Vp + RC_NB * ((nrhs + RC_NB - 1) / RC_NB);
+    const size_t n = (size_t)kc + 3;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    double** bufs[] = {&ctx->d_F, &ctx->d_rhs, &ctx->d_sol, &ctx->d_C, &ctx->d_dots, &ctx->d_bsum,
+                       &ctx->d_ssol, &ctx->d_Minv, &ctx->d_Cg, &ctx->d_phi, &ctx->d_schur};
+    const size_t sizes[] = {frows * Vp, nrhs * Vp, nrhs * Vp, (size_t)kc * 3 * Vp,
+                            (size_t)kc * (kc + 1), 3 * ((size_t)kc + 1), n, n * n, (size_t)kc,
+                            (size_t)kc + 8, kc > RC_KSMALL ? 2 * n * n + 2 * n : 1};
+    ctx->kcap = 0;
+    for (int i = 0; i < 11; ++i) {
+        if (*bufs[i]) (void)hipFree(*bufs[i]);
+        *bufs[i] = nullptr;
+    }
+    // the mixed form's PCG workspace is sized by the right-hand-side count:
+    // dropped here, reallocated at its next use
+    if (ctx->d_tmp2) (void)hipFree(ctx->d_tmp2);
+    if (ctx->d_tmp3) (void)hipFree(ctx->d_tmp3);
+    ctx->d_tmp2 = nullptr;
+    ctx->d_tmp3 = nullptr;
+    for (int i = 0; i < 11; ++i) {
+        hipError_t he = dalloc(bufs[i], sizes[i]);
+        if (he != hipSuccess) {
+            ctx->err = "hipMalloc (buffers for " + std::to_string(kc) +
+                       " constraint rows): " + hipGetErrorString(he);
+            return RC_HIP_ERROR;
+        }
+    }
+    // the factor storage starts finite (its padding outside V x V is read by
+    // the factorisation's tiles)
+    RC_HIP(hipMemset(ctx->d_F, 0, sizeof(double) * frows * Vp));
+    ctx->kcap = kc;
+    ctx->nrhs_cap = (int)nrhs;
+    return RC_OK;
+}
+
 extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, rc_ctx** out) {
     if (!out) return RC_BAD_ARG;
     *out = nullptr;
@@ -270,35 +316,22 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_rs, 2 * (size_t)ctx->E);
     ALLOC(ctx->d_Adiag, (size_t)Vp);
     ALLOC(ctx->d_symv, (size_t)ctx->nblk * ctx->nblk * 3 * RC_NB);
-    // factor plus the right-hand-side rows the factorisation carries along
-    // (3 + 3k <= 3 + 3 RC_KMAX rows, whole 64-row blocks)
-    ALLOC(ctx->d_F, (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, ctx->nblk);
     ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
     ALLOC(ctx->d_status, 4);
-    ctx->nrhs_cap = 3 + 3 * RC_KMAX;
-    ALLOC(ctx->d_rhs, (size_t)ctx->nrhs_cap * Vp);
-    ALLOC(ctx->d_sol, (size_t)ctx->nrhs_cap * Vp);
     ALLOC(ctx->d_tmp, 3 * (size_t)Vp);
-    ALLOC(ctx->d_C, (size_t)RC_KMAX * 3 * Vp);
     ALLOC(ctx->d_ell0, ctx->E);
     ALLOC(ctx->d_dir, 3 * (size_t)Vp);
     ALLOC(ctx->d_dE, 3 * (size_t)Vp);
     ALLOC(ctx->d_gt, 3 * (size_t)Vp);
     ALLOC(ctx->d_set, 1);
     ALLOC(ctx->d_ctl, 1);
-    ALLOC(ctx->d_dots, (size_t)RC_KMAX * (RC_KMAX + 1));
-    ALLOC(ctx->d_bsum, 3 * (RC_KMAX + 1));
-    ALLOC(ctx->d_ssol, RC_KMAX + 3);
-    ALLOC(ctx->d_Minv, (size_t)(RC_KMAX + 3) * (RC_KMAX + 3));
     ALLOC(ctx->d_Ax, 3 * (size_t)Vp);
-    ALLOC(ctx->d_Cg, RC_KMAX);
     ALLOC(ctx->d_stats, 16);
     ALLOC(ctx->d_epart_ls, 1024);
     ALLOC(ctx->d_refine, 8);
     ALLOC(ctx->d_stage, 3 * (size_t)V);
-    ALLOC(ctx->d_phi, RC_KMAX + 8);
 #undef ALLOC
     he = hipHostMalloc((void**)&ctx->d_host_stage, 4096, hipHostMallocDefault);
     if (he != hipSuccess) {
@@ -308,11 +341,6 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ctx->host_stage_bytes = 4096;
     he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
-    // the factor storage starts finite (its padding outside V x V is read
-    // by the factorisation's tiles)
-    if (he == hipSuccess)
-        he = hipMemset(ctx->d_F, 0,
-                       sizeof(double) * (size_t)(Vp + RC_NB * ((3 + 3 * RC_KMAX + RC_NB - 1) / RC_NB)) * Vp);
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
     // the inverse's zero blocks above the diagonal are never written
     if (he == hipSuccess)
@@ -321,6 +349,8 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         ctx->err = std::string("hipMemset: ") + hipGetErrorString(he);
         return fail(RC_HIP_ERROR);
     }
+    s = ensure_k_capacity(ctx, RC_KSMALL);
+    if (s != RC_OK) return fail(s);
     // edge endpoint tables for the pair kernels
     he = hipMemcpy(ctx->ed.i1, ctx->d_i1, sizeof(int32_t) * Epad, hipMemcpyDeviceToDevice);
     if (he == hipSuccess)
@@ -555,7 +585,7 @@ static rc_status make_set(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks, 
                    std::to_string(RC_KMAX);
         return RC_UNSUPPORTED;
     }
-    return RC_OK;
+    return ensure_k_capacity(ctx, row);
 }
 
 extern "C" int32_t rc_constraint_rows(const rc_ctx* ctx, const rc_block* blocks, int32_t nblocks) {

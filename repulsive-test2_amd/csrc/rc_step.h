@@ -5,7 +5,9 @@
 #include <cstdint>
 
 #define RC_MAXPIN 64
-#define RC_KMAX 64   // constraint rows supported on the device (Schur system in LDS)
+#define RC_KMAX 1024  // constraint rows supported on the device
+#define RC_KSMALL 60  // up to here the (k+3)^2 Schur system is factored in LDS; the
+                      // context's k-sized buffers start at this capacity and grow
 
 // ConstraintSet as POD (constraintSet.ts:39-52), in set order. At most one
 // barycenter block and one length block (total or per-edge), plus pins —

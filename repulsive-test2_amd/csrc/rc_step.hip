@@ -348,6 +348,122 @@ __global__ __launch_bounds__(256) void k_schur(const double* __restrict__ dots,
     if (t == 0 && bad) atomicOr(status, 4);
 }
 
+// The same system for n = k + 3 beyond the LDS bound: M and the right-hand
+// sides Z in a global workspace ws (2 n^2 + 2n doubles), one workgroup of
+// 1024 threads. Partial pivoting picks the largest |M[r][c]| with the lowest
+// row among equal values (the serial scan's choice); the back substitution
+// runs column-oriented (x_i of every column, then its products removed from
+// the rows above), parallel over rows and columns.
+__global__ __launch_bounds__(1024) void k_schur_global(const double* __restrict__ dots,
+                                                       const double* __restrict__ bsum, int k,
+                                                       int V, double* __restrict__ sol,
+                                                       double* __restrict__ Minv,
+                                                       double* __restrict__ ws,
+                                                       int32_t* __restrict__ status) {
+    __shared__ double bv[1024];
+    __shared__ int bi[1024];
+    __shared__ int piv_s, bad;
+    const int n = k + 3, w = n + 1;
+    double* M = ws;                   // n x n
+    double* Z = ws + (size_t)n * n;   // n x (n + 1)
+    const int t = threadIdx.x;
+    const double* CN = bsum;
+    const double* Nr = bsum + 3 * k;
+    for (int q = t; q < n * n; q += 1024) {
+        const int i = q / n, j = q % n;
+        double v = 0.0;
+        if (i < k && j < k) v = dots[i * (k + 1) + 1 + j];
+        else if (i < k) v = -CN[i * 3 + (j - k)] / V;
+        else if (j < k) v = CN[j * 3 + (i - k)];
+        M[q] = v;
+    }
+    for (int q = t; q < n * w; q += 1024) {
+        const int i = q / w, j = q % w;
+        double v;
+        if (j == 0) v = (i < k) ? dots[i * (k + 1)] : Nr[i - k];
+        else v = (i == j - 1) ? 1.0 : 0.0;
+        Z[q] = v;
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    for (int c = 0; c < n; ++c) {
+        // pivot: max |M[r][c]|, r >= c, lowest r on ties
+        double best = -1.0;
+        int bidx = c;
+        for (int r = c + t; r < n; r += 1024) {
+            const double a = fabs(M[(size_t)r * n + c]);
+            if (a > best) {
+                best = a;
+                bidx = r;
+            }
+        }
+        bv[t] = best;
+        bi[t] = bidx;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if (t < s) {
+                const double o = bv[t + s];
+                const int oi = bi[t + s];
+                if (o > bv[t] || (o == bv[t] && oi < bi[t])) {
+                    bv[t] = o;
+                    bi[t] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            piv_s = bi[0];
+            if (!(bv[0] > 0.0)) bad = 1;
+        }
+        __syncthreads();
+        const int pr = piv_s;
+        if (pr != c) {
+            for (int j = t; j < n; j += 1024) {
+                const double tmp = M[(size_t)c * n + j];
+                M[(size_t)c * n + j] = M[(size_t)pr * n + j];
+                M[(size_t)pr * n + j] = tmp;
+            }
+            for (int j = t; j < w; j += 1024) {
+                const double tmp = Z[(size_t)c * w + j];
+                Z[(size_t)c * w + j] = Z[(size_t)pr * w + j];
+                Z[(size_t)pr * w + j] = tmp;
+            }
+        }
+        __syncthreads();
+        const double pv = M[(size_t)c * n + c];
+        const int nr = n - c - 1, wm = n - c - 1, wz = w;
+        for (long q = t; q < (long)nr * (wm + wz); q += 1024) {
+            const int r = c + 1 + (int)(q / (wm + wz)), j = (int)(q % (wm + wz));
+            const double f = (pv != 0.0) ? M[(size_t)r * n + c] / pv : 0.0;
+            if (j < wm)
+                M[(size_t)r * n + c + 1 + j] -= f * M[(size_t)c * n + c + 1 + j];
+            else
+                Z[(size_t)r * w + (j - wm)] -= f * Z[(size_t)c * w + (j - wm)];
+        }
+        __syncthreads();
+    }
+    // back substitution, column-oriented
+    for (int i = n - 1; i >= 0; --i) {
+        const double d = M[(size_t)i * n + i];
+        for (int j = t; j < w; j += 1024) {
+            const double x = (d != 0.0) ? Z[(size_t)i * w + j] / d : 0.0;
+            Z[(size_t)i * w + j] = x;
+        }
+        __syncthreads();
+        for (long q = t; q < (long)i * w; q += 1024) {
+            const int r = (int)(q / w), j = (int)(q % w);
+            Z[(size_t)r * w + j] -= M[(size_t)r * n + i] * Z[(size_t)i * w + j];
+        }
+        __syncthreads();
+    }
+    for (int q = t; q < n; q += 1024) sol[q] = Z[(size_t)q * w];
+    for (int q = t; q < n * n; q += 1024) {
+        const int i = q / n, j = q % n;
+        Minv[q] = Z[(size_t)i * w + 1 + j];
+    }
+    if (t == 0 && bad) atomicOr(status, 4);
+}
+
 // g~_c[v] = X_c[v] - sum_q lambda_q Y_{q,c}[v] + t_c / V   (stride Vp)
 __global__ void k_gtilde(const double* __restrict__ X, const double* __restrict__ sol, int k,
                          int V, int Vp, double* __restrict__ gt) {
@@ -1070,7 +1186,11 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
     RC_CHECK_LAUNCH();
     const int n = k + 3;
     const size_t shm = sizeof(double) * ((size_t)n * n + (size_t)n * (n + 1)) + sizeof(int) * n;
-    if (k > 0) {
+    if (k > RC_KSMALL) {
+        hipLaunchKernelGGL(k_schur_global, dim3(1), dim3(1024), 0, ctx->stream, d_dots, d_bsum, k,
+                           ctx->V, d_sol, d_Minv, ctx->d_schur, ctx->d_status);
+        RC_CHECK_LAUNCH();
+    } else if (k > 0) {
         hipLaunchKernelGGL(k_schur, dim3(1), dim3(256), shm, ctx->stream, d_dots, d_bsum, k, ctx->V,
                            d_sol, d_Minv, ctx->d_status);
         RC_CHECK_LAUNCH();

@@ -4,6 +4,7 @@ relative, Gram matrix <= 1e-12 Frobenius-relative, g~/lambda <= 1e-9 (the
 reference's gradient.test.ts gate), step tau / projection iterations exact,
 energy <= 1e-12, vertices <= 1e-9 (lineSearch.test.ts gates)."""
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -335,3 +336,63 @@ def test_reassemble_mixed_precision_is_unsupported():
     with pytest.raises(core.L.RcError, match="reassemble"):
         core.sobolev_step_set(v, e, None, [core.barycenter_block(x0)],
                               core.SobolevStepOptions(precision="mixed"))
+
+
+@pytest.mark.parametrize("E", [64, 128, 256, 512])
+def test_per_edge_lengths_large_k_vs_oracle(E):
+    """Per-edge length set (constraintSet.ts:206-250) beyond the LDS Schur
+    system: k = E + 3 rows, the Schur system factored in global memory and
+    3 + 3k right-hand sides carried by the factorisation. Jittered trefoil
+    with every target 1% longer; frozen and reassembled projection against
+    the oracle's full-saddle LDL^T: tau and projection iterations exact."""
+    v, e = curves.trefoil_jittered(E, 0.01, seed=5)
+    x0 = rco.barycenter_target(v, e)
+    ell0 = core.edge_lengths(v, e) * 1.01
+    cs = [core.barycenter_block(x0), core.edge_lengths_block(ell0)]
+    ocs = rco.ConstraintSet().barycenter(x0).edge_lengths(ell0)
+    for mode, frozen in (("frozen", 1), ("reassemble", 0)):
+        log("per-edge", E, mode)
+        r = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
+            projection_mode=mode, collect_field=True))
+        out, gto, st = rco.sobolev_step(v, e, ocs, rco.default_opts(projection_frozen=frozen))
+        log("  ", r["stats"], st["tau"], st["projection_iterations"], rel(r["descentField"], gto),
+            rel(r["vertices"], out), abs(r["energy"] - st["energy"]) / abs(st["energy"]))
+        assert r["accepted"] == st["accepted"]
+        assert r["stats"]["tau"] == st["tau"]
+        assert r["stats"]["projectionIterations"] == st["projection_iterations"]
+        assert rel(r["descentField"], gto) <= 1e-9
+        assert rel(r["vertices"], out) <= 1e-9
+        assert abs(r["energy"] - st["energy"]) <= 1e-12 * abs(st["energy"])
+        assert r["stats"]["residual"] <= 1e-10
+
+
+def test_per_edge_lengths_at_the_row_limit():
+    """k = 3 + 1021 = 1024 rows, the device limit (the oracle's full saddle
+    at this size takes minutes): residual of the saddle solve, g~ tangent to
+    the constraints (C g~ = 0 to rounding), and the accepted curve meets every
+    block's projection tolerance. One row more is refused."""
+    E = 1021
+    v, e = curves.trefoil_jittered(E, 0.01, seed=6)
+    x0 = rco.barycenter_target(v, e)
+    ell0 = core.edge_lengths(v, e) * 1.005
+    cs = [core.barycenter_block(x0), core.edge_lengths_block(ell0)]
+    log("per-edge at the row limit", E)
+    t0 = time.perf_counter()
+    r = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
+        projection_mode="frozen", collect_field=True))
+    log("  ", r["stats"], f"{time.perf_counter() - t0:.2f}s")
+    assert r["stats"]["residual"] <= 1e-10
+    ocs = rco.ConstraintSet().barycenter(x0).edge_lengths(ell0)
+    _, C, _, _ = rco.eval_constraints(v, e, ocs)
+    gt = np.asarray(r["descentField"])
+    assert np.linalg.norm(C @ gt.T.ravel()) <= 1e-9 * np.linalg.norm(C) * np.linalg.norm(gt)
+    assert r["accepted"]
+    out = r["vertices"]
+    L = core.total_length(out, e)
+    assert np.linalg.norm(core.edge_lengths(out, e) - ell0) <= max(1e-4, 1e-4 * max(1.0, L))
+    with pytest.raises(core.L.RcError, match="exceed"):
+        v2, e2 = curves.trefoil_jittered(E + 1, 0.01, seed=6)
+        core.sobolev_step_set(v2, e2, None,
+                              [core.barycenter_block(x0),
+                               core.edge_lengths_block(core.edge_lengths(v2, e2))],
+                              core.SobolevStepOptions(projection_mode="frozen"))
