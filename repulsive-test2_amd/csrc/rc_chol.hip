@@ -1075,6 +1075,9 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     const int r = nblk - 1 - (int)blockIdx.x;
     const size_t rb = (size_t)r * NB;
     const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
+    // a wave whose 16 output columns are all padding (nrhs <= 16: waves 4-7)
+    // skips its MFMA products; it shares a SIMD with a wave that has work
+    const bool live = wc < nrhs;
     // L_rr^-1: diagonal 64 x 64 block (r & 1) of the 128 x 128 inverse
     // (row-major, leading dimension PB) of panel r / 2
     const double* Lrr = Linv128 + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
@@ -1144,10 +1147,12 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
         sstore_t();
         if (s + 2 < ndep) gload(nblk - 2 - s);
         __syncthreads();
+        if (live) {
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
-                                                       Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
+                                                           Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
+        }
         __syncthreads();
     }
     // Z = L_rr^-T acc (acc through Ys as the B operand)
@@ -1155,20 +1160,24 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     for (int q = 0; q < 4; ++q) Ys[wr + 4 * q + lk][wc + lr] = acc[q];
     __syncthreads();
     d4 x = {0.0, 0.0, 0.0, 0.0};
+    if (live) {
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
-        x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
+        for (int kk = 0; kk < 16; ++kk)
+            x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
+    }
     if (ndep > 0) {
         // X_r = Z - P^T X_{r+1}
         wait_flag(flags + r + 1, epoch, status);
         load_ys(r + 1);
         __syncthreads();
+        if (live) {
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+            for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                x = __builtin_amdgcn_mfma_f64_16x16x4f64(pt[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr],
-                                                         x, 0, 0, 0);
+                for (int q = 0; q < 4; ++q)
+                    x = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        pt[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr], x, 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
