@@ -396,3 +396,23 @@ def test_per_edge_lengths_at_the_row_limit():
                               [core.barycenter_block(x0),
                                core.edge_lengths_block(core.edge_lengths(v2, e2))],
                               core.SobolevStepOptions(projection_mode="frozen"))
+
+
+def test_per_edge_lengths_large_k_mixed_form():
+    """The mixed form's PCG on the 3 + 3k right-hand sides of a per-edge set
+    beyond the LDS Schur system (k = 131; its workspace is sized by k)."""
+    E = 128
+    v, e = curves.trefoil_jittered(E, 0.01, seed=7)
+    x0 = rco.barycenter_target(v, e)
+    cs = [core.barycenter_block(x0), core.edge_lengths_block(core.edge_lengths(v, e))]
+    ocs = rco.ConstraintSet().barycenter(x0).edge_lengths(core.edge_lengths(v, e))
+    dE = rco.gradient(v, e)
+    rc, gto, lamo, _ = rco.solve_constrained_gradient(v, e, dE, ocs)
+    assert rc == 0
+    for prec in ("fp64", "mixed"):
+        gt, lam, res = core.solve_constrained_gradient_set_frozen(v, e, None, 3.0, 6.0, 1e-10, dE,
+                                                                 cs, precision=prec)
+        log("per-edge k=131", prec, rel(gt, gto), rel(lam, lamo), res)
+        assert rel(gt, gto) <= 1e-9
+        assert rel(lam, lamo) <= 1e-8
+        assert res <= 1e-10
