@@ -147,6 +147,15 @@ rc_status rc_gradient_fd(rc_ctx* ctx, const double* xyz, const rc_params* p, dou
  * (StepOptions.collectField). xyz_in is not modified. */
 rc_status rc_step(rc_ctx* ctx, const double* xyz_in, const rc_params* p, int32_t mode,
                   double step_size, double h, double* xyz_out, double* energy, double* field);
+/* Barnes-Hut tangent-point energy: the reference's stage-2 oracle bh_energy
+ * (oracle/tpe_stage2_oracle.py:456-596) — edge BVH with leaf_size edges per
+ * leaf, a cluster replaces its edges for a target edge when it has >= 2
+ * edges, none sharing a vertex with the target, and rx / dist <= theta,
+ * rT <= theta_t (the oracle's default theta_t is 1.5 theta). theta = 0 gives
+ * the exact energy. stats (nullable, 3 entries): visits, approximated edge
+ * references, direct pairs. The tree is built on the host from xyz. */
+rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_params* p, double theta,
+                       double theta_t, int32_t leaf_size, double* energy, int64_t* stats);
 /* assembleAFlat (innerProduct.ts:325-350): V x V row-major A = B + B0. */
 rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A);
 

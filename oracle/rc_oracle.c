@@ -1510,3 +1510,259 @@ int rco_sobolev_step(int V, int E, const double* pos, const int32_t* edges,
     free(lambda);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Barnes-Hut energy — the reference's stage-2 oracle, bh_energy with
+ * build_edge_bvh and bh_ordered_cluster_term (oracle/tpe_stage2_oracle.py:
+ * 456-596). Edge BVH over midpoints and unit tangents: a node holds its edges
+ * in the order its parent passed them, its mass-weighted centre and tangent,
+ * the radii rx (centres) and rT (tangents); a node with more than leaf_size
+ * edges splits at the median of the 6-D coordinate (midpoint, tangent) with
+ * the largest spread, after a stable sort on it. Traversal per target edge I
+ * from the root: a node of >= 2 edges, none sharing a vertex with I, with
+ * 0 < dist, rx / dist <= theta and rT <= theta_t is replaced by one source
+ * point (the centre) carrying the node's mass; a leaf otherwise sums the
+ * exact ordered terms of its disjoint edges; E = 1/2 sum_I. Powers are the
+ * C library pow (the Python oracle's `**`). stats (nullable): visits,
+ * approximated edge references, direct pairs. */
+typedef struct {
+    int start, end;       /* range in the node-order array */
+    int left, right;      /* children, -1 for a leaf */
+    double cen[3], mass, rx, rT;
+} bh_node_t;
+
+typedef struct {
+    const double* C;      /* midpoints [E][3] */
+    const double* T;      /* unit tangents (0 below 1e-14) [E][3] */
+    const double* mass;   /* ell + eps */
+    int* ord;             /* node-order array: each node's edges contiguous */
+    int* tmp;
+    double* key;
+    bh_node_t* nodes;
+    int nnodes, leaf;
+} bh_build_t;
+
+static void bh_stable_sort(int* a, int n, const double* key, int* tmp) {
+    /* bottom-up merge sort on key[a[i]], equal keys keep their order */
+    for (int w = 1; w < n; w *= 2) {
+        for (int lo = 0; lo < n; lo += 2 * w) {
+            int mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = (key[a[j]] < key[a[i]]) ? a[j++] : a[i++];
+            while (i < mid) tmp[k++] = a[i++];
+            while (j < hi) tmp[k++] = a[j++];
+        }
+        memcpy(a, tmp, sizeof(int) * (size_t)n);
+    }
+}
+
+static int bh_build(bh_build_t* b, int start, int end) {
+    const int n = end - start;
+    const int id = b->nnodes++;
+    bh_node_t* nd = &b->nodes[id];
+    nd->start = start;
+    nd->end = end;
+    nd->left = nd->right = -1;
+    double ms = 0.0, c[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+    for (int p = start; p < end; p++) ms += b->mass[b->ord[p]];
+    if (ms > 0) {
+        for (int p = start; p < end; p++) {
+            const int e = b->ord[p];
+            for (int d = 0; d < 3; d++) {
+                c[d] += b->mass[e] * b->C[3 * e + d];
+                t[d] += b->mass[e] * b->T[3 * e + d];
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            c[d] /= ms;
+            t[d] /= ms;
+        }
+    } else if (n > 0) {
+        for (int p = start; p < end; p++)
+            for (int d = 0; d < 3; d++) {
+                c[d] += b->C[3 * b->ord[p] + d];
+                t[d] += b->T[3 * b->ord[p] + d];
+            }
+        for (int d = 0; d < 3; d++) {
+            c[d] /= n;
+            t[d] /= n;
+        }
+    }
+    double rx = 0.0, rT = 0.0;
+    for (int p = start; p < end; p++) {
+        const int e = b->ord[p];
+        double sx = 0.0, st = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double u = b->C[3 * e + d] - c[d], w = b->T[3 * e + d] - t[d];
+            sx += u * u;
+            st += w * w;
+        }
+        if (sqrt(sx) > rx) rx = sqrt(sx);
+        if (sqrt(st) > rT) rT = sqrt(st);
+    }
+    memcpy(nd->cen, c, sizeof(c));
+    nd->mass = ms;
+    nd->rx = rx;
+    nd->rT = rT;
+    if (n > b->leaf) {
+        int dim = 0;
+        double best = -1.0;
+        for (int d = 0; d < 6; d++) {
+            double lo = INFINITY, hi = -INFINITY;
+            for (int p = start; p < end; p++) {
+                const int e = b->ord[p];
+                const double x = d < 3 ? b->C[3 * e + d] : b->T[3 * e + d - 3];
+                if (x < lo) lo = x;
+                if (x > hi) hi = x;
+            }
+            if (hi - lo > best) {
+                best = hi - lo;
+                dim = d;
+            }
+        }
+        for (int p = start; p < end; p++) {
+            const int e = b->ord[p];
+            b->key[e] = dim < 3 ? b->C[3 * e + dim] : b->T[3 * e + dim - 3];
+        }
+        bh_stable_sort(b->ord + start, n, b->key, b->tmp);
+        const int mid = n / 2;
+        if (mid > 0 && n - mid > 0) {
+            const int l = bh_build(b, start, start + mid);
+            const int r = bh_build(b, start + mid, end);
+            b->nodes[id].left = l;
+            b->nodes[id].right = r;
+        }
+    }
+    return id;
+}
+
+typedef struct {
+    int V, E;
+    const double* v;
+    const int32_t* edges;
+    double alpha, beta, eps, theta, theta_t;
+    const bh_build_t* b;
+    const double* C;
+    long long visits, approx, direct;
+} bh_trav_t;
+
+static double bh_term_exact(const bh_trav_t* s, int I, int J) {
+    const double* v = s->v;
+    const int i1 = s->edges[2 * I], i2 = s->edges[2 * I + 1];
+    const int j1 = s->edges[2 * J], j2 = s->edges[2 * J + 1];
+    const double eI[3] = {v[3 * i2] - v[3 * i1], v[3 * i2 + 1] - v[3 * i1 + 1],
+                          v[3 * i2 + 2] - v[3 * i1 + 2]};
+    const double ellI = s->b->mass[I], ellJ = s->b->mass[J];
+    const int is[2] = {i1, i2}, js[2] = {j1, j2};
+    double sumK = 0.0;
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2; c++) {
+            const double d[3] = {v[3 * is[a]] - v[3 * js[c]], v[3 * is[a] + 1] - v[3 * js[c] + 1],
+                                 v[3 * is[a] + 2] - v[3 * js[c] + 2]};
+            const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + s->eps;
+            const double cx = eI[1] * d[2] - eI[2] * d[1], cy = eI[2] * d[0] - eI[0] * d[2],
+                         cz = eI[0] * d[1] - eI[1] * d[0];
+            const double cn = sqrt(cx * cx + cy * cy + cz * cz) + s->eps;
+            sumK += pow(cn, s->alpha) / pow(dn, s->beta);
+        }
+    return 0.25 * pow(ellI, 1.0 - s->alpha) * ellJ * sumK;
+}
+
+static double bh_term_cluster(const bh_trav_t* s, int I, const double* q, double mass) {
+    const double* v = s->v;
+    const int i1 = s->edges[2 * I], i2 = s->edges[2 * I + 1];
+    const double eI[3] = {v[3 * i2] - v[3 * i1], v[3 * i2 + 1] - v[3 * i1 + 1],
+                          v[3 * i2 + 2] - v[3 * i1 + 2]};
+    const int is[2] = {i1, i2};
+    double sum = 0.0;
+    for (int a = 0; a < 2; a++) {
+        const double d[3] = {v[3 * is[a]] - q[0], v[3 * is[a] + 1] - q[1], v[3 * is[a] + 2] - q[2]};
+        const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + s->eps;
+        const double cx = eI[1] * d[2] - eI[2] * d[1], cy = eI[2] * d[0] - eI[0] * d[2],
+                     cz = eI[0] * d[1] - eI[1] * d[0];
+        const double cn = sqrt(cx * cx + cy * cy + cz * cz) + s->eps;
+        sum += pow(cn, s->alpha) / pow(dn, s->beta);
+    }
+    return 0.5 * pow(s->b->mass[I], 1.0 - s->alpha) * mass * sum;
+}
+
+static double bh_traverse(bh_trav_t* s, int I, int ni) {
+    s->visits++;
+    const bh_node_t* nd = &s->b->nodes[ni];
+    const int n = nd->end - nd->start;
+    if (n == 0) return 0.0;
+    const double dx = s->C[3 * I] - nd->cen[0], dy = s->C[3 * I + 1] - nd->cen[1],
+                 dz = s->C[3 * I + 2] - nd->cen[2];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    int admiss = n >= 2;
+    if (admiss)
+        for (int p = nd->start; p < nd->end; p++)
+            if (!disjoint(s->edges, I, s->b->ord[p])) {
+                admiss = 0;
+                break;
+            }
+    admiss = admiss && dist > 0.0 && nd->rx / dist <= s->theta && nd->rT <= s->theta_t;
+    if (admiss) {
+        s->approx += n;
+        return bh_term_cluster(s, I, nd->cen, nd->mass);
+    }
+    if (nd->left < 0) {
+        double val = 0.0;
+        for (int p = nd->start; p < nd->end; p++) {
+            const int J = s->b->ord[p];
+            if (!disjoint(s->edges, I, J)) continue;
+            s->direct++;
+            val += bh_term_exact(s, I, J);
+        }
+        return val;
+    }
+    return bh_traverse(s, I, nd->left) + bh_traverse(s, I, nd->right);
+}
+
+double rco_energy_bh(int V, int E, const double* v, const int32_t* edges, double alpha,
+                     double beta, double eps, double theta, double theta_t, int leaf_size,
+                     int64_t* stats) {
+    double* C = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    double* T = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    double* mass = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    for (int K = 0; K < E; K++) {
+        const int a = edges[2 * K], b = edges[2 * K + 1];
+        double e[3], r2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            e[d] = v[3 * b + d] - v[3 * a + d];
+            r2 += e[d] * e[d];
+            C[3 * K + d] = 0.5 * (v[3 * a + d] + v[3 * b + d]);
+        }
+        const double r = sqrt(r2);
+        mass[K] = r + eps;
+        for (int d = 0; d < 3; d++) T[3 * K + d] = (r >= 1e-14) ? e[d] / r : 0.0;
+    }
+    bh_build_t b;
+    b.C = C;
+    b.T = T;
+    b.mass = mass;
+    b.ord = (int*)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    b.tmp = (int*)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    b.key = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    b.nodes = (bh_node_t*)malloc(sizeof(bh_node_t) * (size_t)(2 * E + 1));
+    b.nnodes = 0;
+    b.leaf = leaf_size;
+    for (int K = 0; K < E; K++) b.ord[K] = K;
+    bh_build(&b, 0, E);
+    bh_trav_t s = {V, E, v, edges, alpha, beta, eps, theta, theta_t, &b, C, 0, 0, 0};
+    double total = 0.0;
+    for (int I = 0; I < E; I++) total += bh_traverse(&s, I, 0);
+    if (stats) {
+        stats[0] = s.visits;
+        stats[1] = s.approx;
+        stats[2] = s.direct;
+    }
+    free(C);
+    free(T);
+    free(mass);
+    free(b.ord);
+    free(b.tmp);
+    free(b.key);
+    free(b.nodes);
+    return 0.5 * total;
+}

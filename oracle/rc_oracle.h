@@ -103,6 +103,12 @@ void rco_gradient_fd(int V, int E, const double* pos, const int32_t* edges,
 void rco_step(int V, int E, const double* pos, const int32_t* edges, double alpha, double beta,
               double eps, int mode, double step_size, double h, double* out, double* energy,
               double* field);
+/* Barnes-Hut energy (the reference's stage-2 oracle, tpe_stage2_oracle.py:
+ * 456-596): edge BVH with leaf_size, spatial ratio theta, tangent radius
+ * theta_t. stats (nullable): visits, approximated edge refs, direct pairs. */
+double rco_energy_bh(int V, int E, const double* pos, const int32_t* edges, double alpha,
+                     double beta, double eps, double theta, double theta_t, int leaf_size,
+                     int64_t* stats);
 /* Same energy with a compensated (Neumaier) outer sum: separates the
  * reference's left-to-right rounding from device error at large E. */
 double rco_energy_compensated(int V, int E, const double* pos, const int32_t* edges,

@@ -125,6 +125,8 @@ def lib():
         L.rco_energy_compensated.argtypes = [i, i, _dp, _ip, d, d, d]
         L.rco_gradient.argtypes = [i, i, _dp, _ip, d, d, d, _dp]
         L.rco_gradient_fd.argtypes = [i, i, _dp, _ip, d, d, d, d, _dp]
+        L.rco_energy_bh.restype = d
+        L.rco_energy_bh.argtypes = [i, i, _dp, _ip, d, d, d, d, d, i, C.POINTER(C.c_int64)]
         L.rco_step.argtypes = [i, i, _dp, _ip, d, d, d, i, d, d, _dp, _dp, _dp]
         for nm in ("rco_assemble_b_high", "rco_assemble_b_low", "rco_assemble_a"):
             getattr(L, nm).argtypes = [i, i, _dp, _ip, d, d, d, _dp]
@@ -191,6 +193,16 @@ def gradient_fd(v, e, alpha=3.0, beta=6.0, eps=1e-10, h=1e-6):
     g = np.zeros_like(v)
     lib().rco_gradient_fd(len(v), len(e), _d(v), _i(e), alpha, beta, eps, h, _d(g))
     return g
+
+
+def energy_bh(v, e, theta, theta_t=None, leaf_size=8, alpha=3.0, beta=6.0, eps=1e-10):
+    """Barnes-Hut energy and (visits, approximated edge refs, direct pairs)."""
+    v, e = _verts(v), _edges(e)
+    st = (C.c_int64 * 3)()
+    tt = 1.5 * theta if theta_t is None else theta_t
+    en = lib().rco_energy_bh(len(v), len(e), _d(v), _i(e), alpha, beta, eps, theta, tt,
+                             leaf_size, st)
+    return en, (st[0], st[1], st[2])
 
 
 MODE_ANALYTICAL, MODE_FINITE_DIFF = 0, 1

@@ -131,6 +131,12 @@ struct rc_ctx {
     double* d_epart_ls = nullptr;
     double* d_refine = nullptr;
 
+    // Barnes-Hut tree on the device (rc_energy_bh), grown on demand
+    double* d_bh_f = nullptr;       // 6 doubles per node
+    int32_t* d_bh_i = nullptr;      // 4 ints per node + node order + positions
+    unsigned long long* d_bh_stats = nullptr;
+    size_t bh_cap_d = 0, bh_cap_i = 0;
+
     // kernel-level profile (rc_kernel_profile_enable / _read)
     int kprof = 0;
     std::vector<hipEvent_t> kev;   // event pool, pairs around trailing-update launches

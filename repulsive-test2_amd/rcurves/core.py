@@ -117,6 +117,20 @@ def gradient_finite_diff(vertices, edges, disjoint_pairs, alpha, beta, epsilon, 
     return g
 
 
+def bh_energy(vertices, edges, alpha, beta, epsilon, theta=0.5, leaf_size=8, theta_t=None):
+    """Barnes-Hut energy (the reference's stage-2 oracle bh_energy,
+    oracle/tpe_stage2_oracle.py:532-596): (energy, {visits,
+    approximated_edge_refs, direct_pairs}); theta_t defaults to 1.5 theta."""
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    tt = 1.5 * theta if theta_t is None else theta_t
+    out = C.c_double()
+    st = (C.c_int64 * 3)()
+    ctx.check(L.lib().rc_energy_bh(ctx.handle, L.dptr(v), C.byref(_params(alpha, beta, epsilon)),
+                                   float(theta), float(tt), int(leaf_size), C.byref(out), st))
+    return out.value, {"visits": st[0], "approximated_edge_refs": st[1], "direct_pairs": st[2]}
+
+
 _MODES = {"analytical": L.MODE_ANALYTICAL, "finiteDiff": L.MODE_FINITE_DIFF}
 
 

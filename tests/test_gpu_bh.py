@@ -1,0 +1,62 @@
+"""Barnes-Hut tangent-point energy on the device (rc_energy_bh) against the
+reference's stage-2 oracle vectors (tests/golden/bh_goldens.json, made by
+tests/golden/make_bh_goldens.py) and the oracle's C restatement: energy to
+1e-12 and the traversal counts (visits, approximated edge references, direct
+pairs) exactly; theta = 0 reproduces the exact energy; the reference's gates
+(oracle/check_bh_stage2.py): clustering fires and the error stays within
+5e-2 at (theta, theta_t) = (0.5, 0.75) and shrinks at (0.25, 0.5)."""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import rc_oracle as rco
+from rcurves import core, curves
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_bh_energy_vs_reference_vectors():
+    G = json.load(open(os.path.join(HERE, "golden", "bh_goldens.json")))
+    for c in G["cases"]:
+        sh = G["shapes"][c["shape"]]
+        log("bh", c["shape"], c["theta"], c["theta_t"], c["leaf_size"])
+        en, st = core.bh_energy(sh["vertices"], sh["edges"], 3.0, 6.0, 1e-10, c["theta"],
+                                c["leaf_size"], c["theta_t"])
+        assert abs(en - c["energy"]) <= 1e-12 * abs(c["energy"]), (c, en)
+        assert (st["visits"], st["approximated_edge_refs"], st["direct_pairs"]) == (
+            c["visits"], c["approximated_edge_refs"], c["direct_pairs"]), (c, st)
+
+
+@pytest.mark.parametrize("n", [2048, 8192])
+def test_bh_energy_vs_oracle_at_size(n):
+    v, e = curves.fourier_curve(n, seed=1)
+    exact = core.calculate_energy(v, e, None, 3.0, 6.0, 1e-10)
+    errs = {}
+    for theta, theta_t, leaf in ((0.5, 0.75, 4), (0.25, 0.5, 4), (0.0, 0.0, 8)):
+        log("bh", n, theta, theta_t, leaf)
+        t0 = time.perf_counter()
+        en, st = core.bh_energy(v, e, 3.0, 6.0, 1e-10, theta, leaf, theta_t)
+        log(f"  {1e3 * (time.perf_counter() - t0):.1f} ms (tree on the host + traversal)", st)
+        eo, so = rco.energy_bh(v, e, theta, theta_t, leaf)
+        assert abs(en - eo) <= 1e-12 * abs(eo)
+        assert (st["visits"], st["approximated_edge_refs"], st["direct_pairs"]) == so
+        errs[theta] = abs(en - exact) / abs(exact)
+    assert errs[0.0] <= 1e-12
+    assert 0.0 < errs[0.25] < errs[0.5] <= 5e-2
+
+
+def test_bh_rejects_bad_parameters():
+    v, e = curves.trefoil_jittered(32, 0.01, seed=1)
+    for theta, tt, leaf in ((-1.0, 0.5, 4), (float("nan"), 0.5, 4), (0.5, 0.75, 0)):
+        with pytest.raises(core.L.RcError):
+            core.bh_energy(v, e, 3.0, 6.0, 1e-10, theta, leaf, tt)

@@ -228,3 +228,26 @@ def test_ldlt_reconstruction_random():
         b = rng.standard_normal(n)
         x = rco.ldlt_solve(m, ipiv, b)
         assert np.linalg.norm(K @ x - b) <= 1e-12 * np.linalg.norm(K, 2) * np.linalg.norm(x)
+
+
+def test_bh_energy_oracle_vs_reference_vectors():
+    # Barnes-Hut energy (tpe_stage2_oracle.py:456-596): the reference oracle's
+    # own outputs (tests/golden/make_bh_goldens.py), 5 curves x 6 settings incl.
+    # theta = 0 (exact) and clustering at leaf 2/4/8: energy 1e-12 and the
+    # traversal counts exactly (they pin the tree and the admissibility rule)
+    import json
+    import os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "bh_goldens.json")))
+    assert len(G["cases"]) == 30
+    clustered = 0
+    for c in G["cases"]:
+        sh = G["shapes"][c["shape"]]
+        en, st = rco.energy_bh(sh["vertices"], sh["edges"], c["theta"], c["theta_t"],
+                               c["leaf_size"])
+        assert abs(en - c["energy"]) <= 1e-12 * abs(c["energy"]), c
+        assert st == (c["visits"], c["approximated_edge_refs"], c["direct_pairs"]), c
+        if c["theta"] == 0.0:
+            assert st[1] == 0
+            assert abs(en - rco.energy(sh["vertices"], sh["edges"])) <= 1e-12 * abs(en)
+        clustered += st[1] > 0
+    assert clustered >= 10
