@@ -1528,7 +1528,7 @@ int rco_sobolev_step(int V, int E, const double* pos, const int32_t* edges,
 typedef struct {
     int start, end;       /* range in the node-order array */
     int left, right;      /* children, -1 for a leaf */
-    double cen[3], mass, rx, rT;
+    double cen[3], tan[3], mass, rx, rT;
 } bh_node_t;
 
 typedef struct {
@@ -1601,6 +1601,7 @@ static int bh_build(bh_build_t* b, int start, int end) {
         if (sqrt(st) > rT) rT = sqrt(st);
     }
     memcpy(nd->cen, c, sizeof(c));
+    memcpy(nd->tan, t, sizeof(t));
     nd->mass = ms;
     nd->rx = rx;
     nd->rT = rT;
@@ -1765,4 +1766,241 @@ double rco_energy_bh(int V, int E, const double* v, const int32_t* edges, double
     free(b.key);
     free(b.nodes);
     return 0.5 * total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block-cluster-tree products of the two edge kernels — the reference's
+ * stage-2 oracle bct_matvec_kernel, bct_kernel_node, exact_edge_kernel_pair
+ * and bct_apply_B_scalar (oracle/tpe_stage2_oracle.py:598-795). Kernel
+ * 'high' (kind 0): k_IJ = 2 * 1/4 ell_I ell_J sum_ij (|p_i - q_j| + eps)^-p;
+ * 'low' (kind 1): 1/4 ell_I ell_J sum_ij [(|T_I x d| + eps)^2 + (|T_J x -d| +
+ * eps)^2] (|d| + eps)^-(4+p), p = 7/3, disjoint pairs only. Dual recursion
+ * from (root, root): a block of >= 3 edges in total, no pair sharing a vertex,
+ * 0 < dist of the centres, max rx / dist <= theta, max rT <= theta_t is one
+ * rank-one term kernel(centres, mean tangents) * mass_I * sum_J mass_J psi_J;
+ * two leaves otherwise sum the exact pairs; else the target splits when it is
+ * not a leaf and the source is a leaf or not larger, else the source. */
+typedef struct {
+    const double* v;
+    const int32_t* edges;
+    double eps, theta, theta_t;
+    int kind, ncomp;
+    const bh_build_t* b;
+    const double* psi;
+    double* phi;
+    long long leaves, adm, dir, pairs;
+} bct_t;
+
+static double bct_exact_pair(const bct_t* s, int I, int J) {
+    if (!disjoint(s->edges, I, J)) return 0.0;
+    const double* v = s->v;
+    const double p = 7.0 / 3.0;
+    const int is[2] = {s->edges[2 * I], s->edges[2 * I + 1]};
+    const int js[2] = {s->edges[2 * J], s->edges[2 * J + 1]};
+    const double ellI = s->b->mass[I], ellJ = s->b->mass[J];
+    const double* TI = s->b->T + 3 * I;
+    const double* TJ = s->b->T + 3 * J;
+    double sh = 0.0, sIJ = 0.0, sJI = 0.0;
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2; c++) {
+            const double d[3] = {v[3 * is[a]] - v[3 * js[c]], v[3 * is[a] + 1] - v[3 * js[c] + 1],
+                                 v[3 * is[a] + 2] - v[3 * js[c] + 2]};
+            const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + s->eps;
+            if (s->kind == 0) {
+                sh += 1.0 / pow(dn, p);
+            } else {
+                const double c1x = TI[1] * d[2] - TI[2] * d[1], c1y = TI[2] * d[0] - TI[0] * d[2],
+                             c1z = TI[0] * d[1] - TI[1] * d[0];
+                const double c2x = -(TJ[1] * d[2] - TJ[2] * d[1]),
+                             c2y = -(TJ[2] * d[0] - TJ[0] * d[2]),
+                             c2z = -(TJ[0] * d[1] - TJ[1] * d[0]);
+                const double n1 = sqrt(c1x * c1x + c1y * c1y + c1z * c1z) + s->eps;
+                const double n2 = sqrt(c2x * c2x + c2y * c2y + c2z * c2z) + s->eps;
+                sIJ += pow(n1, 2.0) / pow(dn, 4.0 + p);
+                sJI += pow(n2, 2.0) / pow(dn, 4.0 + p);
+            }
+        }
+    return s->kind == 0 ? 2.0 * 0.25 * ellI * ellJ * sh : 0.25 * ellI * ellJ * (sIJ + sJI);
+}
+
+static double bct_node_kernel(const bct_t* s, const bh_node_t* t, const bh_node_t* r) {
+    const double d[3] = {t->cen[0] - r->cen[0], t->cen[1] - r->cen[1], t->cen[2] - r->cen[2]};
+    const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + s->eps;
+    if (s->kind == 0) return 2.0 / pow(dn, 7.0 / 3.0);
+    const double c1x = t->tan[1] * d[2] - t->tan[2] * d[1], c1y = t->tan[2] * d[0] - t->tan[0] * d[2],
+                 c1z = t->tan[0] * d[1] - t->tan[1] * d[0];
+    const double c2x = -(r->tan[1] * d[2] - r->tan[2] * d[1]),
+                 c2y = -(r->tan[2] * d[0] - r->tan[0] * d[2]),
+                 c2z = -(r->tan[0] * d[1] - r->tan[1] * d[0]);
+    const double c1 = sqrt(c1x * c1x + c1y * c1y + c1z * c1z) + s->eps;
+    const double c2 = sqrt(c2x * c2x + c2y * c2y + c2z * c2z) + s->eps;
+    return (pow(c1, 2.0) + pow(c2, 2.0)) / pow(dn, 4.0 + 7.0 / 3.0);
+}
+
+static void bct_rec(bct_t* s, int ti, int si) {
+    const bh_node_t* t = &s->b->nodes[ti];
+    const bh_node_t* r = &s->b->nodes[si];
+    const int nt = t->end - t->start, ns = r->end - r->start;
+    const double dx = t->cen[0] - r->cen[0], dy = t->cen[1] - r->cen[1], dz = t->cen[2] - r->cen[2];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    int well = nt + ns >= 3;
+    if (well)
+        for (int p = t->start; p < t->end && well; p++)
+            for (int q = r->start; q < r->end; q++)
+                if (!disjoint(s->edges, s->b->ord[p], s->b->ord[q])) {
+                    well = 0;
+                    break;
+                }
+    well = well && dist > 0.0 && fmax(t->rx, r->rx) / dist <= s->theta &&
+           fmax(t->rT, r->rT) <= s->theta_t;
+    const int nc = s->ncomp;
+    if (well) {
+        s->leaves++;
+        s->adm++;
+        const double kval = bct_node_kernel(s, t, r);
+        double tmp[3] = {0, 0, 0};
+        for (int q = r->start; q < r->end; q++) {
+            const int J = s->b->ord[q];
+            for (int c = 0; c < nc; c++) tmp[c] += s->b->mass[J] * s->psi[nc * J + c];
+        }
+        for (int p = t->start; p < t->end; p++) {
+            const int I = s->b->ord[p];
+            for (int c = 0; c < nc; c++) s->phi[nc * I + c] += s->b->mass[I] * kval * tmp[c];
+        }
+        return;
+    }
+    if (t->left < 0 && r->left < 0) {
+        s->leaves++;
+        s->dir++;
+        for (int p = t->start; p < t->end; p++) {
+            const int I = s->b->ord[p];
+            for (int q = r->start; q < r->end; q++) {
+                const int J = s->b->ord[q];
+                if (!disjoint(s->edges, I, J)) continue;
+                const double k = bct_exact_pair(s, I, J);
+                for (int c = 0; c < nc; c++) s->phi[nc * I + c] += k * s->psi[nc * J + c];
+                s->pairs++;
+            }
+        }
+        return;
+    }
+    const int split_t = t->left >= 0 && (r->left < 0 || nt >= ns);
+    if (split_t) {
+        bct_rec(s, t->left, si);
+        bct_rec(s, t->right, si);
+    } else {
+        bct_rec(s, ti, r->left);
+        bct_rec(s, ti, r->right);
+    }
+}
+
+/* geometry + tree shared by the products */
+static void bct_prepare(int E, const double* v, const int32_t* edges, double eps, int leaf,
+                        bh_build_t* b, double** C, double** T, double** mass) {
+    *C = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    *T = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    *mass = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    for (int K = 0; K < E; K++) {
+        const int a = edges[2 * K], c = edges[2 * K + 1];
+        double e[3], r2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            e[d] = v[3 * c + d] - v[3 * a + d];
+            r2 += e[d] * e[d];
+            (*C)[3 * K + d] = 0.5 * (v[3 * a + d] + v[3 * c + d]);
+        }
+        const double r = sqrt(r2);
+        (*mass)[K] = r + eps;
+        for (int d = 0; d < 3; d++) (*T)[3 * K + d] = (r >= 1e-14) ? e[d] / r : 0.0;
+    }
+    b->C = *C;
+    b->T = *T;
+    b->mass = *mass;
+    b->ord = (int*)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    b->tmp = (int*)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    b->key = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    b->nodes = (bh_node_t*)malloc(sizeof(bh_node_t) * (size_t)(2 * E + 1));
+    b->nnodes = 0;
+    b->leaf = leaf;
+    for (int K = 0; K < E; K++) b->ord[K] = K;
+    bh_build(b, 0, E);
+}
+
+static void bct_release(bh_build_t* b, double* C, double* T, double* mass) {
+    free(C);
+    free(T);
+    free(mass);
+    free(b->ord);
+    free(b->tmp);
+    free(b->key);
+    free(b->nodes);
+}
+
+static void bct_run(int E, const double* v, const int32_t* edges, double eps, const double* psi,
+                    int ncomp, int kind, double theta, double theta_t, const bh_build_t* b,
+                    double* phi, int64_t* stats) {
+    bct_t s = {v, edges, eps, theta, theta_t, kind, ncomp, b, psi, phi, 0, 0, 0, 0};
+    memset(phi, 0, sizeof(double) * (size_t)ncomp * (size_t)E);
+    if (E > 0) bct_rec(&s, 0, 0);
+    if (stats) {
+        stats[0] = s.leaves;
+        stats[1] = s.adm;
+        stats[2] = s.dir;
+        stats[3] = s.pairs;
+    }
+}
+
+void rco_bct_matvec(int V, int E, const double* v, const int32_t* edges, double eps,
+                    const double* psi, int ncomp, int kind, double theta, double theta_t,
+                    int leaf_size, double* phi, int64_t* stats) {
+    (void)V;
+    bh_build_t b;
+    double *C, *T, *mass;
+    bct_prepare(E, v, edges, eps, leaf_size, &b, &C, &T, &mass);
+    bct_run(E, v, edges, eps, psi, ncomp, kind, theta, theta_t, &b, phi, stats);
+    bct_release(&b, C, T, mass);
+}
+
+void rco_bct_apply_b(int V, int E, const double* v, const int32_t* edges, double eps,
+                     const double* u, int kind, double theta, double theta_t, int leaf_size,
+                     double* out) {
+    bh_build_t b;
+    double *C, *T, *mass;
+    bct_prepare(E, v, edges, eps, leaf_size, &b, &C, &T, &mass);
+    const int nc = kind == 0 ? 3 : 1;
+    double* ones = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    double* y = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    double* K1 = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    double* Ky = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    for (int I = 0; I < E; I++) {
+        ones[I] = 1.0;
+        const int a = edges[2 * I], c = edges[2 * I + 1];
+        if (kind == 0) {
+            const double du = (u[c] - u[a]) / mass[I];
+            for (int d = 0; d < 3; d++) y[3 * I + d] = du * T[3 * I + d];
+        } else {
+            y[I] = 0.5 * (u[a] + u[c]);
+        }
+    }
+    bct_run(E, v, edges, eps, ones, 1, kind, theta, theta_t, &b, K1, NULL);
+    bct_run(E, v, edges, eps, y, nc, kind, theta, theta_t, &b, Ky, NULL);
+    memset(out, 0, sizeof(double) * (size_t)V);
+    for (int I = 0; I < E; I++) {
+        const int a = edges[2 * I], c = edges[2 * I + 1];
+        if (kind == 0) {
+            double dot = 0.0;
+            for (int d = 0; d < 3; d++) dot += (K1[I] * y[3 * I + d] - Ky[3 * I + d]) * T[3 * I + d];
+            const double coeff = dot / mass[I];
+            out[a] -= coeff;
+            out[c] += coeff;
+        } else {
+            const double z = K1[I] * y[I] - Ky[I];
+            out[a] += 0.5 * z;
+            out[c] += 0.5 * z;
+        }
+    }
+    free(ones);
+    free(y);
+    free(K1);
+    free(Ky);
+    bct_release(&b, C, T, mass);
 }

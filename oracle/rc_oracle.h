@@ -109,6 +109,17 @@ void rco_step(int V, int E, const double* pos, const int32_t* edges, double alph
 double rco_energy_bh(int V, int E, const double* pos, const int32_t* edges, double alpha,
                      double beta, double eps, double theta, double theta_t, int leaf_size,
                      int64_t* stats);
+/* Block-cluster-tree products (the reference's stage-2 oracle,
+ * tpe_stage2_oracle.py:598-795): phi = K psi for the edge kernel kind
+ * (0 'high', 1 'low'), psi/phi [E][ncomp] (ncomp 1 or 3); stats (nullable, 4):
+ * leaves, admissible blocks, direct blocks, direct pairs. rco_bct_apply_b is
+ * bct_apply_B_scalar: out[V] = B u through two products. */
+void rco_bct_matvec(int V, int E, const double* pos, const int32_t* edges, double eps,
+                    const double* psi, int ncomp, int kind, double theta, double theta_t,
+                    int leaf_size, double* phi, int64_t* stats);
+void rco_bct_apply_b(int V, int E, const double* pos, const int32_t* edges, double eps,
+                     const double* u, int kind, double theta, double theta_t, int leaf_size,
+                     double* out);
 /* Same energy with a compensated (Neumaier) outer sum: separates the
  * reference's left-to-right rounding from device error at large E. */
 double rco_energy_compensated(int V, int E, const double* pos, const int32_t* edges,

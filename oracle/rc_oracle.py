@@ -125,6 +125,9 @@ def lib():
         L.rco_energy_compensated.argtypes = [i, i, _dp, _ip, d, d, d]
         L.rco_gradient.argtypes = [i, i, _dp, _ip, d, d, d, _dp]
         L.rco_gradient_fd.argtypes = [i, i, _dp, _ip, d, d, d, d, _dp]
+        L.rco_bct_matvec.argtypes = [i, i, _dp, _ip, d, _dp, i, i, d, d, i, _dp,
+                                     C.POINTER(C.c_int64)]
+        L.rco_bct_apply_b.argtypes = [i, i, _dp, _ip, d, _dp, i, d, d, i, _dp]
         L.rco_energy_bh.restype = d
         L.rco_energy_bh.argtypes = [i, i, _dp, _ip, d, d, d, d, d, i, C.POINTER(C.c_int64)]
         L.rco_step.argtypes = [i, i, _dp, _ip, d, d, d, i, d, d, _dp, _dp, _dp]
@@ -203,6 +206,29 @@ def energy_bh(v, e, theta, theta_t=None, leaf_size=8, alpha=3.0, beta=6.0, eps=1
     en = lib().rco_energy_bh(len(v), len(e), _d(v), _i(e), alpha, beta, eps, theta, tt,
                              leaf_size, st)
     return en, (st[0], st[1], st[2])
+
+
+def bct_matvec(v, e, psi, kind, theta, theta_t=None, leaf_size=8, eps=1e-10):
+    """K psi through the block cluster tree; kind 'high' | 'low'."""
+    v, e = _verts(v), _edges(e)
+    psi = np.ascontiguousarray(np.asarray(psi, dtype=np.float64))
+    nc = 1 if psi.ndim == 1 else psi.shape[1]
+    phi = np.zeros_like(psi)
+    st = (C.c_int64 * 4)()
+    tt = 1.5 * theta if theta_t is None else theta_t
+    lib().rco_bct_matvec(len(v), len(e), _d(v), _i(e), eps, _d(psi), nc,
+                         0 if kind == "high" else 1, theta, tt, leaf_size, _d(phi), st)
+    return phi, (st[0], st[1], st[2], st[3])
+
+
+def bct_apply_b(v, e, u, kind, theta, theta_t=None, leaf_size=8, eps=1e-10):
+    v, e = _verts(v), _edges(e)
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    out = np.zeros(len(v))
+    tt = 1.5 * theta if theta_t is None else theta_t
+    lib().rco_bct_apply_b(len(v), len(e), _d(v), _i(e), eps, _d(u), 0 if kind == "high" else 1,
+                          theta, tt, leaf_size, _d(out))
+    return out
 
 
 MODE_ANALYTICAL, MODE_FINITE_DIFF = 0, 1

@@ -1,4 +1,5 @@
-"""Writes Barnes-Hut energy vectors of the reference's stage-2 oracle.
+"""Writes Barnes-Hut energy and block-cluster-tree matvec vectors of the
+reference's stage-2 oracle.
 
 The Barnes-Hut energy (oracle/tpe_stage2_oracle.py:456-596 of the reference:
 edge BVH, admissibility with separate spatial and tangent thresholds, cluster
@@ -8,8 +9,11 @@ vectors are produced here by running that oracle once, in a container where
 the reference checkout is mounted. Inputs are the closed trefoils of the
 reference's own gate script (oracle/check_bh_stage2.py: scale 0.25, N = 60
 and 120) and three of its golden fixtures (a junction, two linked rings, a
-knot); outputs are the energy and the traversal counts. Only these numbers
-are written (tests/golden/bh_goldens.json), no reference code.
+knot); outputs are the energy and the traversal counts, and for the
+block-cluster-tree products of the two edge kernels (bct_matvec_kernel,
+bct_apply_B_scalar, :637-795; equally without a golden that admits a block)
+the products of fixed densities and the block counts. Only these numbers are
+written (tests/golden/bh_goldens.json), no reference code.
 
 Usage: python tests/golden/make_bh_goldens.py [/root/reference]
 """
@@ -57,13 +61,33 @@ def main(ref):
                           "visits": st["visits"],
                           "approximated_edge_refs": st["approximated_edge_refs"],
                           "direct_pairs": st["direct_pairs"]})
+    # block-cluster-tree matvecs of the edge kernels (tpe_stage2_oracle.py:598-795):
+    # K psi for a scalar and a 3-vector density, and the Gram-operator apply B u
+    bct = []
+    for name, (v, e) in shapes.items():
+        m, n = len(e), len(v)
+        i = np.arange(m)
+        psi1 = np.sin(i * 0.61) + 0.2
+        psi3 = np.stack([np.sin(0.37 * i), np.cos(0.53 * i), np.sin(0.11 * i) + 0.3], axis=1)
+        u = np.sin(0.7 * np.arange(n)) + 0.1 * np.arange(n) / n
+        for theta, theta_t, leaf in [(0.0, None, 2), (0.5, 0.75, 4), (0.25, 0.5, 4),
+                                     (0.5, None, 8)]:
+            tt = 1.5 * theta if theta_t is None else theta_t
+            for kind in ("high", "low"):
+                phi1, st1 = O.bct_matvec_kernel(v, e, EPS, psi1, kind, theta, leaf, theta_t=tt)
+                phi3, _ = O.bct_matvec_kernel(v, e, EPS, psi3, kind, theta, leaf, theta_t=tt)
+                bu, _ = O.bct_apply_B_scalar(v, e, EPS, u, kind, theta, leaf, theta_t=tt)
+                bct.append({"shape": name, "theta": theta, "theta_t": tt, "leaf_size": leaf,
+                            "kind": kind, "psi1": psi1.tolist(), "psi3": psi3.tolist(),
+                            "u": u.tolist(), "phi1": phi1.tolist(), "phi3": phi3.tolist(),
+                            "Bu": bu.tolist(), "stats": {k: int(x) for k, x in st1.items()}})
     out = {"alpha": ALPHA, "beta": BETA, "epsilon": EPS,
            "shapes": {k: {"vertices": v.tolist(), "edges": e.tolist()}
                       for k, (v, e) in shapes.items()},
-           "cases": cases}
+           "cases": cases, "bct": bct}
     with open(os.path.join(HERE, "bh_goldens.json"), "w") as f:
         json.dump(out, f, indent=1)
-    print(f"wrote {len(cases)} cases")
+    print(f"wrote {len(cases)} energy cases, {len(bct)} matvec cases")
 
 
 if __name__ == "__main__":

@@ -251,3 +251,26 @@ def test_bh_energy_oracle_vs_reference_vectors():
             assert abs(en - rco.energy(sh["vertices"], sh["edges"])) <= 1e-12 * abs(en)
         clustered += st[1] > 0
     assert clustered >= 10
+
+
+def test_bct_products_oracle_vs_reference_vectors():
+    # block-cluster-tree products (tpe_stage2_oracle.py:598-795): K psi for a
+    # scalar and a 3-vector density and the Gram apply B u, both kernels, four
+    # admissibility settings on five curves: 1e-12 (Frobenius-relative) and the
+    # block counts exactly
+    import json
+    import os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "bh_goldens.json")))
+    admitted = 0
+    for c in G["bct"]:
+        sh = G["shapes"][c["shape"]]
+        args = (c["kind"], c["theta"], c["theta_t"], c["leaf_size"])
+        p1, st = rco.bct_matvec(sh["vertices"], sh["edges"], c["psi1"], *args)
+        p3, _ = rco.bct_matvec(sh["vertices"], sh["edges"], c["psi3"], *args)
+        bu = rco.bct_apply_b(sh["vertices"], sh["edges"], c["u"], *args)
+        assert frob_rel(p1, c["phi1"]) <= 1e-12 and frob_rel(p3, c["phi3"]) <= 1e-12, c["shape"]
+        assert frob_rel(bu, c["Bu"]) <= 1e-12
+        s = c["stats"]
+        assert st == (s["leaves"], s["admissible_blocks"], s["direct_blocks"], s["direct_pairs"])
+        admitted += st[1] > 0
+    assert admitted >= 10
