@@ -156,6 +156,18 @@ rc_status rc_step(rc_ctx* ctx, const double* xyz_in, const rc_params* p, int32_t
  * references, direct pairs. The tree is built on the host from xyz. */
 rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_params* p, double theta,
                        double theta_t, int32_t leaf_size, double* energy, int64_t* stats);
+/* Block-cluster-tree products of the fractional-Sobolev edge kernels: the
+ * reference's stage-2 oracle bct_matvec_kernel (oracle/tpe_stage2_oracle.py:
+ * 649-709): phi = K psi with psi, phi double[E][ncomp] (ncomp 1 or 3), kind
+ * 0 'high' / 1 'low'; stats (nullable, 4): leaves, admissible blocks, direct
+ * blocks, direct pairs. rc_bct_apply_b is bct_apply_B_scalar (:763-795):
+ * out[V] = B u through two products. The tree is built on the host from xyz. */
+rc_status rc_bct_matvec(rc_ctx* ctx, const double* xyz, double eps, const double* psi,
+                        int32_t ncomp, int32_t kind, double theta, double theta_t,
+                        int32_t leaf_size, double* phi, int64_t* stats);
+rc_status rc_bct_apply_b(rc_ctx* ctx, const double* xyz, double eps, const double* u,
+                         int32_t kind, double theta, double theta_t, int32_t leaf_size,
+                         double* out);
 /* assembleAFlat (innerProduct.ts:325-350): V x V row-major A = B + B0. */
 rc_status rc_assemble_a(rc_ctx* ctx, const double* xyz, const rc_params* p, double* A);
 

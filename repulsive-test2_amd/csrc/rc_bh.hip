@@ -29,7 +29,7 @@ constexpr int BH_STACK = 64;   // > depth + 1 of a median-split tree over 2^31 e
 struct BhTree {
     std::vector<int32_t> ord, pos;               // node order and its inverse
     std::vector<int32_t> start, end, left, right;
-    std::vector<double> cx, cy, cz, mass, rx, rT;
+    std::vector<double> cx, cy, cz, tx, ty, tz, mass, rx, rT;
 };
 
 struct BhBuild {
@@ -87,6 +87,9 @@ struct BhBuild {
         t.cx.push_back(c[0]);
         t.cy.push_back(c[1]);
         t.cz.push_back(c[2]);
+        t.tx.push_back(tg[0]);
+        t.ty.push_back(tg[1]);
+        t.tz.push_back(tg[2]);
         t.mass.push_back(ms);
         t.rx.push_back(rx);
         t.rT.push_back(rT);
@@ -357,5 +360,443 @@ extern "C" rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_param
     *energy = ctx->d_host_stage[0];
     if (stats)
         for (int q = 0; q < 3; ++q) stats[q] = (int64_t)hst[q];
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Block-cluster-tree products of the edge kernels (bct_matvec_kernel,
+// bct_kernel_node, exact_edge_kernel_pair, bct_apply_B_scalar:
+// oracle/tpe_stage2_oracle.py:598-795). The host runs the dual recursion from
+// (root, root) over the same tree and records the admissible blocks (rank-one
+// terms with the node kernel of the centres and mean tangents, evaluated on
+// the host) by target node and the direct leaf-leaf blocks by target leaf.
+// The device forms the source moments sum_J mass_J psi_J per node, then each
+// target edge I adds mass_I * sum kval * moment over the admissible blocks of
+// the nodes on its leaf-to-root path and the exact pair kernels of its direct
+// blocks. Kernels: 'high' k_IJ = 2 * 1/4 ell_I ell_J sum_ij d^-p, 'low'
+// 1/4 ell_I ell_J sum_ij [(|T_I x d| + eps)^2 + (|T_J x -d| + eps)^2] d^-(4+p),
+// d = |p_i - q_j| + eps, p = 7/3, disjoint pairs only.
+
+namespace {
+
+struct BctDev {
+    const int32_t *start, *end, *parent, *leaf_of, *ord;
+    const int32_t *adm_off, *adm_src, *dir_off, *dir_src;
+    const double* kval;
+};
+
+__global__ void k_bct_moments(EdgeSoA ed, BctDev t, int nn, const double* __restrict__ psi,
+                              int nc, double* __restrict__ mom) {
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= nn) return;
+    double s[3] = {0, 0, 0};
+    for (int p = t.start[nd]; p < t.end[nd]; ++p) {
+        const int J = t.ord[p];
+        const double mj = ed.ell[J];
+        for (int c = 0; c < nc; ++c) s[c] += mj * psi[(size_t)nc * J + c];
+    }
+    for (int c = 0; c < nc; ++c) mom[(size_t)nc * nd + c] = s[c];
+}
+
+__global__ __launch_bounds__(256) void k_bct_apply(EdgeSoA ed, int E, BctDev t, int kind,
+                                                   double eps, const double* __restrict__ psi,
+                                                   int nc, const double* __restrict__ mom,
+                                                   double* __restrict__ phi) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= E) return;
+    double far[3] = {0, 0, 0}, near[3] = {0, 0, 0};
+    for (int nd = t.leaf_of[I]; nd >= 0; nd = t.parent[nd])
+        for (int q = t.adm_off[nd]; q < t.adm_off[nd + 1]; ++q) {
+            const double kv = t.kval[q];
+            const int sn = t.adm_src[q];
+            for (int c = 0; c < nc; ++c) far[c] += kv * mom[(size_t)nc * sn + c];
+        }
+    const int i1 = ed.i1[I], i2 = ed.i2[I];
+    const double PI[6] = {ed.ax[I], ed.ay[I], ed.az[I], ed.bx[I], ed.by[I], ed.bz[I]};
+    const double TI[3] = {ed.ux[I], ed.uy[I], ed.uz[I]};
+    const double ellI = ed.ell[I];
+    const int leaf = t.leaf_of[I];
+    for (int q = t.dir_off[leaf]; q < t.dir_off[leaf + 1]; ++q) {
+        const int sn = t.dir_src[q];
+        for (int p = t.start[sn]; p < t.end[sn]; ++p) {
+            const int J = t.ord[p];
+            const int j1 = ed.i1[J], j2 = ed.i2[J];
+            if (i1 == j1 || i1 == j2 || i2 == j1 || i2 == j2) continue;
+            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
+            const double TJ[3] = {ed.ux[J], ed.uy[J], ed.uz[J]};
+            double sh = 0.0, sl = 0.0;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const double dx = PI[3 * a] - PJ[3 * b], dy = PI[3 * a + 1] - PJ[3 * b + 1],
+                                 dz = PI[3 * a + 2] - PJ[3 * b + 2];
+                    const double dn = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+                    const double y = rcm::rcbrt(dn);
+                    const double y2 = y * y, y4 = y2 * y2;
+                    const double dp = y4 * y2 * y;   // dn^(-7/3)
+                    if (kind == 0) {
+                        sh += dp;
+                    } else {
+                        const double c1x = TI[1] * dz - TI[2] * dy, c1y = TI[2] * dx - TI[0] * dz,
+                                     c1z = TI[0] * dy - TI[1] * dx;
+                        const double c2x = -(TJ[1] * dz - TJ[2] * dy),
+                                     c2y = -(TJ[2] * dx - TJ[0] * dz),
+                                     c2z = -(TJ[0] * dy - TJ[1] * dx);
+                        const double n1 = rcm::sqrt_(c1x * c1x + c1y * c1y + c1z * c1z) + eps;
+                        const double n2 = rcm::sqrt_(c2x * c2x + c2y * c2y + c2z * c2z) + eps;
+                        const double r = rcm::rcp(dn), r2 = r * r;
+                        sl += (n1 * n1 + n2 * n2) * (r2 * r2) * dp;
+                    }
+                }
+            const double k = kind == 0 ? 2.0 * 0.25 * ellI * ed.ell[J] * sh
+                                       : 0.25 * ellI * ed.ell[J] * sl;
+            for (int c = 0; c < nc; ++c) near[c] += k * psi[(size_t)nc * J + c];
+        }
+    }
+    for (int c = 0; c < nc; ++c) phi[(size_t)nc * I + c] = ellI * far[c] + near[c];
+}
+
+// y of bct_apply_B_scalar: 'high' ((u_b - u_a) / ell) T per edge (3 comps),
+// 'low' (u_a + u_b) / 2; ones for K 1.
+__global__ void k_bct_density(EdgeSoA ed, int E, int kind, const double* __restrict__ u,
+                              double* __restrict__ y, double* __restrict__ ones) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= E) return;
+    const double ua = u[ed.i1[I]], ub = u[ed.i2[I]];
+    if (kind == 0) {
+        const double du = (ub - ua) / ed.ell[I];
+        y[3 * (size_t)I] = du * ed.ux[I];
+        y[3 * (size_t)I + 1] = du * ed.uy[I];
+        y[3 * (size_t)I + 2] = du * ed.uz[I];
+    } else {
+        y[I] = 0.5 * (ua + ub);
+    }
+    ones[I] = 1.0;
+}
+
+// out[v] from z = K1 y - Ky over the incident edges in edge order ('low':
+// +z/2 at both ends; 'high': apply_Dt, coeff = (z . T) / ell, -coeff at i1,
+// +coeff at i2).
+__global__ void k_bct_gather(EdgeSoA ed, int V, int kind, const int32_t* __restrict__ vOff,
+                             const int32_t* __restrict__ vEdge, const int32_t* __restrict__ vSlot,
+                             const double* __restrict__ y, const double* __restrict__ K1,
+                             const double* __restrict__ Ky, double* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double s = 0.0;
+    for (int q = vOff[v]; q < vOff[v + 1]; ++q) {
+        const int I = vEdge[q];
+        if (kind == 0) {
+            const double T[3] = {ed.ux[I], ed.uy[I], ed.uz[I]};
+            double dot = 0.0;
+            for (int d = 0; d < 3; ++d) dot += (K1[I] * y[3 * (size_t)I + d] - Ky[3 * (size_t)I + d]) * T[d];
+            const double coeff = dot / ed.ell[I];
+            s += vSlot[q] ? coeff : -coeff;
+        } else {
+            s += 0.5 * (K1[I] * y[I] - Ky[I]);
+        }
+    }
+    out[v] = s;
+}
+
+// Host: tree over xyz, the dual recursion's block lists and the counts.
+struct BctHost {
+    BhTree tree;
+    std::vector<int32_t> parent, leaf_of, adm_off, adm_src, dir_off, dir_src;
+    std::vector<double> kval;
+    int64_t leaves = 0, adm = 0, dir = 0, pairs = 0;
+};
+
+static void bct_build_host(const rc_ctx* ctx, const double* xyz, double eps, double theta,
+                           double theta_t, int leaf, int kind, BctHost& H) {
+    const int E = ctx->E, V = ctx->V;
+    std::vector<double> C(3 * (size_t)E), T(3 * (size_t)E), m(E);
+    for (int K = 0; K < E; ++K) {
+        const int a = ctx->h_edges[2 * K], b = ctx->h_edges[2 * K + 1];
+        double e[3], r2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            e[d] = xyz[3 * b + d] - xyz[3 * a + d];
+            r2 += e[d] * e[d];
+            C[3 * K + d] = 0.5 * (xyz[3 * a + d] + xyz[3 * b + d]);
+        }
+        const double r = std::sqrt(r2);
+        m[K] = r + eps;
+        for (int d = 0; d < 3; ++d) T[3 * K + d] = (r >= 1e-14) ? e[d] / r : 0.0;
+    }
+    BhTree& t = H.tree;
+    t.ord.resize(E);
+    for (int K = 0; K < E; ++K) t.ord[K] = K;
+    BhBuild bld{C, T, m, std::vector<double>(E), leaf, t};
+    if (E > 0) bld.build(0, E);
+    t.pos.resize(E);
+    for (int q = 0; q < E; ++q) t.pos[t.ord[q]] = q;
+    const int nn = (int)t.start.size();
+    H.parent.assign(nn, -1);
+    for (int i = 0; i < nn; ++i)
+        if (t.left[i] >= 0) {
+            H.parent[t.left[i]] = i;
+            H.parent[t.right[i]] = i;
+        }
+    H.leaf_of.assign(E, -1);
+    for (int i = 0; i < nn; ++i)
+        if (t.left[i] < 0)
+            for (int p = t.start[i]; p < t.end[i]; ++p) H.leaf_of[t.ord[p]] = i;
+    // vertex -> incident edges
+    std::vector<int32_t> voff(V + 1, 0), vedge(2 * (size_t)E);
+    for (int K = 0; K < E; ++K) {
+        voff[ctx->h_edges[2 * K] + 1]++;
+        voff[ctx->h_edges[2 * K + 1] + 1]++;
+    }
+    for (int v = 0; v < V; ++v) voff[v + 1] += voff[v];
+    std::vector<int32_t> fill(voff.begin(), voff.end() - 1);
+    for (int K = 0; K < E; ++K) {
+        vedge[fill[ctx->h_edges[2 * K]]++] = K;
+        vedge[fill[ctx->h_edges[2 * K + 1]]++] = K;
+    }
+    auto disjoint_blocks = [&](int ti, int si) {
+        for (int p = t.start[ti]; p < t.end[ti]; ++p) {
+            const int I = t.ord[p];
+            for (int w = 0; w < 2; ++w) {
+                const int v = ctx->h_edges[2 * I + w];
+                for (int q = voff[v]; q < voff[v + 1]; ++q) {
+                    const int pj = t.pos[vedge[q]];
+                    if (pj >= t.start[si] && pj < t.end[si]) return false;
+                }
+            }
+        }
+        return true;
+    };
+    auto node_kernel = [&](int ti, int si) {
+        const double d[3] = {t.cx[ti] - t.cx[si], t.cy[ti] - t.cy[si], t.cz[ti] - t.cz[si]};
+        const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + eps;
+        if (kind == 0) return 2.0 / std::pow(dn, 7.0 / 3.0);
+        const double tT[3] = {t.tx[ti], t.ty[ti], t.tz[ti]}, tS[3] = {t.tx[si], t.ty[si], t.tz[si]};
+        const double c1x = tT[1] * d[2] - tT[2] * d[1], c1y = tT[2] * d[0] - tT[0] * d[2],
+                     c1z = tT[0] * d[1] - tT[1] * d[0];
+        const double c2x = -(tS[1] * d[2] - tS[2] * d[1]), c2y = -(tS[2] * d[0] - tS[0] * d[2]),
+                     c2z = -(tS[0] * d[1] - tS[1] * d[0]);
+        const double c1 = std::sqrt(c1x * c1x + c1y * c1y + c1z * c1z) + eps;
+        const double c2 = std::sqrt(c2x * c2x + c2y * c2y + c2z * c2z) + eps;
+        return (std::pow(c1, 2.0) + std::pow(c2, 2.0)) / std::pow(dn, 4.0 + 7.0 / 3.0);
+    };
+    std::vector<std::pair<int, int>> adm, dir;   // (target, source)
+    std::vector<double> kv;
+    std::vector<std::pair<int, int>> stack;
+    if (nn > 0) stack.push_back({0, 0});
+    while (!stack.empty()) {   // same visiting order as the recursion (left first)
+        const auto [ti, si] = stack.back();
+        stack.pop_back();
+        const int nt = t.end[ti] - t.start[ti], ns = t.end[si] - t.start[si];
+        const double dx = t.cx[ti] - t.cx[si], dy = t.cy[ti] - t.cy[si], dz = t.cz[ti] - t.cz[si];
+        const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const bool well = nt + ns >= 3 && dist > 0.0 &&
+                          std::max(t.rx[ti], t.rx[si]) / dist <= theta &&
+                          std::max(t.rT[ti], t.rT[si]) <= theta_t && disjoint_blocks(ti, si);
+        if (well) {
+            ++H.leaves;
+            ++H.adm;
+            adm.push_back({ti, si});
+            kv.push_back(node_kernel(ti, si));
+            continue;
+        }
+        if (t.left[ti] < 0 && t.left[si] < 0) {
+            ++H.leaves;
+            ++H.dir;
+            dir.push_back({ti, si});
+            for (int p = t.start[ti]; p < t.end[ti]; ++p) {
+                const int I = t.ord[p];
+                const int a1 = ctx->h_edges[2 * I], a2 = ctx->h_edges[2 * I + 1];
+                for (int q = t.start[si]; q < t.end[si]; ++q) {
+                    const int J = t.ord[q];
+                    const int b1 = ctx->h_edges[2 * J], b2 = ctx->h_edges[2 * J + 1];
+                    if (a1 != b1 && a1 != b2 && a2 != b1 && a2 != b2) ++H.pairs;
+                }
+            }
+            continue;
+        }
+        const bool split_t = t.left[ti] >= 0 && (t.left[si] < 0 || nt >= ns);
+        if (split_t) {
+            stack.push_back({t.right[ti], si});
+            stack.push_back({t.left[ti], si});
+        } else {
+            stack.push_back({ti, t.right[si]});
+            stack.push_back({ti, t.left[si]});
+        }
+    }
+    // CSR by target node (stable: recursion order within a node)
+    H.adm_off.assign(nn + 1, 0);
+    for (const auto& b : adm) H.adm_off[b.first + 1]++;
+    for (int i = 0; i < nn; ++i) H.adm_off[i + 1] += H.adm_off[i];
+    H.adm_src.resize(adm.size());
+    H.kval.resize(adm.size());
+    {
+        std::vector<int32_t> f(H.adm_off.begin(), H.adm_off.end() - 1);
+        for (size_t q = 0; q < adm.size(); ++q) {
+            const int at = f[adm[q].first]++;
+            H.adm_src[at] = adm[q].second;
+            H.kval[at] = kv[q];
+        }
+    }
+    H.dir_off.assign(nn + 1, 0);
+    for (const auto& b : dir) H.dir_off[b.first + 1]++;
+    for (int i = 0; i < nn; ++i) H.dir_off[i + 1] += H.dir_off[i];
+    H.dir_src.resize(dir.size());
+    {
+        std::vector<int32_t> f(H.dir_off.begin(), H.dir_off.end() - 1);
+        for (const auto& b : dir) H.dir_src[f[b.first]++] = b.second;
+    }
+}
+
+// Uploads the host lists into the context's growable buffers; returns the
+// device view and the double scratch after the kernel values.
+static rc_status bct_upload(rc_ctx* ctx, const BctHost& H, size_t extra_d, BctDev& t,
+                            double** scratch) {
+    const int nn = (int)H.tree.start.size(), E = ctx->E;
+    const size_t na = H.adm_src.size(), ndr = H.dir_src.size();
+    const size_t ni = 3 * (size_t)nn + 2 * (size_t)E + 2 * ((size_t)nn + 1) + na + ndr;
+    const size_t nd = na + extra_d;
+    if (ni > ctx->bct_cap_i || nd > ctx->bct_cap_d) {
+        if (ctx->d_bct_i) (void)hipFree(ctx->d_bct_i);
+        if (ctx->d_bct_f) (void)hipFree(ctx->d_bct_f);
+        ctx->d_bct_i = nullptr;
+        ctx->d_bct_f = nullptr;
+        ctx->bct_cap_i = ctx->bct_cap_d = 0;
+        RC_HIP(hipMalloc((void**)&ctx->d_bct_i, sizeof(int32_t) * std::max<size_t>(ni, 1)));
+        RC_HIP(hipMalloc((void**)&ctx->d_bct_f, sizeof(double) * std::max<size_t>(nd, 1)));
+        ctx->bct_cap_i = ni;
+        ctx->bct_cap_d = nd;
+    }
+    std::vector<int32_t> hi;
+    hi.reserve(ni);
+    auto put = [&](const std::vector<int32_t>& v) { hi.insert(hi.end(), v.begin(), v.end()); };
+    put(H.tree.start);
+    put(H.tree.end);
+    put(H.parent);
+    put(H.leaf_of);
+    put(H.tree.ord);
+    put(H.adm_off);
+    put(H.adm_src);
+    put(H.dir_off);
+    put(H.dir_src);
+    RC_HIP(hipMemcpy(ctx->d_bct_i, hi.data(), sizeof(int32_t) * hi.size(), hipMemcpyHostToDevice));
+    if (na) RC_HIP(hipMemcpy(ctx->d_bct_f, H.kval.data(), sizeof(double) * na, hipMemcpyHostToDevice));
+    const int32_t* g = ctx->d_bct_i;
+    t.start = g; g += nn;
+    t.end = g; g += nn;
+    t.parent = g; g += nn;
+    t.leaf_of = g; g += E;
+    t.ord = g; g += E;
+    t.adm_off = g; g += nn + 1;
+    t.adm_src = g; g += na;
+    t.dir_off = g; g += nn + 1;
+    t.dir_src = g;
+    t.kval = ctx->d_bct_f;
+    *scratch = ctx->d_bct_f + na;
+    return RC_OK;
+}
+
+static rc_status bct_check(rc_ctx* ctx, double eps, double theta, double theta_t, int leaf,
+                           int kind) {
+    if (!std::isfinite(eps) || !(theta >= 0.0) || !std::isfinite(theta) || !(theta_t >= 0.0) ||
+        !std::isfinite(theta_t) || leaf < 1 || (kind != 0 && kind != 1)) {
+        ctx->err = "block-cluster-tree parameters out of range";
+        return RC_BAD_ARG;
+    }
+    return RC_OK;
+}
+
+static rc_status bct_geometry(rc_ctx* ctx, const double* xyz, double eps) {
+    rc_params p{3.0, 6.0, eps};
+    rc_status s = rc_upload_positions_aos(ctx, xyz, ctx->d_pos);
+    if (s == RC_OK) s = rc_launch_edge_prep(ctx, ctx->d_pos, &p);
+    return s;
+}
+
+}  // namespace
+
+extern "C" rc_status rc_bct_matvec(rc_ctx* ctx, const double* xyz, double eps, const double* psi,
+                                   int32_t ncomp, int32_t kind, double theta, double theta_t,
+                                   int32_t leaf_size, double* phi, int64_t* stats) {
+    if (!ctx || !xyz || !psi || !phi || (ncomp != 1 && ncomp != 3)) return RC_BAD_ARG;
+    rc_status s = bct_check(ctx, eps, theta, theta_t, leaf_size, kind);
+    if (s != RC_OK) return s;
+    const int E = ctx->E;
+    BctHost H;
+    bct_build_host(ctx, xyz, eps, theta, theta_t, leaf_size, kind, H);
+    const int nn = (int)H.tree.start.size();
+    BctDev t;
+    double* scr = nullptr;
+    const size_t nin = (size_t)ncomp * E, nmom = 3 * (size_t)nn, nout = (size_t)ncomp * E;
+    s = bct_upload(ctx, H, nin + nmom + nout, t, &scr);
+    if (s == RC_OK) s = bct_geometry(ctx, xyz, eps);
+    if (s != RC_OK) return s;
+    double *d_psi = scr, *d_mom = scr + nin, *d_phi = scr + nin + nmom;
+    RC_HIP(hipMemcpyAsync(d_psi, psi, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_bct_moments, dim3((nn + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
+                       t, nn, (const double*)d_psi, ncomp, d_mom);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bct_apply, dim3((E + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed, E,
+                       t, kind, eps, (const double*)d_psi, ncomp, (const double*)d_mom, d_phi);
+    RC_CHECK_LAUNCH();
+    RC_HIP(hipMemcpyAsync(phi, d_phi, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats[0] = H.leaves;
+        stats[1] = H.adm;
+        stats[2] = H.dir;
+        stats[3] = H.pairs;
+    }
+    return RC_OK;
+}
+
+extern "C" rc_status rc_bct_apply_b(rc_ctx* ctx, const double* xyz, double eps, const double* u,
+                                    int32_t kind, double theta, double theta_t,
+                                    int32_t leaf_size, double* out) {
+    if (!ctx || !xyz || !u || !out) return RC_BAD_ARG;
+    rc_status s = bct_check(ctx, eps, theta, theta_t, leaf_size, kind);
+    if (s != RC_OK) return s;
+    const int E = ctx->E, V = ctx->V;
+    BctHost H;
+    bct_build_host(ctx, xyz, eps, theta, theta_t, leaf_size, kind, H);
+    const int nn = (int)H.tree.start.size();
+    const int nc = kind == 0 ? 3 : 1;
+    BctDev t;
+    double* scr = nullptr;
+    // u, y, ones, moments (3 per node), K1, Ky, out
+    const size_t need = (size_t)V + 3 * (size_t)E + E + 3 * (size_t)nn + E + 3 * (size_t)E + V;
+    s = bct_upload(ctx, H, need, t, &scr);
+    if (s == RC_OK) s = bct_geometry(ctx, xyz, eps);
+    if (s != RC_OK) return s;
+    double* d_u = scr;
+    double* d_y = d_u + V;
+    double* d_one = d_y + 3 * (size_t)E;
+    double* d_mom = d_one + E;
+    double* d_K1 = d_mom + 3 * (size_t)nn;
+    double* d_Ky = d_K1 + E;
+    double* d_out = d_Ky + 3 * (size_t)E;
+    RC_HIP(hipMemcpyAsync(d_u, u, sizeof(double) * V, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_bct_density, dim3((E + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
+                       E, (int)kind, (const double*)d_u, d_y, d_one);
+    RC_CHECK_LAUNCH();
+    const dim3 gn((nn + 255) / 256), ge((E + 255) / 256);
+    hipLaunchKernelGGL(k_bct_moments, gn, dim3(256), 0, ctx->stream, ctx->ed, t, nn,
+                       (const double*)d_one, 1, d_mom);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bct_apply, ge, dim3(256), 0, ctx->stream, ctx->ed, E, t, (int)kind, eps,
+                       (const double*)d_one, 1, (const double*)d_mom, d_K1);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bct_moments, gn, dim3(256), 0, ctx->stream, ctx->ed, t, nn,
+                       (const double*)d_y, nc, d_mom);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bct_apply, ge, dim3(256), 0, ctx->stream, ctx->ed, E, t, (int)kind, eps,
+                       (const double*)d_y, nc, (const double*)d_mom, d_Ky);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bct_gather, dim3((V + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed, V,
+                       (int)kind, ctx->d_vOff, ctx->d_vEdge, ctx->d_vSlot, (const double*)d_y,
+                       (const double*)d_K1, (const double*)d_Ky, d_out);
+    RC_CHECK_LAUNCH();
+    RC_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * V, hipMemcpyDeviceToHost, ctx->stream));
+    RC_HIP(hipStreamSynchronize(ctx->stream));
     return RC_OK;
 }

@@ -136,6 +136,10 @@ struct rc_ctx {
     int32_t* d_bh_i = nullptr;      // 4 ints per node + node order + positions
     unsigned long long* d_bh_stats = nullptr;
     size_t bh_cap_d = 0, bh_cap_i = 0;
+    // block-cluster-tree products (rc_bct_matvec, rc_bct_apply_b), grown on demand
+    double* d_bct_f = nullptr;
+    int32_t* d_bct_i = nullptr;
+    size_t bct_cap_d = 0, bct_cap_i = 0;
 
     // kernel-level profile (rc_kernel_profile_enable / _read)
     int kprof = 0;

@@ -131,6 +131,45 @@ def bh_energy(vertices, edges, alpha, beta, epsilon, theta=0.5, leaf_size=8, the
     return out.value, {"visits": st[0], "approximated_edge_refs": st[1], "direct_pairs": st[2]}
 
 
+_KINDS = {"high": 0, "low": 1}
+
+
+def bct_matvec_kernel(vertices, edges, epsilon, psi, kind, theta=0.5, leaf_size=8, theta_t=None):
+    """Block-cluster-tree product K psi of the 'high' or 'low' edge kernel
+    (stage-2 oracle bct_matvec_kernel, oracle/tpe_stage2_oracle.py:649-709):
+    (phi, {leaves, admissible_blocks, direct_blocks, direct_pairs})."""
+    if kind not in _KINDS:
+        raise L.RcError(L.RC_BAD_ARG, f"unknown kernel kind {kind!r}")
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    psi = np.ascontiguousarray(np.asarray(psi, dtype=np.float64))
+    nc = 1 if psi.ndim == 1 else psi.shape[1]
+    phi = np.zeros_like(psi)
+    st = (C.c_int64 * 4)()
+    tt = 1.5 * theta if theta_t is None else theta_t
+    ctx.check(L.lib().rc_bct_matvec(ctx.handle, L.dptr(v), float(epsilon), L.dptr(psi), nc,
+                                    _KINDS[kind], float(theta), float(tt), int(leaf_size),
+                                    L.dptr(phi), st))
+    return phi, {"leaves": st[0], "admissible_blocks": st[1], "direct_blocks": st[2],
+                 "direct_pairs": st[3]}
+
+
+def bct_apply_b_scalar(vertices, edges, epsilon, u, kind, theta=0.5, leaf_size=8, theta_t=None):
+    """B u through the block cluster tree (stage-2 oracle bct_apply_B_scalar,
+    oracle/tpe_stage2_oracle.py:763-795)."""
+    if kind not in _KINDS:
+        raise L.RcError(L.RC_BAD_ARG, f"unknown kernel kind {kind!r}")
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    out = np.zeros(len(v))
+    tt = 1.5 * theta if theta_t is None else theta_t
+    ctx.check(L.lib().rc_bct_apply_b(ctx.handle, L.dptr(v), float(epsilon), L.dptr(u),
+                                     _KINDS[kind], float(theta), float(tt), int(leaf_size),
+                                     L.dptr(out)))
+    return out
+
+
 _MODES = {"analytical": L.MODE_ANALYTICAL, "finiteDiff": L.MODE_FINITE_DIFF}
 
 

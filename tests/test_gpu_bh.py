@@ -3,8 +3,10 @@ reference's stage-2 oracle vectors (tests/golden/bh_goldens.json, made by
 tests/golden/make_bh_goldens.py) and the oracle's C restatement: energy to
 1e-12 and the traversal counts (visits, approximated edge references, direct
 pairs) exactly; theta = 0 reproduces the exact energy; the reference's gates
-(oracle/check_bh_stage2.py): clustering fires and the error stays within
-5e-2 at (theta, theta_t) = (0.5, 0.75) and shrinks at (0.25, 0.5)."""
+(oracle/check_bh_stage2.py): clustering fires and the energy error stays
+within 5e-2 at (theta, theta_t) = (0.5, 0.75) and shrinks at (0.25, 0.5).
+Block-cluster-tree products of the edge kernels and the Gram apply likewise
+against the reference vectors and the oracle (block counts exact)."""
 import json
 import os
 import sys
@@ -60,3 +62,57 @@ def test_bh_rejects_bad_parameters():
     for theta, tt, leaf in ((-1.0, 0.5, 4), (float("nan"), 0.5, 4), (0.5, 0.75, 0)):
         with pytest.raises(core.L.RcError):
             core.bh_energy(v, e, 3.0, 6.0, 1e-10, theta, leaf, tt)
+
+
+def frel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_bct_products_vs_reference_vectors():
+    G = json.load(open(os.path.join(HERE, "golden", "bh_goldens.json")))
+    for c in G["bct"]:
+        sh = G["shapes"][c["shape"]]
+        log("bct", c["shape"], c["kind"], c["theta"], c["leaf_size"])
+        args = dict(theta=c["theta"], leaf_size=c["leaf_size"], theta_t=c["theta_t"])
+        p1, st = core.bct_matvec_kernel(sh["vertices"], sh["edges"], 1e-10, c["psi1"], c["kind"],
+                                        **args)
+        p3, _ = core.bct_matvec_kernel(sh["vertices"], sh["edges"], 1e-10, c["psi3"], c["kind"],
+                                       **args)
+        bu = core.bct_apply_b_scalar(sh["vertices"], sh["edges"], 1e-10, c["u"], c["kind"], **args)
+        assert frel(p1, c["phi1"]) <= 1e-12 and frel(p3, c["phi3"]) <= 1e-12, c["shape"]
+        assert frel(bu, c["Bu"]) <= 1e-12
+        s = c["stats"]
+        assert (st["leaves"], st["admissible_blocks"], st["direct_blocks"], st["direct_pairs"]) == (
+            s["leaves"], s["admissible_blocks"], s["direct_blocks"], s["direct_pairs"])
+
+
+@pytest.mark.parametrize("kind", ["high", "low"])
+def test_bct_products_vs_oracle_at_size(kind):
+    v, e = curves.fourier_curve(2048, seed=2)
+    m = len(e)
+    psi = np.sin(0.3 * np.arange(m)) + 0.5
+    u = np.cos(0.05 * np.arange(len(v)))
+    exact, _ = rco.bct_matvec(v, e, psi, kind, 0.0, 0.0, 8)
+    errs = {}
+    for theta, tt, leaf in ((0.5, 0.75, 4), (0.25, 0.5, 8), (0.0, 0.0, 8)):
+        log("bct size", kind, theta, leaf)
+        t0 = time.perf_counter()
+        ph, st = core.bct_matvec_kernel(v, e, 1e-10, psi, kind, theta, leaf, tt)
+        log(f"  {1e3 * (time.perf_counter() - t0):.1f} ms (block lists on the host + products)",
+            st)
+        po, so = rco.bct_matvec(v, e, psi, kind, theta, tt, leaf)
+        assert frel(ph, po) <= 1e-12
+        assert (st["leaves"], st["admissible_blocks"], st["direct_blocks"],
+                st["direct_pairs"]) == so
+        err = frel(ph, exact)
+        errs[theta] = err
+        if theta == 0.0:
+            assert err <= 1e-12 and st["admissible_blocks"] == 0
+        else:
+            # the reference gates only that blocks are admitted (check_bh_stage2.py)
+            assert st["admissible_blocks"] > 0 and 0.0 < err < 1.0
+        bu = core.bct_apply_b_scalar(v, e, 1e-10, u, kind, theta, leaf, tt)
+        assert frel(bu, rco.bct_apply_b(v, e, u, kind, theta, tt, leaf)) <= 1e-12
+    log("  relative errors against the exact product", errs)
+    assert errs[0.25] < errs[0.5]
