@@ -156,6 +156,11 @@ rc_status rc_step(rc_ctx* ctx, const double* xyz_in, const rc_params* p, int32_t
  * references, direct pairs. The tree is built on the host from xyz. */
 rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_params* p, double theta,
                        double theta_t, int32_t leaf_size, double* energy, int64_t* stats);
+/* Barnes-Hut differential as the stage-2 oracle forms it: the forward
+ * difference (E_bh(x + h e_vd) - E_bh(x)) / h, a fresh tree per moved
+ * coordinate (tpe_stage2_oracle.py:133-141, 1289-1290). AoS out. */
+rc_status rc_gradient_bh(rc_ctx* ctx, const double* xyz, const rc_params* p, double theta,
+                         double theta_t, int32_t leaf_size, double h, double* grad);
 /* Block-cluster-tree products of the fractional-Sobolev edge kernels: the
  * reference's stage-2 oracle bct_matvec_kernel (oracle/tpe_stage2_oracle.py:
  * 649-709): phi = K psi with psi, phi double[E][ncomp] (ncomp 1 or 3), kind

@@ -2004,3 +2004,23 @@ void rco_bct_apply_b(int V, int E, const double* v, const int32_t* edges, double
     free(Ky);
     bct_release(&b, C, T, mass);
 }
+
+/* Forward difference of the Barnes-Hut energy (the reference's stage-2
+ * oracle: finite_difference_gradient of bh_energy, tpe_stage2_oracle.py:
+ * 133-141, 1289-1290): the tree is rebuilt for every moved coordinate. */
+void rco_gradient_bh_fd(int V, int E, const double* v, const int32_t* edges, double alpha,
+                        double beta, double eps, double theta, double theta_t, int leaf_size,
+                        double h, double* g) {
+    const double e0 = rco_energy_bh(V, E, v, edges, alpha, beta, eps, theta, theta_t, leaf_size,
+                                    NULL);
+    double* pert = (double*)malloc(sizeof(double) * 3 * (size_t)(V > 0 ? V : 1));
+    for (int i = 0; i < V; i++)
+        for (int d = 0; d < 3; d++) {
+            memcpy(pert, v, sizeof(double) * 3 * (size_t)V);
+            pert[3 * i + d] += h;
+            const double e1 = rco_energy_bh(V, E, pert, edges, alpha, beta, eps, theta, theta_t,
+                                            leaf_size, NULL);
+            g[3 * i + d] = (e1 - e0) / h;
+        }
+    free(pert);
+}

@@ -109,6 +109,10 @@ void rco_step(int V, int E, const double* pos, const int32_t* edges, double alph
 double rco_energy_bh(int V, int E, const double* pos, const int32_t* edges, double alpha,
                      double beta, double eps, double theta, double theta_t, int leaf_size,
                      int64_t* stats);
+/* Forward difference of the Barnes-Hut energy, tree rebuilt per move. */
+void rco_gradient_bh_fd(int V, int E, const double* pos, const int32_t* edges, double alpha,
+                        double beta, double eps, double theta, double theta_t, int leaf_size,
+                        double h, double* grad);
 /* Block-cluster-tree products (the reference's stage-2 oracle,
  * tpe_stage2_oracle.py:598-795): phi = K psi for the edge kernel kind
  * (0 'high', 1 'low'), psi/phi [E][ncomp] (ncomp 1 or 3); stats (nullable, 4):

@@ -128,6 +128,7 @@ def lib():
         L.rco_bct_matvec.argtypes = [i, i, _dp, _ip, d, _dp, i, i, d, d, i, _dp,
                                      C.POINTER(C.c_int64)]
         L.rco_bct_apply_b.argtypes = [i, i, _dp, _ip, d, _dp, i, d, d, i, _dp]
+        L.rco_gradient_bh_fd.argtypes = [i, i, _dp, _ip, d, d, d, d, d, i, d, _dp]
         L.rco_energy_bh.restype = d
         L.rco_energy_bh.argtypes = [i, i, _dp, _ip, d, d, d, d, d, i, C.POINTER(C.c_int64)]
         L.rco_step.argtypes = [i, i, _dp, _ip, d, d, d, i, d, d, _dp, _dp, _dp]
@@ -206,6 +207,16 @@ def energy_bh(v, e, theta, theta_t=None, leaf_size=8, alpha=3.0, beta=6.0, eps=1
     en = lib().rco_energy_bh(len(v), len(e), _d(v), _i(e), alpha, beta, eps, theta, tt,
                              leaf_size, st)
     return en, (st[0], st[1], st[2])
+
+
+def gradient_bh_fd(v, e, theta, theta_t=None, leaf_size=8, h=1e-6, alpha=3.0, beta=6.0,
+                   eps=1e-10):
+    v, e = _verts(v), _edges(e)
+    g = np.zeros_like(v)
+    tt = 1.5 * theta if theta_t is None else theta_t
+    lib().rco_gradient_bh_fd(len(v), len(e), _d(v), _i(e), alpha, beta, eps, theta, tt,
+                             leaf_size, h, _d(g))
+    return g
 
 
 def bct_matvec(v, e, psi, kind, theta, theta_t=None, leaf_size=8, eps=1e-10):

@@ -800,3 +800,27 @@ extern "C" rc_status rc_bct_apply_b(rc_ctx* ctx, const double* xyz, double eps, 
     RC_HIP(hipStreamSynchronize(ctx->stream));
     return RC_OK;
 }
+
+// Forward difference of the Barnes-Hut energy (the stage-2 oracle's
+// finite_difference_gradient of bh_energy, tpe_stage2_oracle.py:133-141,
+// 1289-1290): the tree depends on the curve, so every moved coordinate is a
+// fresh tree and traversal (3V + 1 Barnes-Hut energies).
+extern "C" rc_status rc_gradient_bh(rc_ctx* ctx, const double* xyz, const rc_params* p,
+                                    double theta, double theta_t, int32_t leaf_size, double h,
+                                    double* grad) {
+    if (!ctx || !xyz || !p || !grad || !std::isfinite(h) || h == 0.0) return RC_BAD_ARG;
+    double e0 = 0.0;
+    rc_status s = rc_energy_bh(ctx, xyz, p, theta, theta_t, leaf_size, &e0, nullptr);
+    if (s != RC_OK) return s;
+    std::vector<double> pert(xyz, xyz + 3 * (size_t)ctx->V);
+    for (int i = 0; i < 3 * ctx->V; ++i) {
+        const double keep = pert[i];
+        pert[i] = keep + h;
+        double e1 = 0.0;
+        s = rc_energy_bh(ctx, pert.data(), p, theta, theta_t, leaf_size, &e1, nullptr);
+        pert[i] = keep;
+        if (s != RC_OK) return s;
+        grad[i] = (e1 - e0) / h;
+    }
+    return RC_OK;
+}

@@ -117,7 +117,8 @@ class KernelProfile(C.Structure):
 
 EXPORTS = (
     "rc_default_step_opts", "rc_ctx_create", "rc_ctx_destroy", "rc_last_error", "rc_energy",
-    "rc_gradient", "rc_gradient_fd", "rc_step", "rc_energy_bh", "rc_bct_matvec",
+    "rc_gradient", "rc_gradient_fd", "rc_step", "rc_energy_bh", "rc_gradient_bh",
+    "rc_bct_matvec",
     "rc_bct_apply_b", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
     "rc_constraint_rows", "rc_sobolev_step", "rc_set_positions", "rc_get_positions",
     "rc_sobolev_step_resident", "rc_kernel_profile_enable", "rc_kernel_profile_read",
@@ -148,6 +149,8 @@ def lib():
     L.rc_gradient_fd.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, _dp]
     L.rc_energy_bh.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, C.c_double, C.c_int32,
                                _dp, C.POINTER(C.c_int64)]
+    L.rc_gradient_bh.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, C.c_double, C.c_int32,
+                                 C.c_double, _dp]
     L.rc_bct_matvec.argtypes = [vp, _dp, C.c_double, _dp, C.c_int32, C.c_int32, C.c_double,
                                 C.c_double, C.c_int32, _dp, C.POINTER(C.c_int64)]
     L.rc_bct_apply_b.argtypes = [vp, _dp, C.c_double, _dp, C.c_int32, C.c_double, C.c_double,
@@ -171,7 +174,7 @@ def lib():
     L.rc_kernel_profile_enable.argtypes = [vp, C.c_int32]
     L.rc_kernel_profile_read.argtypes = [vp, C.POINTER(KernelProfile)]
     for name in ("rc_ctx_create", "rc_energy", "rc_gradient", "rc_gradient_fd", "rc_step",
-                 "rc_energy_bh", "rc_bct_matvec", "rc_bct_apply_b",
+                 "rc_energy_bh", "rc_gradient_bh", "rc_bct_matvec", "rc_bct_apply_b",
                  "rc_assemble_a", "rc_penalties_eval",
                  "rc_solve_constrained_gradient", "rc_sobolev_step", "rc_set_positions",
                  "rc_get_positions", "rc_sobolev_step_resident", "rc_kernel_profile_enable",

@@ -131,6 +131,19 @@ def bh_energy(vertices, edges, alpha, beta, epsilon, theta=0.5, leaf_size=8, the
     return out.value, {"visits": st[0], "approximated_edge_refs": st[1], "direct_pairs": st[2]}
 
 
+def bh_gradient_fd(vertices, edges, alpha, beta, epsilon, theta=0.5, leaf_size=8, theta_t=None,
+                   h=1e-6):
+    """Forward difference of the Barnes-Hut energy (the stage-2 oracle's
+    finite_difference_gradient of bh_energy)."""
+    v = _verts(vertices)
+    ctx = context_for(edges, len(v))
+    tt = 1.5 * theta if theta_t is None else theta_t
+    g = np.zeros_like(v)
+    ctx.check(L.lib().rc_gradient_bh(ctx.handle, L.dptr(v), C.byref(_params(alpha, beta, epsilon)),
+                                     float(theta), float(tt), int(leaf_size), float(h), L.dptr(g)))
+    return g
+
+
 _KINDS = {"high": 0, "low": 1}
 
 

@@ -12,8 +12,9 @@ and 120) and three of its golden fixtures (a junction, two linked rings, a
 knot); outputs are the energy and the traversal counts, and for the
 block-cluster-tree products of the two edge kernels (bct_matvec_kernel,
 bct_apply_B_scalar, :637-795; equally without a golden that admits a block)
-the products of fixed densities and the block counts. Only these numbers are
-written (tests/golden/bh_goldens.json), no reference code.
+the products of fixed densities and the block counts, and the Barnes-Hut
+differential (its forward difference, :133-141, 1289-1290). Only these
+numbers are written (tests/golden/bh_goldens.json), no reference code.
 
 Usage: python tests/golden/make_bh_goldens.py [/root/reference]
 """
@@ -81,13 +82,24 @@ def main(ref):
                             "kind": kind, "psi1": psi1.tolist(), "psi3": psi3.tolist(),
                             "u": u.tolist(), "phi1": phi1.tolist(), "phi3": phi3.tolist(),
                             "Bu": bu.tolist(), "stats": {k: int(x) for k, x in st1.items()}})
-    out = {"alpha": ALPHA, "beta": BETA, "epsilon": EPS,
+    # Barnes-Hut differential: the oracle's forward difference of bh_energy
+    # (finite_difference_gradient, h = 1e-6; tpe_stage2_oracle.py:133-141,1289-1290)
+    grads = []
+    for name in ("trefoil60", "linked-rings", "knot"):
+        v, e = shapes[name]
+        for theta, theta_t, leaf in [(0.5, 0.75, 4), (0.0, 0.0, 8)]:
+            g = O.finite_difference_gradient(
+                lambda V, th=theta, tt=theta_t, lf=leaf: O.bh_energy(
+                    V, e, ALPHA, BETA, EPS, th, lf, theta_t=tt)[0], v)
+            grads.append({"shape": name, "theta": theta, "theta_t": theta_t, "leaf_size": leaf,
+                          "h": 1e-6, "dE": g.tolist()})
+    out = {"alpha": ALPHA, "beta": BETA, "epsilon": EPS, "bh_gradients": grads,
            "shapes": {k: {"vertices": v.tolist(), "edges": e.tolist()}
                       for k, (v, e) in shapes.items()},
            "cases": cases, "bct": bct}
     with open(os.path.join(HERE, "bh_goldens.json"), "w") as f:
         json.dump(out, f, indent=1)
-    print(f"wrote {len(cases)} energy cases, {len(bct)} matvec cases")
+    print(f"wrote {len(cases)} energy cases, {len(bct)} matvec cases, {len(grads)} gradients")
 
 
 if __name__ == "__main__":

@@ -116,3 +116,22 @@ def test_bct_products_vs_oracle_at_size(kind):
         assert frel(bu, rco.bct_apply_b(v, e, u, kind, theta, tt, leaf)) <= 1e-12
     log("  relative errors against the exact product", errs)
     assert errs[0.25] < errs[0.5]
+
+
+def test_bh_gradient_vs_reference_vectors():
+    G = json.load(open(os.path.join(HERE, "golden", "bh_goldens.json")))
+    for c in G["bh_gradients"]:
+        sh = G["shapes"][c["shape"]]
+        v, e = np.asarray(sh["vertices"]), np.asarray(sh["edges"], np.int32)
+        log("bh gradient", c["shape"], c["theta"], c["leaf_size"])
+        g = core.bh_gradient_fd(v, e, 3.0, 6.0, 1e-10, c["theta"], c["leaf_size"], c["theta_t"],
+                                c["h"])
+        E = rco.energy(v, e)
+        m = len(e)
+        # the device and the reference sum each energy in different orders:
+        # 2 sqrt(P) quanta of ulp(E)/h bound the two differences' rounding
+        tol = 2.0 * np.sqrt(m * (m - 3)) * float(np.spacing(E)) / c["h"]
+        assert np.abs(g - np.asarray(c["dE"])).max() <= tol, c["shape"]
+        if c["theta"] == 0.0:   # no cluster: the exact energy's forward difference
+            ge = core.gradient_finite_diff(v, e, None, 3.0, 6.0, 1e-10, c["h"])
+            assert np.abs(g - ge).max() <= tol

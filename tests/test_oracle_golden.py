@@ -274,3 +274,18 @@ def test_bct_products_oracle_vs_reference_vectors():
         assert st == (s["leaves"], s["admissible_blocks"], s["direct_blocks"], s["direct_pairs"])
         admitted += st[1] > 0
     assert admitted >= 10
+
+
+def test_bh_gradient_oracle_vs_reference_vectors():
+    # forward difference of the Barnes-Hut energy (tpe_stage2_oracle.py:
+    # 133-141, 1289-1290): agreement to the resolution of a difference of two
+    # energies (quanta of ulp(E)/h), a handful of quanta here
+    import json
+    import os
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "bh_goldens.json")))
+    for c in G["bh_gradients"]:
+        sh = G["shapes"][c["shape"]]
+        g = rco.gradient_bh_fd(sh["vertices"], sh["edges"], c["theta"], c["theta_t"],
+                               c["leaf_size"], c["h"])
+        q = float(np.spacing(rco.energy(sh["vertices"], sh["edges"]))) / c["h"]
+        assert np.abs(g - np.asarray(c["dE"])).max() <= 8 * q, c["shape"]
