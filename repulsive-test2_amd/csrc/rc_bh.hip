@@ -381,32 +381,45 @@ namespace {
 
 struct BctDev {
     const int32_t *start, *end, *parent, *leaf_of, *ord;
-    const int32_t *adm_off, *adm_src, *dir_off, *dir_src;
+    const int32_t *adm_off, *adm_src, *dir_off, *dir_src;   // dir_src: source edges
     const double* kval;
 };
 
-__global__ void k_bct_moments(EdgeSoA ed, BctDev t, int nn, const double* __restrict__ psi,
-                              int nc, double* __restrict__ mom) {
-    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per node: the lanes stride over the node's edges, then a
+// fixed-order butterfly sum.
+__global__ __launch_bounds__(256) void k_bct_moments(EdgeSoA ed, BctDev t, int nn,
+                                                     const double* __restrict__ psi, int nc,
+                                                     double* __restrict__ mom) {
+    const int lane = threadIdx.x & 63;
+    const int nd = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (nd >= nn) return;
     double s[3] = {0, 0, 0};
-    for (int p = t.start[nd]; p < t.end[nd]; ++p) {
+    for (int p = t.start[nd] + lane; p < t.end[nd]; p += 64) {
         const int J = t.ord[p];
         const double mj = ed.ell[J];
         for (int c = 0; c < nc; ++c) s[c] += mj * psi[(size_t)nc * J + c];
     }
-    for (int c = 0; c < nc; ++c) mom[(size_t)nc * nd + c] = s[c];
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_xor(s[c], off, 64);
+    if (lane == 0)
+        for (int c = 0; c < nc; ++c) mom[(size_t)nc * nd + c] = s[c];
 }
 
+// One wave per target edge I: the lanes stride over the admissible blocks of
+// the nodes on I's leaf-to-root path and over the source edges of I's direct
+// blocks (concatenated per target leaf on the host), then a fixed-order
+// butterfly sum.
 __global__ __launch_bounds__(256) void k_bct_apply(EdgeSoA ed, int E, BctDev t, int kind,
                                                    double eps, const double* __restrict__ psi,
                                                    int nc, const double* __restrict__ mom,
                                                    double* __restrict__ phi) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int I = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (I >= E) return;
     double far[3] = {0, 0, 0}, near[3] = {0, 0, 0};
     for (int nd = t.leaf_of[I]; nd >= 0; nd = t.parent[nd])
-        for (int q = t.adm_off[nd]; q < t.adm_off[nd + 1]; ++q) {
+        for (int q = t.adm_off[nd] + lane; q < t.adm_off[nd + 1]; q += 64) {
             const double kv = t.kval[q];
             const int sn = t.adm_src[q];
             for (int c = 0; c < nc; ++c) far[c] += kv * mom[(size_t)nc * sn + c];
@@ -416,45 +429,49 @@ __global__ __launch_bounds__(256) void k_bct_apply(EdgeSoA ed, int E, BctDev t, 
     const double TI[3] = {ed.ux[I], ed.uy[I], ed.uz[I]};
     const double ellI = ed.ell[I];
     const int leaf = t.leaf_of[I];
-    for (int q = t.dir_off[leaf]; q < t.dir_off[leaf + 1]; ++q) {
-        const int sn = t.dir_src[q];
-        for (int p = t.start[sn]; p < t.end[sn]; ++p) {
-            const int J = t.ord[p];
-            const int j1 = ed.i1[J], j2 = ed.i2[J];
-            if (i1 == j1 || i1 == j2 || i2 == j1 || i2 == j2) continue;
-            const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
-            const double TJ[3] = {ed.ux[J], ed.uy[J], ed.uz[J]};
-            double sh = 0.0, sl = 0.0;
+    for (int q = t.dir_off[leaf] + lane; q < t.dir_off[leaf + 1]; q += 64) {
+        const int J = t.dir_src[q];
+        const int j1 = ed.i1[J], j2 = ed.i2[J];
+        if (i1 == j1 || i1 == j2 || i2 == j1 || i2 == j2) continue;
+        const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
+        const double TJ[3] = {ed.ux[J], ed.uy[J], ed.uz[J]};
+        double sh = 0.0, sl = 0.0;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const double dx = PI[3 * a] - PJ[3 * b], dy = PI[3 * a + 1] - PJ[3 * b + 1],
-                                 dz = PI[3 * a + 2] - PJ[3 * b + 2];
-                    const double dn = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
-                    const double y = rcm::rcbrt(dn);
-                    const double y2 = y * y, y4 = y2 * y2;
-                    const double dp = y4 * y2 * y;   // dn^(-7/3)
-                    if (kind == 0) {
-                        sh += dp;
-                    } else {
-                        const double c1x = TI[1] * dz - TI[2] * dy, c1y = TI[2] * dx - TI[0] * dz,
-                                     c1z = TI[0] * dy - TI[1] * dx;
-                        const double c2x = -(TJ[1] * dz - TJ[2] * dy),
-                                     c2y = -(TJ[2] * dx - TJ[0] * dz),
-                                     c2z = -(TJ[0] * dy - TJ[1] * dx);
-                        const double n1 = rcm::sqrt_(c1x * c1x + c1y * c1y + c1z * c1z) + eps;
-                        const double n2 = rcm::sqrt_(c2x * c2x + c2y * c2y + c2z * c2z) + eps;
-                        const double r = rcm::rcp(dn), r2 = r * r;
-                        sl += (n1 * n1 + n2 * n2) * (r2 * r2) * dp;
-                    }
+            for (int b = 0; b < 2; ++b) {
+                const double dx = PI[3 * a] - PJ[3 * b], dy = PI[3 * a + 1] - PJ[3 * b + 1],
+                             dz = PI[3 * a + 2] - PJ[3 * b + 2];
+                const double dn = rcm::sqrt_(dx * dx + dy * dy + dz * dz) + eps;
+                const double y = rcm::rcbrt(dn);
+                const double y2 = y * y, y4 = y2 * y2;
+                const double dp = y4 * y2 * y;   // dn^(-7/3)
+                if (kind == 0) {
+                    sh += dp;
+                } else {
+                    const double c1x = TI[1] * dz - TI[2] * dy, c1y = TI[2] * dx - TI[0] * dz,
+                                 c1z = TI[0] * dy - TI[1] * dx;
+                    const double c2x = -(TJ[1] * dz - TJ[2] * dy), c2y = -(TJ[2] * dx - TJ[0] * dz),
+                                 c2z = -(TJ[0] * dy - TJ[1] * dx);
+                    const double n1 = rcm::sqrt_(c1x * c1x + c1y * c1y + c1z * c1z) + eps;
+                    const double n2 = rcm::sqrt_(c2x * c2x + c2y * c2y + c2z * c2z) + eps;
+                    const double r = rcm::rcp(dn), r2 = r * r;
+                    sl += (n1 * n1 + n2 * n2) * (r2 * r2) * dp;
                 }
-            const double k = kind == 0 ? 2.0 * 0.25 * ellI * ed.ell[J] * sh
-                                       : 0.25 * ellI * ed.ell[J] * sl;
-            for (int c = 0; c < nc; ++c) near[c] += k * psi[(size_t)nc * J + c];
+            }
+        const double k = kind == 0 ? 2.0 * 0.25 * ellI * ed.ell[J] * sh
+                                   : 0.25 * ellI * ed.ell[J] * sl;
+        for (int c = 0; c < nc; ++c) near[c] += k * psi[(size_t)nc * J + c];
+    }
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            far[c] += __shfl_xor(far[c], off, 64);
+            near[c] += __shfl_xor(near[c], off, 64);
         }
     }
-    for (int c = 0; c < nc; ++c) phi[(size_t)nc * I + c] = ellI * far[c] + near[c];
+    if (lane == 0)
+        for (int c = 0; c < nc; ++c) phi[(size_t)nc * I + c] = ellI * far[c] + near[c];
 }
 
 // y of bct_apply_B_scalar: 'high' ((u_b - u_a) / ell) T per edge (3 comps),
@@ -638,13 +655,16 @@ static void bct_build_host(const rc_ctx* ctx, const double* xyz, double eps, dou
             H.kval[at] = kv[q];
         }
     }
+    // the source edges of each target leaf's direct blocks, in block order
     H.dir_off.assign(nn + 1, 0);
-    for (const auto& b : dir) H.dir_off[b.first + 1]++;
+    for (const auto& b : dir) H.dir_off[b.first + 1] += t.end[b.second] - t.start[b.second];
     for (int i = 0; i < nn; ++i) H.dir_off[i + 1] += H.dir_off[i];
-    H.dir_src.resize(dir.size());
+    H.dir_src.resize(H.dir_off[nn]);
     {
         std::vector<int32_t> f(H.dir_off.begin(), H.dir_off.end() - 1);
-        for (const auto& b : dir) H.dir_src[f[b.first]++] = b.second;
+        for (const auto& b : dir)
+            for (int p = t.start[b.second]; p < t.end[b.second]; ++p)
+                H.dir_src[f[b.first]++] = t.ord[p];
     }
 }
 
@@ -733,10 +753,10 @@ extern "C" rc_status rc_bct_matvec(rc_ctx* ctx, const double* xyz, double eps, c
     if (s != RC_OK) return s;
     double *d_psi = scr, *d_mom = scr + nin, *d_phi = scr + nin + nmom;
     RC_HIP(hipMemcpyAsync(d_psi, psi, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_bct_moments, dim3((nn + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
+    hipLaunchKernelGGL(k_bct_moments, dim3((nn + 3) / 4), dim3(256), 0, ctx->stream, ctx->ed,
                        t, nn, (const double*)d_psi, ncomp, d_mom);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_bct_apply, dim3((E + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed, E,
+    hipLaunchKernelGGL(k_bct_apply, dim3((E + 3) / 4), dim3(256), 0, ctx->stream, ctx->ed, E,
                        t, kind, eps, (const double*)d_psi, ncomp, (const double*)d_mom, d_phi);
     RC_CHECK_LAUNCH();
     RC_HIP(hipMemcpyAsync(phi, d_phi, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
@@ -779,7 +799,7 @@ extern "C" rc_status rc_bct_apply_b(rc_ctx* ctx, const double* xyz, double eps, 
     hipLaunchKernelGGL(k_bct_density, dim3((E + 255) / 256), dim3(256), 0, ctx->stream, ctx->ed,
                        E, (int)kind, (const double*)d_u, d_y, d_one);
     RC_CHECK_LAUNCH();
-    const dim3 gn((nn + 255) / 256), ge((E + 255) / 256);
+    const dim3 gn((nn + 3) / 4), ge((E + 3) / 4);
     hipLaunchKernelGGL(k_bct_moments, gn, dim3(256), 0, ctx->stream, ctx->ed, t, nn,
                        (const double*)d_one, 1, d_mom);
     RC_CHECK_LAUNCH();
