@@ -57,7 +57,7 @@ def make_workload(V):
     return v, e, cs, pen
 
 
-def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
+def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False, projection="frozen"):
     """Warm up, then time exactly `steps` resident steps from v0. With kprof,
     HIP events bracket every trailing-update launch of the timed steps."""
     from rcurves import _lib as L
@@ -65,7 +65,7 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
     arr, nb, keep = core._blocks_array(cs)
     penc = pen.to_c()
     opts = core.SobolevStepOptions(precision=precision, collect_timings=True,
-                                   projection_mode="frozen").to_c()
+                                   projection_mode=projection).to_c()
     ctx.check(L.lib().rc_set_positions(ctx.handle, L.dptr(np.ascontiguousarray(v0))))
     st = L.StepStats()
     tm = L.Timings()
@@ -89,7 +89,7 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
 
     for i in range(warmup):
         one(False)
-        log(f"[{precision}] warmup {i}: accepted={st.accepted} tau={st.tau} E={st.energy:.12g} "
+        log(f"[{precision}{'' if projection == 'frozen' else '/' + projection}] warmup {i}: accepted={st.accepted} tau={st.tau} E={st.energy:.12g} "
             f"trials={st.trials} refine={st.refine_iters}")
     prof = None
     if kprof:
@@ -103,7 +103,7 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False):
         ctx.check(L.lib().rc_kernel_profile_read(ctx.handle, C.byref(kp)))
         ctx.check(L.lib().rc_kernel_profile_enable(ctx.handle, 0))
         prof = {"ms": kp.syrk_ms, "flop": kp.syrk_flop, "launches": kp.syrk_launches}
-    log(f"[{precision}] {steps} steps in {dt:.4f}s; last accepted={st.accepted} tau={st.tau} "
+    log(f"[{precision}{'' if projection == 'frozen' else '/' + projection}] {steps} steps in {dt:.4f}s; last accepted={st.accepted} tau={st.tau} "
         f"E={st.energy:.12g} trials={trials}")
     del keep
     return {"t": dt / steps, "phase": {k: v / steps for k, v in phase.items()}, "trials": trials,
@@ -267,6 +267,11 @@ def main():
             acc = accuracy(v, e, cs, pen)
         except L.RcError as err:
             log("mixed form unavailable:", err)
+    # the reference function's default projection: the saddle refactored at
+    # every projection iterate (a few steps; not the headline form)
+    rre = None
+    if not args.fp64_only:
+        rre = run_form(ctx, v, cs, pen, "fp64", min(args.steps, 3), 1, projection="reassemble")
     # headline form: mixed when it passes the accuracy gates, else FP64
     mixed_ok = (rmx is not None and all(rmx["accepted"]) and "energy_rel_err" in acc
                 and acc["energy_rel_err"] <= 1e-10 and acc["direction_rel_err"] <= 1e-8)
@@ -295,6 +300,9 @@ def main():
                    "mixed_accepted": rmx["accepted"] if rmx else None,
                    "mixed_passes_gates": mixed_ok,
                    "speedup_mixed_vs_fp64": (r64["t"] / rmx["t"]) if rmx else None,
+                   "reassemble_steps_per_s": (1.0 / rre["t"]) if rre else None,
+                   "reassemble_phase_ms": ({k: round(x, 4) for k, x in rre["phase"].items()}
+                                           if rre else None),
                    "phase_ms": {k: round(x, 4) for k, x in sel["phase"].items()},
                    "phase_ms_fp64": {k: round(x, 4) for k, x in r64["phase"].items()},
                    "trials_per_step": r64["trials"], **acc},
