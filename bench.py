@@ -249,8 +249,8 @@ def main():
                     help="also run one device-to-device copy of this size (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the Sobolev step has no multi-GPU path in this build "
-              "(DESIGN.md describes how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the Sobolev step has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from rcurves import _lib as L
     if args.copy_calibration:
