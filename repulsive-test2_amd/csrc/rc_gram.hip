@@ -227,15 +227,38 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
                 // tile-local indices: lo in the U tile, hi in the W tile
                 const int rl = swap ? c : r, rh = swap ? r : c;
                 double val = 0.0;
-                for (int q1 = begU[rl]; q1 < begU[rl + 1]; ++q1) {
-                    const int e1 = incU[q1];
-                    const int li = e1 & 0xffff;
-                    const double si = (e1 >> 16) ? -1.0 : 1.0;
-                    for (int q2 = begW[rh]; q2 < begW[rh + 1]; ++q2) {
-                        const int e2 = incW[q2];
-                        const int lj = e2 & 0xffff;
-                        const double sj = (e2 >> 16) ? -1.0 : 1.0;
-                        val -= si * sj * Xt[li][lj] + Yt[li][lj];
+                const int q10 = begU[rl], n1 = begU[rl + 1] - q10;
+                const int q20 = begW[rh], n2 = begW[rh + 1] - q20;
+                if (n1 <= 2 && n2 <= 2) {
+                    // vertices of degree <= 2 (curves, and all but the junctions of
+                    // networks): the same terms in the same order, unrolled so the
+                    // LDS reads issue together
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        if (a >= n1) break;
+                        const int e1 = incU[q10 + a];
+                        const int li = e1 & 0xffff;
+                        const double si = (e1 >> 16) ? -1.0 : 1.0;
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) {
+                            if (b >= n2) break;
+                            const int e2 = incW[q20 + b];
+                            const int lj = e2 & 0xffff;
+                            const double sj = (e2 >> 16) ? -1.0 : 1.0;
+                            val -= si * sj * Xt[li][lj] + Yt[li][lj];
+                        }
+                    }
+                } else {
+                    for (int q1 = q10; q1 < q10 + n1; ++q1) {
+                        const int e1 = incU[q1];
+                        const int li = e1 & 0xffff;
+                        const double si = (e1 >> 16) ? -1.0 : 1.0;
+                        for (int q2 = q20; q2 < q20 + n2; ++q2) {
+                            const int e2 = incW[q2];
+                            const int lj = e2 & 0xffff;
+                            const double sj = (e2 >> 16) ? -1.0 : 1.0;
+                            val -= si * sj * Xt[li][lj] + Yt[li][lj];
+                        }
                     }
                 }
                 if (pass == 0)
