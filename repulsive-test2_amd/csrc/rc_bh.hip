@@ -174,73 +174,121 @@ __device__ __forceinline__ double bh_cluster(const P& pw, double eps, const doub
     return 0.5 * lpI * mass * s;
 }
 
+// Work item (f, I): target edge I against frontier node f (the nodes of depth
+// 2, and leaves above that depth, in left-first order). The nodes above f are
+// walked first, as the traversal from the root meets them: each counts its
+// visit in the item that is its leftmost frontier descendant, and the first
+// admissible one contributes its cluster term in that item and ends every
+// item below it. Otherwise the item traverses f's subtree with an explicit
+// stack (left child first). The per-item sums are added per edge in frontier
+// order.
+struct BhFront {
+    const int32_t* node;    // frontier node
+    const int32_t* anc;     // [2] ancestors (root, child), -1 if none
+    const int32_t* first;   // [2] 1 if this item is the ancestor's leftmost frontier descendant
+    int nf;
+};
+
 template <class P>
-__global__ __launch_bounds__(256) void k_energy_bh(EdgeSoA ed, int E, BhDev t,
+__device__ __forceinline__ bool bh_admissible(const BhDev& t, int ni, int i1, int i2,
+                                              const int32_t* __restrict__ vOff,
+                                              const int32_t* __restrict__ vEdge, double mx,
+                                              double my, double mz, double theta, double theta_t) {
+    const int s = t.start[ni], e = t.end[ni];
+    if (e - s < 2) return false;
+    const double dx = mx - t.cx[ni], dy = my - t.cy[ni], dz = mz - t.cz[ni];
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(dist > 0.0 && t.rx[ni] / dist <= theta && t.rT[ni] <= theta_t)) return false;
+    for (int q = vOff[i1]; q < vOff[i1 + 1]; ++q) {
+        const int p = t.pos[vEdge[q]];
+        if (p >= s && p < e) return false;
+    }
+    for (int q = vOff[i2]; q < vOff[i2 + 1]; ++q) {
+        const int p = t.pos[vEdge[q]];
+        if (p >= s && p < e) return false;
+    }
+    return true;
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_energy_bh(EdgeSoA ed, int E, BhDev t, BhFront fr,
                                                    const int32_t* __restrict__ vOff,
                                                    const int32_t* __restrict__ vEdge, P pw,
                                                    double eps, double theta, double theta_t,
                                                    double* __restrict__ part,
                                                    unsigned long long* __restrict__ stats) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= E) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= fr.nf * E) return;
+    const int f = gid / E, I = gid - f * E;   // a wave: 64 consecutive edges, one subtree
     const int i1 = ed.i1[I], i2 = ed.i2[I];
     const double PI[6] = {ed.ax[I], ed.ay[I], ed.az[I], ed.bx[I], ed.by[I], ed.bz[I]};
     const double eI[3] = {PI[3] - PI[0], PI[4] - PI[1], PI[5] - PI[2]};
     const double mx = 0.5 * (PI[0] + PI[3]), my = 0.5 * (PI[1] + PI[4]),
                  mz = 0.5 * (PI[2] + PI[5]);
     const double lpI = ed.lpow[I];
-    int stack[BH_STACK];
-    int sp = 0;
-    stack[sp++] = 0;
     double sum = 0.0;
     unsigned long long visits = 0, approx = 0, direct = 0;
-    while (sp > 0) {
-        const int ni = stack[--sp];
-        ++visits;
-        const int s = t.start[ni], e = t.end[ni];
-        const int n = e - s;
-        if (n == 0) continue;
-        const double dx = mx - t.cx[ni], dy = my - t.cy[ni], dz = mz - t.cz[ni];
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        bool admiss = n >= 2 && dist > 0.0 && t.rx[ni] / dist <= theta && t.rT[ni] <= theta_t;
-        if (admiss) {
-            // no edge through an endpoint of I (I included) in the node
-            for (int q = vOff[i1]; q < vOff[i1 + 1] && admiss; ++q) {
-                const int p = t.pos[vEdge[q]];
-                if (p >= s && p < e) admiss = false;
+    bool stop = false;
+    for (int k = 0; k < 2 && !stop; ++k) {
+        const int a = fr.anc[2 * f + k];
+        if (a < 0) break;
+        const bool first = fr.first[2 * f + k] != 0;
+        if (first) ++visits;
+        if (bh_admissible<P>(t, a, i1, i2, vOff, vEdge, mx, my, mz, theta, theta_t)) {
+            if (first) {
+                approx += t.end[a] - t.start[a];
+                sum += bh_cluster(pw, eps, PI, eI, lpI, t.cx[a], t.cy[a], t.cz[a], t.mass[a]);
             }
-            for (int q = vOff[i2]; q < vOff[i2 + 1] && admiss; ++q) {
-                const int p = t.pos[vEdge[q]];
-                if (p >= s && p < e) admiss = false;
-            }
+            stop = true;
         }
-        if (admiss) {
-            approx += n;
-            sum += bh_cluster(pw, eps, PI, eI, lpI, t.cx[ni], t.cy[ni], t.cz[ni], t.mass[ni]);
-            continue;
-        }
-        if (t.left[ni] < 0) {
-            for (int p = s; p < e; ++p) {
-                const int J = t.ord[p];
-                const int j1 = ed.i1[J], j2 = ed.i2[J];
-                if (i1 == j1 || i1 == j2 || i2 == j1 || i2 == j2) continue;
-                ++direct;
-                const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
-                sum += bh_exact(pw, eps, PI, eI, lpI, PJ, ed.ell[J]);
-            }
-            continue;
-        }
-        if (sp + 2 > BH_STACK) {   // cannot happen for a median split; counted, not faulted
-            atomicAdd(stats + 3, 1ull);
-            continue;
-        }
-        stack[sp++] = t.right[ni];
-        stack[sp++] = t.left[ni];
     }
-    part[I] = sum;
+    if (!stop) {
+        int stack[BH_STACK];
+        int sp = 0;
+        stack[sp++] = fr.node[f];
+        while (sp > 0) {
+            const int ni = stack[--sp];
+            ++visits;
+            const int s = t.start[ni], e = t.end[ni];
+            if (e - s == 0) continue;
+            if (bh_admissible<P>(t, ni, i1, i2, vOff, vEdge, mx, my, mz, theta, theta_t)) {
+                approx += e - s;
+                sum += bh_cluster(pw, eps, PI, eI, lpI, t.cx[ni], t.cy[ni], t.cz[ni], t.mass[ni]);
+                continue;
+            }
+            if (t.left[ni] < 0) {
+                for (int p = s; p < e; ++p) {
+                    const int J = t.ord[p];
+                    const int j1 = ed.i1[J], j2 = ed.i2[J];
+                    if (i1 == j1 || i1 == j2 || i2 == j1 || i2 == j2) continue;
+                    ++direct;
+                    const double PJ[6] = {ed.ax[J], ed.ay[J], ed.az[J], ed.bx[J], ed.by[J], ed.bz[J]};
+                    sum += bh_exact(pw, eps, PI, eI, lpI, PJ, ed.ell[J]);
+                }
+                continue;
+            }
+            if (sp + 2 > BH_STACK) {   // cannot happen for a median split; counted, not faulted
+                atomicAdd(stats + 3, 1ull);
+                continue;
+            }
+            stack[sp++] = t.right[ni];
+            stack[sp++] = t.left[ni];
+        }
+    }
+    part[(size_t)f * E + I] = sum;
     atomicAdd(stats + 0, visits);
     atomicAdd(stats + 1, approx);
     atomicAdd(stats + 2, direct);
+}
+
+// Per edge, the item sums in frontier order.
+__global__ void k_bh_items(const double* __restrict__ part, int E, int nf,
+                           double* __restrict__ out) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= E) return;
+    double s = 0.0;
+    for (int f = 0; f < nf; ++f) s += part[(size_t)f * E + I];
+    out[I] = s;
 }
 
 // Fixed-order sum of the per-edge partials, times 1/2.
@@ -330,21 +378,77 @@ extern "C" rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_param
     const int32_t* g = ctx->d_bh_i;
     t.start = g; t.end = g + nn; t.left = g + 2 * (size_t)nn; t.right = g + 3 * (size_t)nn;
     t.ord = g + 4 * (size_t)nn; t.pos = g + 4 * (size_t)nn + E;
-    double* part = ctx->d_gpart;   // >= E doubles
-    const dim3 grid((E + 255) / 256);
+    // frontier: nodes of depth 2 and leaves above it, left-first, with their
+    // ancestors and whether the item is each ancestor's leftmost descendant
+    std::vector<int32_t> fnode, fanc, ffirst;
+    {
+        struct It { int n, d, a0, a1; };
+        std::vector<It> st{{0, 0, -1, -1}};
+        while (!st.empty()) {
+            const It it = st.back();
+            st.pop_back();
+            if (it.d == 2 || tree.left[it.n] < 0) {
+                fnode.push_back(it.n);
+                fanc.push_back(it.a0);
+                fanc.push_back(it.a1);
+                ffirst.push_back(0);
+                ffirst.push_back(0);
+                continue;
+            }
+            const int a0 = it.d == 0 ? it.n : it.a0, a1 = it.d == 1 ? it.n : it.a1;
+            st.push_back({tree.right[it.n], it.d + 1, a0, a1});
+            st.push_back({tree.left[it.n], it.d + 1, a0, a1});
+        }
+        // leftmost frontier descendant of each ancestor: its first item in order
+        std::vector<char> seen(nn, 0);
+        for (size_t f = 0; f < fnode.size(); ++f)
+            for (int k = 0; k < 2; ++k) {
+                const int a = fanc[2 * f + k];
+                if (a >= 0 && !seen[a]) {
+                    seen[a] = 1;
+                    ffirst[2 * f + k] = 1;
+                }
+            }
+    }
+    const int nf = (int)fnode.size();
+    // device copies of the frontier: after the tree arrays in the int buffer
+    if (ctx->bh_front_cap < 5 * (size_t)nf) {
+        if (ctx->d_bh_front) (void)hipFree(ctx->d_bh_front);
+        ctx->d_bh_front = nullptr;
+        ctx->bh_front_cap = 0;
+        RC_HIP(hipMalloc((void**)&ctx->d_bh_front, sizeof(int32_t) * 5 * (size_t)nf));
+        ctx->bh_front_cap = 5 * (size_t)nf;
+    }
+    std::vector<int32_t> hfr;
+    hfr.reserve(5 * (size_t)nf);
+    hfr.insert(hfr.end(), fnode.begin(), fnode.end());
+    hfr.insert(hfr.end(), fanc.begin(), fanc.end());
+    hfr.insert(hfr.end(), ffirst.begin(), ffirst.end());
+    RC_HIP(hipMemcpyAsync(ctx->d_bh_front, hfr.data(), sizeof(int32_t) * hfr.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    BhFront fr{ctx->d_bh_front, ctx->d_bh_front + nf, ctx->d_bh_front + 3 * (size_t)nf, nf};
+    // item sums [nf][E] and per-edge sums [E]: the gradient partial buffer
+    // holds 6 ntiles Epad >= 5 E doubles (nf <= 4)
+    double* part = ctx->d_gpart;
+    double* per_edge = ctx->d_gpart + (size_t)nf * E;
+    const dim3 grid(((size_t)nf * E + 255) / 256);
     if (p->alpha == 3.0 && p->beta == 6.0) {
-        hipLaunchKernelGGL(k_energy_bh<Pow36>, grid, dim3(256), 0, ctx->stream, ctx->ed, E, t,
+        hipLaunchKernelGGL(k_energy_bh<Pow36>, grid, dim3(256), 0, ctx->stream, ctx->ed, E, t, fr,
                            ctx->d_vOff, ctx->d_vEdge, Pow36{}, p->epsilon, theta, theta_t, part,
                            ctx->d_bh_stats);
     } else {
         const double sg = (p->beta - 1) / p->alpha;
         PowGeneric pw{p->alpha, p->beta, 2 * (sg - 1) + 1};
         hipLaunchKernelGGL(k_energy_bh<PowGeneric>, grid, dim3(256), 0, ctx->stream, ctx->ed, E,
-                           t, ctx->d_vOff, ctx->d_vEdge, pw, p->epsilon, theta, theta_t, part,
+                           t, fr, ctx->d_vOff, ctx->d_vEdge, pw, p->epsilon, theta, theta_t, part,
                            ctx->d_bh_stats);
     }
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_bh_reduce, dim3(1), dim3(1024), 0, ctx->stream, part, E, ctx->d_scal);
+    hipLaunchKernelGGL(k_bh_items, dim3((E + 255) / 256), dim3(256), 0, ctx->stream, part, E, nf,
+                       per_edge);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_bh_reduce, dim3(1), dim3(1024), 0, ctx->stream, (const double*)per_edge,
+                       E, ctx->d_scal);
     RC_CHECK_LAUNCH();
     RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_scal, sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));

@@ -136,6 +136,8 @@ struct rc_ctx {
     int32_t* d_bh_i = nullptr;      // 4 ints per node + node order + positions
     unsigned long long* d_bh_stats = nullptr;
     size_t bh_cap_d = 0, bh_cap_i = 0;
+    int32_t* d_bh_front = nullptr;  // traversal work items (frontier of the tree)
+    size_t bh_front_cap = 0;
     // block-cluster-tree products (rc_bct_matvec, rc_bct_apply_b), grown on demand
     double* d_bct_f = nullptr;
     int32_t* d_bct_i = nullptr;
