@@ -108,7 +108,7 @@ static void free_all(rc_ctx* ctx) {
     void* sp[] = {ctx->d_set, ctx->d_ctl, ctx->d_dots, ctx->d_bsum, ctx->d_ssol, ctx->d_Minv,
                   ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_refine,
                   ctx->d_stage, ctx->d_bh_f, ctx->d_bh_i, ctx->d_bh_stats, ctx->d_bct_f,
-                  ctx->d_bct_i, ctx->d_bh_front};
+                  ctx->d_bct_i, ctx->d_bh_front, ctx->d_bh_part};
     for (void* p : sp)
         if (p) (void)hipFree(p);
     if (ctx->d_host_stage) (void)hipHostFree(ctx->d_host_stage);

@@ -25,6 +25,7 @@
 namespace {
 
 constexpr int BH_STACK = 64;   // > depth + 1 of a median-split tree over 2^31 edges
+constexpr int BH_FD = 3;       // work items: the subtrees at this depth
 
 struct BhTree {
     std::vector<int32_t> ord, pos;               // node order and its inverse
@@ -175,7 +176,7 @@ __device__ __forceinline__ double bh_cluster(const P& pw, double eps, const doub
 }
 
 // Work item (f, I): target edge I against frontier node f (the nodes of depth
-// 2, and leaves above that depth, in left-first order). The nodes above f are
+// BH_FD, and leaves above that depth, in left-first order). The nodes above f are
 // walked first, as the traversal from the root meets them: each counts its
 // visit in the item that is its leftmost frontier descendant, and the first
 // admissible one contributes its cluster term in that item and ends every
@@ -184,8 +185,8 @@ __device__ __forceinline__ double bh_cluster(const P& pw, double eps, const doub
 // order.
 struct BhFront {
     const int32_t* node;    // frontier node
-    const int32_t* anc;     // [2] ancestors (root, child), -1 if none
-    const int32_t* first;   // [2] 1 if this item is the ancestor's leftmost frontier descendant
+    const int32_t* anc;     // [BH_FD] ancestors from the root down, -1 past the last
+    const int32_t* first;   // [BH_FD] 1 if this item is the ancestor's leftmost frontier descendant
     int nf;
 };
 
@@ -229,10 +230,10 @@ __global__ __launch_bounds__(256) void k_energy_bh(EdgeSoA ed, int E, BhDev t, B
     double sum = 0.0;
     unsigned long long visits = 0, approx = 0, direct = 0;
     bool stop = false;
-    for (int k = 0; k < 2 && !stop; ++k) {
-        const int a = fr.anc[2 * f + k];
+    for (int k = 0; k < BH_FD && !stop; ++k) {
+        const int a = fr.anc[BH_FD * f + k];
         if (a < 0) break;
-        const bool first = fr.first[2 * f + k] != 0;
+        const bool first = fr.first[BH_FD * f + k] != 0;
         if (first) ++visits;
         if (bh_admissible<P>(t, a, i1, i2, vOff, vEdge, mx, my, mz, theta, theta_t)) {
             if (first) {
@@ -378,59 +379,78 @@ extern "C" rc_status rc_energy_bh(rc_ctx* ctx, const double* xyz, const rc_param
     const int32_t* g = ctx->d_bh_i;
     t.start = g; t.end = g + nn; t.left = g + 2 * (size_t)nn; t.right = g + 3 * (size_t)nn;
     t.ord = g + 4 * (size_t)nn; t.pos = g + 4 * (size_t)nn + E;
-    // frontier: nodes of depth 2 and leaves above it, left-first, with their
-    // ancestors and whether the item is each ancestor's leftmost descendant
+    // frontier: nodes of depth BH_FD and leaves above it, left-first, with
+    // their ancestors and whether the item is each ancestor's leftmost
+    // descendant
     std::vector<int32_t> fnode, fanc, ffirst;
     {
-        struct It { int n, d, a0, a1; };
-        std::vector<It> st{{0, 0, -1, -1}};
+        struct It { int n, d; int a[BH_FD]; };
+        It root{0, 0, {}};
+        for (int k = 0; k < BH_FD; ++k) root.a[k] = -1;
+        std::vector<It> st{root};
         while (!st.empty()) {
             const It it = st.back();
             st.pop_back();
-            if (it.d == 2 || tree.left[it.n] < 0) {
+            if (it.d == BH_FD || tree.left[it.n] < 0) {
                 fnode.push_back(it.n);
-                fanc.push_back(it.a0);
-                fanc.push_back(it.a1);
-                ffirst.push_back(0);
-                ffirst.push_back(0);
+                for (int k = 0; k < BH_FD; ++k) {
+                    fanc.push_back(it.a[k]);
+                    ffirst.push_back(0);
+                }
                 continue;
             }
-            const int a0 = it.d == 0 ? it.n : it.a0, a1 = it.d == 1 ? it.n : it.a1;
-            st.push_back({tree.right[it.n], it.d + 1, a0, a1});
-            st.push_back({tree.left[it.n], it.d + 1, a0, a1});
+            It ch = it;
+            ch.a[it.d] = it.n;
+            ch.d = it.d + 1;
+            ch.n = tree.right[it.n];
+            st.push_back(ch);
+            ch.n = tree.left[it.n];
+            st.push_back(ch);
         }
         // leftmost frontier descendant of each ancestor: its first item in order
         std::vector<char> seen(nn, 0);
         for (size_t f = 0; f < fnode.size(); ++f)
-            for (int k = 0; k < 2; ++k) {
-                const int a = fanc[2 * f + k];
+            for (int k = 0; k < BH_FD; ++k) {
+                const int a = fanc[BH_FD * f + k];
                 if (a >= 0 && !seen[a]) {
                     seen[a] = 1;
-                    ffirst[2 * f + k] = 1;
+                    ffirst[BH_FD * f + k] = 1;
                 }
             }
     }
     const int nf = (int)fnode.size();
     // device copies of the frontier: after the tree arrays in the int buffer
-    if (ctx->bh_front_cap < 5 * (size_t)nf) {
+    const size_t nfr = (1 + 2 * (size_t)BH_FD) * nf;
+    if (ctx->bh_front_cap < nfr) {
         if (ctx->d_bh_front) (void)hipFree(ctx->d_bh_front);
         ctx->d_bh_front = nullptr;
         ctx->bh_front_cap = 0;
-        RC_HIP(hipMalloc((void**)&ctx->d_bh_front, sizeof(int32_t) * 5 * (size_t)nf));
-        ctx->bh_front_cap = 5 * (size_t)nf;
+        RC_HIP(hipMalloc((void**)&ctx->d_bh_front, sizeof(int32_t) * nfr));
+        ctx->bh_front_cap = nfr;
     }
     std::vector<int32_t> hfr;
-    hfr.reserve(5 * (size_t)nf);
+    hfr.reserve(nfr);
     hfr.insert(hfr.end(), fnode.begin(), fnode.end());
     hfr.insert(hfr.end(), fanc.begin(), fanc.end());
     hfr.insert(hfr.end(), ffirst.begin(), ffirst.end());
     RC_HIP(hipMemcpyAsync(ctx->d_bh_front, hfr.data(), sizeof(int32_t) * hfr.size(),
                           hipMemcpyHostToDevice, ctx->stream));
-    BhFront fr{ctx->d_bh_front, ctx->d_bh_front + nf, ctx->d_bh_front + 3 * (size_t)nf, nf};
-    // item sums [nf][E] and per-edge sums [E]: the gradient partial buffer
-    // holds 6 ntiles Epad >= 5 E doubles (nf <= 4)
+    BhFront fr{ctx->d_bh_front, ctx->d_bh_front + nf,
+                ctx->d_bh_front + (1 + (size_t)BH_FD) * nf, nf};
+    // item sums [nf][E] and per-edge sums [E] (nf <= 2^BH_FD)
+    const size_t need = ((size_t)nf + 1) * E;
     double* part = ctx->d_gpart;
-    double* per_edge = ctx->d_gpart + (size_t)nf * E;
+    if (need > (size_t)ctx->ntiles * 6 * ctx->Epad) {
+        if (ctx->bh_part_cap < need) {
+            if (ctx->d_bh_part) (void)hipFree(ctx->d_bh_part);
+            ctx->d_bh_part = nullptr;
+            ctx->bh_part_cap = 0;
+            RC_HIP(hipMalloc((void**)&ctx->d_bh_part, sizeof(double) * need));
+            ctx->bh_part_cap = need;
+        }
+        part = ctx->d_bh_part;
+    }
+    double* per_edge = part + (size_t)nf * E;
     const dim3 grid(((size_t)nf * E + 255) / 256);
     if (p->alpha == 3.0 && p->beta == 6.0) {
         hipLaunchKernelGGL(k_energy_bh<Pow36>, grid, dim3(256), 0, ctx->stream, ctx->ed, E, t, fr,

@@ -138,6 +138,8 @@ struct rc_ctx {
     size_t bh_cap_d = 0, bh_cap_i = 0;
     int32_t* d_bh_front = nullptr;  // traversal work items (frontier of the tree)
     size_t bh_front_cap = 0;
+    double* d_bh_part = nullptr;    // work-item sums when the gradient partials are too small
+    size_t bh_part_cap = 0;
     // block-cluster-tree products (rc_bct_matvec, rc_bct_apply_b), grown on demand
     double* d_bct_f = nullptr;
     int32_t* d_bct_i = nullptr;
