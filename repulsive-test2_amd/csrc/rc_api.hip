@@ -233,7 +233,7 @@ static rc_status ensure_k_capacity(rc_ctx* ctx, int k) {
                        &ctx->d_ssol, &ctx->d_Minv, &ctx->d_Cg, &ctx->d_phi, &ctx->d_schur};
     const size_t sizes[] = {frows * Vp, nrhs * Vp, nrhs * Vp, (size_t)kc * 3 * Vp,
                             (size_t)kc * (kc + 1), 3 * ((size_t)kc + 1), n, n * n, (size_t)kc,
-                            (size_t)kc + 8, kc > RC_KSMALL ? 2 * n * n + 2 * n : 1};
+                            (size_t)kc + 8, kc > RC_KSMALL ? 3 * n * n + 2 * n : 1};
     ctx->kcap = 0;
     for (int i = 0; i < 11; ++i) {
         if (*bufs[i]) (void)hipFree(*bufs[i]);

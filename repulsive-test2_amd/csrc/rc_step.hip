@@ -348,120 +348,134 @@ __global__ __launch_bounds__(256) void k_schur(const double* __restrict__ dots,
     if (t == 0 && bad) atomicOr(status, 4);
 }
 
-// The same system for n = k + 3 beyond the LDS bound: M and the right-hand
-// sides Z in a global workspace ws (2 n^2 + 2n doubles), one workgroup of
-// 1024 threads. Partial pivoting picks the largest |M[r][c]| with the lowest
-// row among equal values (the serial scan's choice); the back substitution
-// runs column-oriented (x_i of every column, then its products removed from
-// the rows above), parallel over rows and columns.
-__global__ __launch_bounds__(1024) void k_schur_global(const double* __restrict__ dots,
-                                                       const double* __restrict__ bsum, int k,
-                                                       int V, double* __restrict__ sol,
-                                                       double* __restrict__ Minv,
-                                                       double* __restrict__ ws,
-                                                       int32_t* __restrict__ status) {
-    __shared__ double bv[1024];
-    __shared__ int bi[1024];
-    __shared__ int piv_s, bad;
+// The same system for n = k + 3 beyond the LDS bound, in a global workspace
+// (M n x n, the right-hand sides Z n x (n + 1), the solution X n x (n + 1)):
+// filled by k_schur_fill, then per column c a pivot step (one workgroup: the
+// largest |M[r][c]|, r >= c, lowest row on ties, and the row swap) and an
+// elimination step over the rows below (all workgroups), then per row i,
+// bottom up, one back-substitution step (x_i = Z_i / M_ii into X, its products
+// removed from the rows above) — the partial-pivot LU of k_schur with the
+// work of each step spread over the device.
+__global__ void k_schur_fill(const double* __restrict__ dots, const double* __restrict__ bsum,
+                             int k, int V, double* __restrict__ ws) {
     const int n = k + 3, w = n + 1;
-    double* M = ws;                   // n x n
-    double* Z = ws + (size_t)n * n;   // n x (n + 1)
-    const int t = threadIdx.x;
+    double* M = ws;
+    double* Z = ws + (size_t)n * n;
     const double* CN = bsum;
     const double* Nr = bsum + 3 * k;
-    for (int q = t; q < n * n; q += 1024) {
-        const int i = q / n, j = q % n;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < (size_t)n * n) {
+        const int i = (int)(q / n), j = (int)(q % n);
         double v = 0.0;
-        if (i < k && j < k) v = dots[i * (k + 1) + 1 + j];
+        if (i < k && j < k) v = dots[(size_t)i * (k + 1) + 1 + j];
         else if (i < k) v = -CN[i * 3 + (j - k)] / V;
         else if (j < k) v = CN[j * 3 + (i - k)];
         M[q] = v;
-    }
-    for (int q = t; q < n * w; q += 1024) {
-        const int i = q / w, j = q % w;
+    } else if (q < (size_t)n * n + (size_t)n * w) {
+        const size_t e = q - (size_t)n * n;
+        const int i = (int)(e / w), j = (int)(e % w);
         double v;
-        if (j == 0) v = (i < k) ? dots[i * (k + 1)] : Nr[i - k];
+        if (j == 0) v = (i < k) ? dots[(size_t)i * (k + 1)] : Nr[i - k];
         else v = (i == j - 1) ? 1.0 : 0.0;
-        Z[q] = v;
+        Z[e] = v;
     }
-    if (t == 0) bad = 0;
+}
+
+__global__ __launch_bounds__(1024) void k_schur_pivot(int n, int c, double* __restrict__ ws,
+                                                      int32_t* __restrict__ status) {
+    __shared__ double bv[1024];
+    __shared__ int bi[1024];
+    const int w = n + 1, t = threadIdx.x;
+    double* M = ws;
+    double* Z = ws + (size_t)n * n;
+    double best = -1.0;
+    int bidx = c;
+    for (int r = c + t; r < n; r += 1024) {
+        const double a = fabs(M[(size_t)r * n + c]);
+        if (a > best) {
+            best = a;
+            bidx = r;
+        }
+    }
+    bv[t] = best;
+    bi[t] = bidx;
     __syncthreads();
-    for (int c = 0; c < n; ++c) {
-        // pivot: max |M[r][c]|, r >= c, lowest r on ties
-        double best = -1.0;
-        int bidx = c;
-        for (int r = c + t; r < n; r += 1024) {
-            const double a = fabs(M[(size_t)r * n + c]);
-            if (a > best) {
-                best = a;
-                bidx = r;
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            const double o = bv[t + s];
+            const int oi = bi[t + s];
+            if (o > bv[t] || (o == bv[t] && oi < bi[t])) {
+                bv[t] = o;
+                bi[t] = oi;
             }
-        }
-        bv[t] = best;
-        bi[t] = bidx;
-        __syncthreads();
-        for (int s = 512; s > 0; s >>= 1) {
-            if (t < s) {
-                const double o = bv[t + s];
-                const int oi = bi[t + s];
-                if (o > bv[t] || (o == bv[t] && oi < bi[t])) {
-                    bv[t] = o;
-                    bi[t] = oi;
-                }
-            }
-            __syncthreads();
-        }
-        if (t == 0) {
-            piv_s = bi[0];
-            if (!(bv[0] > 0.0)) bad = 1;
-        }
-        __syncthreads();
-        const int pr = piv_s;
-        if (pr != c) {
-            for (int j = t; j < n; j += 1024) {
-                const double tmp = M[(size_t)c * n + j];
-                M[(size_t)c * n + j] = M[(size_t)pr * n + j];
-                M[(size_t)pr * n + j] = tmp;
-            }
-            for (int j = t; j < w; j += 1024) {
-                const double tmp = Z[(size_t)c * w + j];
-                Z[(size_t)c * w + j] = Z[(size_t)pr * w + j];
-                Z[(size_t)pr * w + j] = tmp;
-            }
-        }
-        __syncthreads();
-        const double pv = M[(size_t)c * n + c];
-        const int nr = n - c - 1, wm = n - c - 1, wz = w;
-        for (long q = t; q < (long)nr * (wm + wz); q += 1024) {
-            const int r = c + 1 + (int)(q / (wm + wz)), j = (int)(q % (wm + wz));
-            const double f = (pv != 0.0) ? M[(size_t)r * n + c] / pv : 0.0;
-            if (j < wm)
-                M[(size_t)r * n + c + 1 + j] -= f * M[(size_t)c * n + c + 1 + j];
-            else
-                Z[(size_t)r * w + (j - wm)] -= f * Z[(size_t)c * w + (j - wm)];
         }
         __syncthreads();
     }
-    // back substitution, column-oriented
-    for (int i = n - 1; i >= 0; --i) {
-        const double d = M[(size_t)i * n + i];
+    const int pr = bi[0];
+    if (t == 0 && !(bv[0] > 0.0)) atomicOr(status, 4);
+    if (pr != c) {
+        for (int j = t; j < n; j += 1024) {
+            const double tmp = M[(size_t)c * n + j];
+            M[(size_t)c * n + j] = M[(size_t)pr * n + j];
+            M[(size_t)pr * n + j] = tmp;
+        }
         for (int j = t; j < w; j += 1024) {
-            const double x = (d != 0.0) ? Z[(size_t)i * w + j] / d : 0.0;
-            Z[(size_t)i * w + j] = x;
+            const double tmp = Z[(size_t)c * w + j];
+            Z[(size_t)c * w + j] = Z[(size_t)pr * w + j];
+            Z[(size_t)pr * w + j] = tmp;
         }
-        __syncthreads();
-        for (long q = t; q < (long)i * w; q += 1024) {
-            const int r = (int)(q / w), j = (int)(q % w);
-            Z[(size_t)r * w + j] -= M[(size_t)r * n + i] * Z[(size_t)i * w + j];
-        }
-        __syncthreads();
     }
-    for (int q = t; q < n; q += 1024) sol[q] = Z[(size_t)q * w];
-    for (int q = t; q < n * n; q += 1024) {
-        const int i = q / n, j = q % n;
-        Minv[q] = Z[(size_t)i * w + 1 + j];
+}
+
+// rows r = c + 1 + blockIdx.y; the threads stride over the trailing columns
+// of M and all columns of Z
+__global__ __launch_bounds__(256) void k_schur_elim(int n, int c, double* __restrict__ ws) {
+    const int w = n + 1;
+    double* M = ws;
+    double* Z = ws + (size_t)n * n;
+    const int r = c + 1 + (int)blockIdx.y;
+    const double pv = M[(size_t)c * n + c];
+    const double f = (pv != 0.0) ? M[(size_t)r * n + c] / pv : 0.0;
+    const int wm = n - c - 1;
+    for (int j = (int)(blockIdx.x * blockDim.x + threadIdx.x); j < wm + w;
+         j += (int)(gridDim.x * blockDim.x)) {
+        if (j < wm)
+            M[(size_t)r * n + c + 1 + j] -= f * M[(size_t)c * n + c + 1 + j];
+        else
+            Z[(size_t)r * w + (j - wm)] -= f * Z[(size_t)c * w + (j - wm)];
     }
-    if (t == 0 && bad) atomicOr(status, 4);
+}
+
+// back substitution step i: X_i = Z_i / M_ii (row blockIdx.y == i), and the
+// rows r < i (blockIdx.y == r) lose M_ri X_i
+__global__ __launch_bounds__(256) void k_schur_back(int n, int i, double* __restrict__ ws) {
+    const int w = n + 1;
+    const double* M = ws;
+    double* Z = ws + (size_t)n * n;
+    double* X = Z + (size_t)n * w;
+    const int r = (int)blockIdx.y;
+    const double d = M[(size_t)i * n + i];
+    const double m = (r < i) ? M[(size_t)r * n + i] : 0.0;
+    for (int j = (int)(blockIdx.x * blockDim.x + threadIdx.x); j < w;
+         j += (int)(gridDim.x * blockDim.x)) {
+        const double x = (d != 0.0) ? Z[(size_t)i * w + j] / d : 0.0;
+        if (r == i)
+            X[(size_t)i * w + j] = x;
+        else
+            Z[(size_t)r * w + j] -= m * x;
+    }
+}
+
+__global__ void k_schur_out(int n, const double* __restrict__ ws, double* __restrict__ sol,
+                            double* __restrict__ Minv) {
+    const int w = n + 1;
+    const double* X = ws + (size_t)n * n + (size_t)n * w;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < (size_t)n) sol[q] = X[q * w];
+    if (q < (size_t)n * n) {
+        const size_t i = q / n, j = q % n;
+        Minv[q] = X[i * w + 1 + j];
+    }
 }
 
 // g~_c[v] = X_c[v] - sum_q lambda_q Y_{q,c}[v] + t_c / V   (stride Vp)
@@ -851,13 +865,52 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
                                                 double* solv, double* sh) {
     const int t = threadIdx.x;
     const int n = k + 3;
-    for (int i = t; i < n; i += CT) {
-        double s = 0.0;
-        for (int j = 0; j < k; ++j) s += Minv[i * n + j] * phi[j];
-        solv[i] = s;
+    if (k > RC_KSMALL) {
+        // one wave per row of M^{-1}, its lanes over the columns (coalesced)
+        const int lane = t & 63;
+        for (int i = t >> 6; i < n; i += CT / 64) {
+            double s = 0.0;
+            for (int j = lane; j < k; j += 64) s += Minv[(size_t)i * n + j] * phi[j];
+            s = wsum(s);
+            if (lane == 0) solv[i] = s;
+        }
+    } else {
+        for (int i = t; i < n; i += CT) {
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += Minv[i * n + j] * phi[j];
+            solv[i] = s;
+        }
     }
     __syncthreads();
     double nf = 0.0;
+    if (k > RC_KSMALL) {
+        // a vertex's three components per thread, four partial sums over
+        // the rows each: twelve independent loads in flight per thread
+        for (int v = t; v < V; v += CT) {
+            double s[3][4] = {};
+            int p = 0;
+            for (; p + 4 <= k; p += 4)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double lam = solv[p + u];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        s[c][u] += lam * Y[(size_t)(3 * (p + u) + c) * Vp + v];
+                }
+            for (; p < k; ++p)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) s[c][0] += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double x = solv[k + c] / V - ((s[c][0] + s[c][1]) + (s[c][2] + s[c][3]));
+                if (!(x - x == 0.0)) nf = 1.0;
+                cur[(size_t)c * V + v] += x;
+            }
+        }
+        nf = block_sum(nf, sh);
+        __syncthreads();
+        return nf > 0.0;
+    }
     for (int q = t; q < 3 * V; q += CT) {
         const int c = q / V, v = q % V;
         double x = solv[k + c] / V;
@@ -1187,8 +1240,26 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
     const int n = k + 3;
     const size_t shm = sizeof(double) * ((size_t)n * n + (size_t)n * (n + 1)) + sizeof(int) * n;
     if (k > RC_KSMALL) {
-        hipLaunchKernelGGL(k_schur_global, dim3(1), dim3(1024), 0, ctx->stream, d_dots, d_bsum, k,
-                           ctx->V, d_sol, d_Minv, ctx->d_schur, ctx->d_status);
+        const size_t nfill = (size_t)n * n + (size_t)n * (n + 1);
+        hipLaunchKernelGGL(k_schur_fill, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0,
+                           ctx->stream, d_dots, d_bsum, k, ctx->V, ctx->d_schur);
+        RC_CHECK_LAUNCH();
+        for (int c = 0; c < n; ++c) {
+            hipLaunchKernelGGL(k_schur_pivot, dim3(1), dim3(1024), 0, ctx->stream, n, c,
+                               ctx->d_schur, ctx->d_status);
+            if (c + 1 < n) {
+                const int cols = (n - c - 1) + (n + 1);
+                hipLaunchKernelGGL(k_schur_elim, dim3((cols + 255) / 256, n - c - 1), dim3(256), 0,
+                                   ctx->stream, n, c, ctx->d_schur);
+            }
+        }
+        RC_CHECK_LAUNCH();
+        for (int i = n - 1; i >= 0; --i)
+            hipLaunchKernelGGL(k_schur_back, dim3((n + 1 + 255) / 256, i + 1), dim3(256), 0,
+                               ctx->stream, n, i, ctx->d_schur);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_schur_out, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, n, (const double*)ctx->d_schur, d_sol, d_Minv);
         RC_CHECK_LAUNCH();
     } else if (k > 0) {
         hipLaunchKernelGGL(k_schur, dim3(1), dim3(256), shm, ctx->stream, d_dots, d_bsum, k, ctx->V,
