@@ -61,5 +61,5 @@ struct LSCtl {
     int32_t proj_iters_acc;
     int32_t need_solve;   // reassemble mode: the trial waits for a solve at cur
     int32_t proj_it;      // reassemble mode: corrections applied to the trial so far
-    int32_t _pad;
+    int32_t corr_bad;     // a correction applied by k_corr_apply was not finite
 };

@@ -1015,7 +1015,10 @@ __global__ __launch_bounds__(CT) void k_ls_project(
 // phase 1 applies the correction with the Y and M^{-1} just factored at cur
 // (a singular system there fails the trial, as the reference's solve throws)
 // and checks again. Not converged with corrections left: need_solve = 1 and
-// Phi is kept in phi_g for the next launch. Single workgroup.
+// Phi is kept in phi_g for the next launch. Single workgroup. The frozen
+// projection of a set beyond the LDS Schur bound runs the same phases with
+// the correction (Y and M^{-1} of gamma0) applied over the device by
+// k_corr_solv / k_corr_apply between them; phase 2 only checks.
 __global__ __launch_bounds__(CT) void k_ls_project_re(
     int phase, LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
     DevPen pen, const double* __restrict__ pos0, const double* __restrict__ dir,
@@ -1040,14 +1043,23 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
     const int maxit = ctl->max_iter;
     int it = 0;
     if (phase == 0) {
+        if (t == 0) ctl->corr_bad = 0;
         trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
     } else {
         it = ctl->proj_it;
-        bool bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
-        if (!bad) {
-            for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
+        bool bad;
+        if (phase == 2) {
+            // frozen correction already applied to cur by k_corr_solv / k_corr_apply
+            bad = ctl->corr_bad != 0;
             __syncthreads();
-            bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
+            if (t == 0) ctl->corr_bad = 0;
+        } else {
+            bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
+            if (!bad) {
+                for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
+                __syncthreads();
+                bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
+            }
         }
         if (bad) {
             if (t == 0) ctl->need_solve = 0;
@@ -1070,6 +1082,54 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
         ctl->proj_it = it;
         ctl->need_solve = 1;
         ctl->skip_energy = 1;
+    }
+}
+
+// Frozen correction spread over the device (sets beyond the LDS Schur bound):
+// solv = M^{-1} [Phi; 0] with a wave per row, then x = -Y lambda + N t / V
+// added to cur, a workgroup per (64 vertices, component) whose 16 waves take
+// the rows p = w mod 16 (four partial sums each, combined in a fixed order).
+__global__ __launch_bounds__(256) void k_corr_solv(int k, const double* __restrict__ Minv,
+                                                   const double* __restrict__ phi_g,
+                                                   double* __restrict__ solv) {
+    const int n = k + 3;
+    const int lane = threadIdx.x & 63;
+    const int i = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (i >= n) return;  // whole waves
+    double s = 0.0;
+    for (int j = lane; j < k; j += 64) s += Minv[(size_t)i * n + j] * phi_g[j];
+    s = wsum(s);
+    if (lane == 0) solv[i] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_corr_apply(int k, const double* __restrict__ Y,
+                                                     const double* __restrict__ solv, int V,
+                                                     int Vp, double* __restrict__ cur,
+                                                     LSCtl* ctl) {
+    __shared__ double red[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = blockIdx.y;
+    const int v = (int)blockIdx.x * 64 + lane;
+    const bool live = v < V;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (live) {
+        int p = w;
+        for (; p + 48 < k; p += 64) {
+            s0 += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+            s1 += solv[p + 16] * Y[(size_t)(3 * (p + 16) + c) * Vp + v];
+            s2 += solv[p + 32] * Y[(size_t)(3 * (p + 32) + c) * Vp + v];
+            s3 += solv[p + 48] * Y[(size_t)(3 * (p + 48) + c) * Vp + v];
+        }
+        for (; p < k; p += 16) s0 += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+    }
+    red[w][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (w == 0 && live) {
+        double s = 0.0;
+        for (int q = 0; q < 16; ++q) s += red[q][lane];
+        const double x = solv[k + c] / V - s;
+        if (!(x - x == 0.0)) atomicOr(&ctl->corr_bad, 1);
+        cur[(size_t)c * V + v] += x;
     }
 }
 
@@ -1330,6 +1390,19 @@ rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const De
                        ctx->d_ell0, pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
                        ctx->d_vOff, ctx->d_vEdge, p->alpha, p->epsilon, ctx->d_phi,
                        (const int32_t*)ctx->d_status);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv,
+                         const double* phi_g, double* solv, double* cur, LSCtl* d_ctl) {
+    const int n = k + 3;
+    const int Vp = ctx->nblk * RC_NB;
+    hipLaunchKernelGGL(k_corr_solv, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, k, Minv, phi_g,
+                       solv);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_corr_apply, dim3((ctx->V + 63) / 64, 3), dim3(1024), 0, ctx->stream, k, Y,
+                       (const double*)solv, ctx->V, Vp, cur, d_ctl);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
