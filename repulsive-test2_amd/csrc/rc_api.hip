@@ -320,7 +320,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_Adiag, (size_t)Vp);
     ALLOC(ctx->d_symv, (size_t)ctx->nblk * ctx->nblk * 3 * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
-    ALLOC(ctx->d_flags, ctx->nblk);
+    ALLOC(ctx->d_flags, (size_t)ctx->nblk * RC_BWD_MAXCH);
     ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
     ALLOC(ctx->d_status, 4);
     ALLOC(ctx->d_tmp, 3 * (size_t)Vp);
@@ -342,7 +342,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         return fail(RC_HIP_ERROR);
     }
     ctx->host_stage_bytes = 4096;
-    he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk);
+    he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk * RC_BWD_MAXCH);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
     // the inverse's zero blocks above the diagonal are never written
@@ -368,6 +368,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
                    " co-resident workgroups, device holds " + std::to_string(cores);
         return fail(RC_UNSUPPORTED);
     }
+    ctx->bwd_chunks = std::max(1, std::min(RC_BWD_MAXCH, cores / ctx->nblk));
     *out = ctx;
     return RC_OK;
 }

@@ -1051,9 +1051,10 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
     publish(flags + r, epoch);
 }
 
-// Backward substitution L^T X = Y (FP64 factor) on nrhs <= RHS_CHUNK columns
-// of Y (column-major, leading dimension ldb), in place; block b solves the
-// 64-row block r = nblk-1-b. 8 waves, wave w owns the 16 x 16 output tile
+// Backward substitution L^T X = Y (FP64 factor) on nrhs columns of Y
+// (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
+// grid row y (flags row y: nblk flags); block (b, y) solves the 64-row block
+// r = nblk-1-b of its chunk. 8 waves, wave w owns the 16 x 16 output tile
 // (rows 16 (w & 3), columns 16 (w >> 2)).
 //   X_r = L_rr^-T (Y_r - sum_{c > r} L_cr^T X_c)
 //       = Z_r - P_r^T X_{r+1},  Z_r = L_rr^-T (Y_r - sum_{c >= r+2} L_cr^T X_c),
@@ -1070,6 +1071,10 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
                                                   int32_t* __restrict__ status) {
     __shared__ double Lt[NB][NB + 1];         // A operand L_cr^T (Lt[m][i]); staging at start
     __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (X_c, the accumulator for Z)
+    // chunk blockIdx.y: columns 32 y .. of the launch's nrhs, its own flag row
+    B += (size_t)blockIdx.y * RHS_CHUNK * ldb;
+    nrhs = min(RHS_CHUNK, nrhs - (int)blockIdx.y * RHS_CHUNK);
+    flags += (size_t)blockIdx.y * nblk;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int r = nblk - 1 - (int)blockIdx.x;
@@ -1409,10 +1414,14 @@ static rc_status solve_impl(rc_ctx* ctx, const T* F, const T* Linv, double* d_B,
 // right-hand-side rows.
 rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
     const int Vp = ctx->nblk * NB;
-    for (int c0 = 0; c0 < nrhs; c0 += RHS_CHUNK) {
-        const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
+    // bwd_chunks chunks of RHS_CHUNK columns per launch: nblk x bwd_chunks
+    // workgroups, all co-resident (the waits need every producer running)
+    const int per = RHS_CHUNK * ctx->bwd_chunks;
+    for (int c0 = 0; c0 < nrhs; c0 += per) {
+        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
+        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
         const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_bwd, dim3(ctx->nblk), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL(k_trsm_bwd, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
                            (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
                            d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
                            ctx->d_status);

@@ -14,6 +14,7 @@
 #define RC_VTILE 32         // vertices per ownership tile of the Gram assembly
 #define RC_RMAX 40          // max incident edges of one vertex tile (assembly LDS bound)
 #define RC_NB 64            // Cholesky / triangular-solve block size
+#define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
 #define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
 
 // Structure-of-arrays per-edge geometry refreshed by k_edge_prep for the
@@ -36,6 +37,7 @@ struct rc_ctx {
     int V = 0, E = 0, Epad = 0, ntiles = 0, npairs = 0;
     int nvt = 0, ngram = 0;   // vertex tiles, assembly tile pairs
     int nblk = 0;             // Cholesky blocks of RC_NB rows (always even)
+    int bwd_chunks = 1;       // right-hand-side chunks per backward-sweep launch
     int maxdeg = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
