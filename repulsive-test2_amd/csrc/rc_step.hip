@@ -349,13 +349,13 @@ __global__ __launch_bounds__(256) void k_schur(const double* __restrict__ dots,
 }
 
 // The same system for n = k + 3 beyond the LDS bound, in a global workspace
-// (M n x n, the right-hand sides Z n x (n + 1), the solution X n x (n + 1)):
+// (M n x n, the right-hand sides Z n x (n + 1), U^T in an n x (n + 1) area):
 // filled by k_schur_fill, then per column c a pivot step (one workgroup: the
 // largest |M[r][c]|, r >= c, lowest row on ties, and the row swap) and an
-// elimination step over the rows below (all workgroups), then per row i,
-// bottom up, one back-substitution step (x_i = Z_i / M_ii into X, its products
-// removed from the rows above) — the partial-pivot LU of k_schur with the
-// work of each step spread over the device.
+// elimination step over the rows below (all workgroups), then the back
+// substitution of every right-hand-side column in one launch — the
+// partial-pivot LU of k_schur with the work of each step spread over the
+// device.
 __global__ void k_schur_fill(const double* __restrict__ dots, const double* __restrict__ bsum,
                              int k, int V, double* __restrict__ ws) {
     const int n = k + 3, w = n + 1;
@@ -446,30 +446,41 @@ __global__ __launch_bounds__(256) void k_schur_elim(int n, int c, double* __rest
     }
 }
 
-// back substitution step i: X_i = Z_i / M_ii (row blockIdx.y == i), and the
-// rows r < i (blockIdx.y == r) lose M_ri X_i
-__global__ __launch_bounds__(256) void k_schur_back(int n, int i, double* __restrict__ ws) {
-    const int w = n + 1;
+// back substitution: U^T (the upper triangle of the eliminated M) into the
+// X area, then one workgroup per right-hand-side column j of Z, bottom up:
+// x_i = z_i / U_ii, z_r -= U_ri x_i for r < i (the column in LDS, each row
+// owned by one thread); the solution replaces the column of Z.
+__global__ void k_schur_ut(int n, double* __restrict__ ws) {
     const double* M = ws;
+    double* Ut = ws + (size_t)n * n + (size_t)n * (n + 1);
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)n * n) return;
+    const size_t i = q / n, r = q % n;
+    if (r <= i) Ut[q] = M[r * n + i];
+}
+
+__global__ __launch_bounds__(256) void k_schur_back_cols(int n, double* __restrict__ ws) {
+    __shared__ double z[RC_KMAX + 3];
+    __shared__ double xs[RC_KMAX + 3];
+    const int w = n + 1, t = threadIdx.x, j = blockIdx.x;
     double* Z = ws + (size_t)n * n;
-    double* X = Z + (size_t)n * w;
-    const int r = (int)blockIdx.y;
-    const double d = M[(size_t)i * n + i];
-    const double m = (r < i) ? M[(size_t)r * n + i] : 0.0;
-    for (int j = (int)(blockIdx.x * blockDim.x + threadIdx.x); j < w;
-         j += (int)(gridDim.x * blockDim.x)) {
-        const double x = (d != 0.0) ? Z[(size_t)i * w + j] / d : 0.0;
-        if (r == i)
-            X[(size_t)i * w + j] = x;
-        else
-            Z[(size_t)r * w + j] -= m * x;
+    const double* Ut = Z + (size_t)n * w;
+    for (int r = t; r < n; r += 256) z[r] = Z[(size_t)r * w + j];
+    __syncthreads();
+    for (int i = n - 1; i >= 0; --i) {
+        const double d = Ut[(size_t)i * n + i];
+        const double x = (d != 0.0) ? z[i] / d : 0.0;
+        for (int r = t; r < i; r += 256) z[r] -= Ut[(size_t)i * n + r] * x;
+        if (t == 0) xs[i] = x;
+        __syncthreads();
     }
+    for (int r = t; r < n; r += 256) Z[(size_t)r * w + j] = xs[r];
 }
 
 __global__ void k_schur_out(int n, const double* __restrict__ ws, double* __restrict__ sol,
                             double* __restrict__ Minv) {
     const int w = n + 1;
-    const double* X = ws + (size_t)n * n + (size_t)n * w;
+    const double* X = ws + (size_t)n * n;  // the solved columns of Z
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < (size_t)n) sol[q] = X[q * w];
     if (q < (size_t)n * n) {
@@ -1314,9 +1325,11 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
             }
         }
         RC_CHECK_LAUNCH();
-        for (int i = n - 1; i >= 0; --i)
-            hipLaunchKernelGGL(k_schur_back, dim3((n + 1 + 255) / 256, i + 1), dim3(256), 0,
-                               ctx->stream, n, i, ctx->d_schur);
+        hipLaunchKernelGGL(k_schur_ut, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, n, ctx->d_schur);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_schur_back_cols, dim3(n + 1), dim3(256), 0, ctx->stream, n,
+                           ctx->d_schur);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_schur_out, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256), 0,
                            ctx->stream, n, (const double*)ctx->d_schur, d_sol, d_Minv);
