@@ -242,6 +242,58 @@ __global__ __launch_bounds__(256) void k_dots(const double* __restrict__ L, size
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// The same products for many rows (sets beyond the LDS Schur bound): a
+// workgroup per 8 x 8 tile of (p, q), each loaded entry used eight times;
+// per-thread partial sums over j = t mod 256, then wave and workgroup sums.
+__global__ __launch_bounds__(256) void k_dots_tile(const double* __restrict__ L, size_t ls,
+                                                   int np, const double* __restrict__ R,
+                                                   size_t rs, int nr, size_t len,
+                                                   double* __restrict__ out) {
+    __shared__ double red[4][64];
+    const int ntq = (nr + 7) / 8;
+    const int p0 = ((int)blockIdx.x / ntq) * 8, q0 = ((int)blockIdx.x % ntq) * 8;
+    const double* a[8];
+    const double* b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        // rows past the end read the last row; their sums are not stored
+        a[i] = L + (size_t)min(p0 + i, np - 1) * ls;
+        b[i] = R + (size_t)min(q0 + i, nr - 1) * rs;
+    }
+    double acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) acc[i][m] = 0.0;
+    for (size_t j = threadIdx.x; j < len; j += 256) {
+        double av[8], bv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            av[i] = a[i][j];
+            bv[i] = b[i][j];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int m = 0; m < 8; ++m) acc[i][m] = fma(av[i], bv[m], acc[i][m]);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double v = wsum(acc[i][m]);
+            if (lane == 0) red[w][i * 8 + m] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int i = threadIdx.x >> 3, m = threadIdx.x & 7;
+        const double v = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                         (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        if (p0 + i < np && q0 + m < nr) out[(size_t)(p0 + i) * nr + q0 + m] = v;
+    }
+}
+
 // Column sums of the coordinate blocks of C rows: CN[p][c] = sum_v C[p][c*Vp+v];
 // also Nr[c] = sum_v dE[c*Vp + v] at CN[k*3 + c].
 __global__ __launch_bounds__(256) void k_block_sums(const double* __restrict__ C,
@@ -1301,8 +1353,14 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
     if (k > 0) {
         // dots[p][0] = <C_p, X_r>, dots[p][1+q] = <C_p, Y_q>; X columns are
         // consecutive Vp blocks, so X_r and Y_q are contiguous 3Vp vectors.
-        hipLaunchKernelGGL(k_dots, dim3(k * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, len, X,
-                           len, k + 1, len, d_dots);
+        if (k > RC_KSMALL) {
+            const unsigned tiles = (unsigned)(((k + 7) / 8) * ((k + 1 + 7) / 8));
+            hipLaunchKernelGGL(k_dots_tile, dim3(tiles), dim3(256), 0, ctx->stream,
+                               (const double*)ctx->d_C, len, k, X, len, k + 1, len, d_dots);
+        } else {
+            hipLaunchKernelGGL(k_dots, dim3(k * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, len,
+                               X, len, k + 1, len, d_dots);
+        }
         RC_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(k_block_sums, dim3(3 * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, dE, k,
