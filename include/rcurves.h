@@ -84,6 +84,15 @@ typedef struct {
     int32_t gradient_mode;      /* dE by RC_MODE_ANALYTICAL (default) | RC_MODE_FINITE_DIFF
                                  * (SobolevStepOptions.mode, optimizer.ts:96,278-282) */
     double fd_h;                /* forward-difference step h, default 1e-6 (optimizer.ts:18) */
+    int32_t saddle_refine;      /* refinement steps of the saddle solution with residuals
+                                 * formed in double-double (compensated) arithmetic, default 1;
+                                 * 0 = off. No counterpart in the reference, whose FP64 LDL^T
+                                 * stops at its own rounding: kappa(A) ~ 1e9 at V = 8192 leaves
+                                 * an FP64 solve ~2e-8 from the exact solution of the assembled
+                                 * system, the refined one at its rounding. */
+    int32_t _pad3;
+    double saddle_refine_tol;   /* mixed form: PCG relative residual of each refinement
+                                 * correction, default 1e-6 */
 } rc_step_opts;
 
 /* SobolevStepStats (optimizer.ts:196-202) + line-search diagnostics. */
@@ -111,6 +120,11 @@ typedef struct {
     int32_t reason;
     int32_t trials;
     int32_t refine_iters;
+    double saddle_ir_residual;   /* compensated ||rhs - K z|| / max(1, ||rhs||) of the returned
+                                  * solution (0 when saddle_refine = 0) */
+    double saddle_ir_correction; /* ||dx|| / ||g~|| of the last refinement correction */
+    int32_t saddle_ir_steps;     /* refinement corrections applied */
+    int32_t _pad2;
 } rc_step_stats;
 
 /* Per-phase milliseconds, keys of phaseTimings.ts:9-20 (sub-phases overlap). */

@@ -13,6 +13,31 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Threads of the pair loops. The per-pair terms of a block of rows I are
+ * computed in parallel; they are then accumulated in the reference's (I, J)
+ * order — the energy by one thread, the gradient and the Gram matrices by
+ * threads that each own a range of output rows and walk the pairs in order
+ * applying only their rows' updates. Every output entry therefore receives
+ * the same operands in the same order as the sequential loop: results are
+ * bit-identical for any thread count. */
+static int g_threads = 0; /* 0: the OpenMP default (OMP_NUM_THREADS) */
+
+void rco_set_threads(int n) { g_threads = n < 0 ? 0 : n; }
+
+int rco_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+#define RCO_IB 64 /* rows I per block of the pair loops */
 
 /* ------------------------------------------------------------------------ */
 /* JavaScript Math.pow / ** semantics as executed by the reference's engine
@@ -72,43 +97,65 @@ void rco_default_opts(rco_step_opts* o) {
 
 /* ------------------------------------------------------------------------ */
 /* calculateEnergy — tangentPointEnergy.ts:50-111. Ordered disjoint pairs,
- * J ascending (calculateDisjointPairs :33-47), eps after every norm, /2. */
+ * J ascending (calculateDisjointPairs :33-47), eps after every norm, /2.
+ * energy_row writes the pair terms 0.25 ell_I^(1-alpha) ell_J sumK of row I
+ * (term[J], disjoint J only); the caller adds them in (I, J) order. */
+static void energy_row(const double* v, const int32_t* edges, int E, int I, double alpha,
+                       double beta, double eps, double* term) {
+    const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
+    const double eIx = v[3 * i2] - v[3 * i1];
+    const double eIy = v[3 * i2 + 1] - v[3 * i1 + 1];
+    const double eIz = v[3 * i2 + 2] - v[3 * i1 + 2];
+    const double ell_I = sqrt(eIx * eIx + eIy * eIy + eIz * eIz) + eps;
+    for (int J = 0; J < E; J++) {
+        if (!disjoint(edges, I, J)) continue;
+        const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
+        const double eJx = v[3 * j2] - v[3 * j1];
+        const double eJy = v[3 * j2 + 1] - v[3 * j1 + 1];
+        const double eJz = v[3 * j2 + 2] - v[3 * j1 + 2];
+        const double ell_J = sqrt(eJx * eJx + eJy * eJy + eJz * eJz) + eps;
+        const int is[2] = {i1, i2}, js[2] = {j1, j2};
+        double sumK = 0.0;
+        for (int a = 0; a < 2; a++) {
+            for (int b = 0; b < 2; b++) {
+                const int i = is[a], j = js[b];
+                const double dx = v[3 * i] - v[3 * j];
+                const double dy = v[3 * i + 1] - v[3 * j + 1];
+                const double dz = v[3 * i + 2] - v[3 * j + 2];
+                const double d_norm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+                const double cx = eIy * dz - eIz * dy;
+                const double cy = eIz * dx - eIx * dz;
+                const double cz = eIx * dy - eIy * dx;
+                const double c_norm = sqrt(cx * cx + cy * cy + cz * cz) + eps;
+                sumK += js_pow(c_norm, alpha) / js_pow(d_norm, beta);
+            }
+        }
+        term[J] = 0.25 * js_pow(ell_I, 1 - alpha) * ell_J * sumK;
+    }
+}
+
+/* Pair terms of rows I0 .. I0+nI-1 into buf[r * E + J] (parallel). */
+static void energy_rows(const double* v, const int32_t* edges, int E, int I0, int nI,
+                        double alpha, double beta, double eps, double* buf) {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(rco_threads()) schedule(dynamic, 1)
+#endif
+    for (int r = 0; r < nI; r++) energy_row(v, edges, E, I0 + r, alpha, beta, eps, buf + (size_t)r * E);
+}
+
 double rco_energy(int V, int E, const double* v, const int32_t* edges,
                   double alpha, double beta, double eps) {
     (void)V;
     double total = 0.0;
-    for (int I = 0; I < E; I++) {
-        const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
-        const double eIx = v[3 * i2] - v[3 * i1];
-        const double eIy = v[3 * i2 + 1] - v[3 * i1 + 1];
-        const double eIz = v[3 * i2 + 2] - v[3 * i1 + 2];
-        const double ell_I = sqrt(eIx * eIx + eIy * eIy + eIz * eIz) + eps;
-        for (int J = 0; J < E; J++) {
-            if (!disjoint(edges, I, J)) continue;
-            const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
-            const double eJx = v[3 * j2] - v[3 * j1];
-            const double eJy = v[3 * j2 + 1] - v[3 * j1 + 1];
-            const double eJz = v[3 * j2 + 2] - v[3 * j1 + 2];
-            const double ell_J = sqrt(eJx * eJx + eJy * eJy + eJz * eJz) + eps;
-            const int is[2] = {i1, i2}, js[2] = {j1, j2};
-            double sumK = 0.0;
-            for (int a = 0; a < 2; a++) {
-                for (int b = 0; b < 2; b++) {
-                    const int i = is[a], j = js[b];
-                    const double dx = v[3 * i] - v[3 * j];
-                    const double dy = v[3 * i + 1] - v[3 * j + 1];
-                    const double dz = v[3 * i + 2] - v[3 * j + 2];
-                    const double d_norm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
-                    const double cx = eIy * dz - eIz * dy;
-                    const double cy = eIz * dx - eIx * dz;
-                    const double cz = eIx * dy - eIy * dx;
-                    const double c_norm = sqrt(cx * cx + cy * cy + cz * cz) + eps;
-                    sumK += js_pow(c_norm, alpha) / js_pow(d_norm, beta);
-                }
-            }
-            total += 0.25 * js_pow(ell_I, 1 - alpha) * ell_J * sumK;
-        }
+    double* buf = (double*)malloc(sizeof(double) * RCO_IB * (size_t)(E > 0 ? E : 1));
+    for (int I0 = 0; I0 < E; I0 += RCO_IB) {
+        const int nI = E - I0 < RCO_IB ? E - I0 : RCO_IB;
+        energy_rows(v, edges, E, I0, nI, alpha, beta, eps, buf);
+        for (int r = 0; r < nI; r++)
+            for (int J = 0; J < E; J++)
+                if (disjoint(edges, I0 + r, J)) total += buf[(size_t)r * E + J];
     }
+    free(buf);
     return total / 2;
 }
 
@@ -151,46 +198,29 @@ void rco_step(int V, int E, const double* v, const int32_t* edges, double alpha,
     free(g);
 }
 
-/* Same terms, Neumaier-compensated accumulation of the pair energies. */
+/* Same terms, Neumaier-compensated accumulation of the pair energies (in the
+ * same (I, J) order): the sum's rounding drops to O(u) instead of O(P u), so
+ * it separates the reference's left-to-right rounding from a device result
+ * summed in another order at large E. */
 double rco_energy_compensated(int V, int E, const double* v, const int32_t* edges,
                               double alpha, double beta, double eps) {
     (void)V;
     double s = 0.0, comp = 0.0;
-    for (int I = 0; I < E; I++) {
-        const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
-        const double eIx = v[3 * i2] - v[3 * i1];
-        const double eIy = v[3 * i2 + 1] - v[3 * i1 + 1];
-        const double eIz = v[3 * i2 + 2] - v[3 * i1 + 2];
-        const double ell_I = sqrt(eIx * eIx + eIy * eIy + eIz * eIz) + eps;
-        for (int J = 0; J < E; J++) {
-            if (!disjoint(edges, I, J)) continue;
-            const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
-            const double eJx = v[3 * j2] - v[3 * j1];
-            const double eJy = v[3 * j2 + 1] - v[3 * j1 + 1];
-            const double eJz = v[3 * j2 + 2] - v[3 * j1 + 2];
-            const double ell_J = sqrt(eJx * eJx + eJy * eJy + eJz * eJz) + eps;
-            const int is[2] = {i1, i2}, js[2] = {j1, j2};
-            double sumK = 0.0;
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
-                    const int i = is[a], j = js[b];
-                    const double dx = v[3 * i] - v[3 * j];
-                    const double dy = v[3 * i + 1] - v[3 * j + 1];
-                    const double dz = v[3 * i + 2] - v[3 * j + 2];
-                    const double d_norm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
-                    const double cx = eIy * dz - eIz * dy;
-                    const double cy = eIz * dx - eIx * dz;
-                    const double cz = eIx * dy - eIy * dx;
-                    const double c_norm = sqrt(cx * cx + cy * cy + cz * cz) + eps;
-                    sumK += js_pow(c_norm, alpha) / js_pow(d_norm, beta);
-                }
-            const double term = 0.25 * js_pow(ell_I, 1 - alpha) * ell_J * sumK;
-            const double t = s + term;
-            if (fabs(s) >= fabs(term)) comp += (s - t) + term;
-            else comp += (term - t) + s;
-            s = t;
-        }
+    double* buf = (double*)malloc(sizeof(double) * RCO_IB * (size_t)(E > 0 ? E : 1));
+    for (int I0 = 0; I0 < E; I0 += RCO_IB) {
+        const int nI = E - I0 < RCO_IB ? E - I0 : RCO_IB;
+        energy_rows(v, edges, E, I0, nI, alpha, beta, eps, buf);
+        for (int r = 0; r < nI; r++)
+            for (int J = 0; J < E; J++) {
+                if (!disjoint(edges, I0 + r, J)) continue;
+                const double term = buf[(size_t)r * E + J];
+                const double t = s + term;
+                if (fabs(s) >= fabs(term)) comp += (s - t) + term;
+                else comp += (term - t) + s;
+                s = t;
+            }
     }
+    free(buf);
     return (s + comp) / 2;
 }
 
@@ -253,71 +283,131 @@ static inline void add_to_grad(double* g, int idx, double x, double y, double z)
     g[3 * idx + 2] += js_or0(z);
 }
 
+/* Guarded unit tangent of edge K as the gradient forms it
+ * (tangentPointEnergy.ts:181-186,261-266,293-297): e / ||e||, 0 below 1e-14;
+ * *re = ||e||. */
+static void edge_unit(const double* v, const int32_t* edges, int K, double* e, double* eh,
+                      double* re) {
+    const int k1 = edges[2 * K], k2 = edges[2 * K + 1];
+    e[0] = v[3 * k2] - v[3 * k1];
+    e[1] = v[3 * k2 + 1] - v[3 * k1 + 1];
+    e[2] = v[3 * k2 + 2] - v[3 * k1 + 2];
+    *re = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    eh[0] = eh[1] = eh[2] = 0;
+    if (!(*re < 1e-14)) {
+        const double inv = 1 / *re;
+        eh[0] = inv * e[0];
+        eh[1] = inv * e[1];
+        eh[2] = inv * e[2];
+    }
+}
+
+/* Per ordered pair of the gradient: sI, sJ, base, then dd[4][3], de[4][3]. */
+#define GR_Q 27
+
+/* Pair quantities of row I (gradientAnalytical's inner loop, before its
+ * addToGrad calls) into q[J * GR_Q], disjoint J only. */
+static void gradient_row(const double* v, const int32_t* edges, int E, int I, double alpha,
+                         double beta, double eps, double* q) {
+    const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
+    double eI[3], eIh[3], reI;
+    edge_unit(v, edges, I, eI, eIh, &reI);
+    const double ell_I = reI + eps;
+    const double ell_I_pow = js_pow(ell_I, 1 - alpha);
+    for (int J = 0; J < E; J++) {
+        if (!disjoint(edges, I, J)) continue;
+        const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
+        double eJ[3], eJh[3], reJ;
+        edge_unit(v, edges, J, eJ, eJh, &reJ);
+        const double ell_J = reJ + eps;
+        const int ti[4] = {i1, i1, i2, i2};
+        const int tj[4] = {j1, j2, j1, j2};
+        double* o = q + (size_t)J * GR_Q;
+        double sumF = 0;
+        for (int p = 0; p < 4; p++) {
+            const double dx = v[3 * ti[p]] - v[3 * tj[p]];
+            const double dy = v[3 * ti[p] + 1] - v[3 * tj[p] + 1];
+            const double dz = v[3 * ti[p] + 2] - v[3 * tj[p] + 2];
+            const kd_t t = kernel_derivs(eI[0], eI[1], eI[2], dx, dy, dz, alpha, beta, eps);
+            sumF += t.f;
+            for (int c = 0; c < 3; c++) {
+                o[3 + 3 * p + c] = t.dd[c];
+                o[15 + 3 * p + c] = t.de[c];
+            }
+        }
+        o[0] = 0.25 * ell_J * sumF;
+        o[1] = 0.25 * ell_I_pow * sumF;
+        o[2] = 0.25 * ell_I_pow * ell_J;
+    }
+}
+
+/* The addToGrad calls of rows I0 .. I0+nI-1 in the reference's order,
+ * applied only to vertices lo <= idx < hi. */
+static void gradient_scatter(const double* v, const int32_t* edges, int E, int I0, int nI,
+                             double alpha, double eps, const double* buf, int lo, int hi,
+                             double* g) {
+#define ADDG(idx, x, y, z) \
+    do { if ((idx) >= lo && (idx) < hi) add_to_grad(g, (idx), (x), (y), (z)); } while (0)
+    for (int r = 0; r < nI; r++) {
+        const int I = I0 + r;
+        const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
+        const int ihit = (i1 >= lo && i1 < hi) || (i2 >= lo && i2 < hi);
+        double eI[3], eIh[3], reI;
+        edge_unit(v, edges, I, eI, eIh, &reI);
+        const double ell_I = reI + eps;
+        const double dPowCoeff = (1 - alpha) * js_pow(ell_I, -alpha);
+        const double dP1x = dPowCoeff * (-1 * eIh[0]), dP1y = dPowCoeff * (-1 * eIh[1]),
+                     dP1z = dPowCoeff * (-1 * eIh[2]);
+        const double dP2x = dPowCoeff * eIh[0], dP2y = dPowCoeff * eIh[1], dP2z = dPowCoeff * eIh[2];
+        for (int J = 0; J < E; J++) {
+            if (!disjoint(edges, I, J)) continue;
+            const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
+            if (!ihit && !(j1 >= lo && j1 < hi) && !(j2 >= lo && j2 < hi)) continue;
+            double eJ[3], eJh[3], reJ;
+            edge_unit(v, edges, J, eJ, eJh, &reJ);
+            const double dJ1x = -1 * eJh[0], dJ1y = -1 * eJh[1], dJ1z = -1 * eJh[2];
+            const int ti[4] = {i1, i1, i2, i2};
+            const int tj[4] = {j1, j2, j1, j2};
+            const double* o = buf + ((size_t)r * E + J) * GR_Q;
+            const double sI = o[0], sJ = o[1], base = o[2];
+            ADDG(i1, sI * dP1x, sI * dP1y, sI * dP1z);
+            ADDG(i2, sI * dP2x, sI * dP2y, sI * dP2z);
+            ADDG(j1, sJ * dJ1x, sJ * dJ1y, sJ * dJ1z);
+            ADDG(j2, sJ * eJh[0], sJ * eJh[1], sJ * eJh[2]);
+            for (int p = 0; p < 4; p++) {
+                const double* dd = o + 3 + 3 * p;
+                const double* de = o + 15 + 3 * p;
+                ADDG(ti[p], base * dd[0], base * dd[1], base * dd[2]);
+                ADDG(tj[p], -base * dd[0], -base * dd[1], -base * dd[2]);
+                ADDG(i1, -base * de[0], -base * de[1], -base * de[2]);
+                ADDG(i2, base * de[0], base * de[1], base * de[2]);
+            }
+        }
+    }
+#undef ADDG
+}
+
 /* gradientAnalytical — tangentPointEnergy.ts:139-392 (same addToGrad order). */
 void rco_gradient(int V, int E, const double* v, const int32_t* edges,
                   double alpha, double beta, double eps, double* g) {
     memset(g, 0, sizeof(double) * 3 * (size_t)V);
-    for (int I = 0; I < E; I++) {
-        const int i1 = edges[2 * I], i2 = edges[2 * I + 1];
-        const double eIx = v[3 * i2] - v[3 * i1];
-        const double eIy = v[3 * i2 + 1] - v[3 * i1 + 1];
-        const double eIz = v[3 * i2 + 2] - v[3 * i1 + 2];
-        const double reI = sqrt(eIx * eIx + eIy * eIy + eIz * eIz);
-        double eIhx = 0, eIhy = 0, eIhz = 0;
-        if (!(reI < 1e-14)) {
-            const double inv = 1 / reI;
-            eIhx = inv * eIx;
-            eIhy = inv * eIy;
-            eIhz = inv * eIz;
-        }
-        const double ell_I = reI + eps;
-        const double ell_I_pow = js_pow(ell_I, 1 - alpha);
-        const double dPowCoeff = (1 - alpha) * js_pow(ell_I, -alpha);
-        const double dP1x = dPowCoeff * (-1 * eIhx), dP1y = dPowCoeff * (-1 * eIhy),
-                     dP1z = dPowCoeff * (-1 * eIhz);
-        const double dP2x = dPowCoeff * eIhx, dP2y = dPowCoeff * eIhy, dP2z = dPowCoeff * eIhz;
-        for (int J = 0; J < E; J++) {
-            if (!disjoint(edges, I, J)) continue;
-            const int j1 = edges[2 * J], j2 = edges[2 * J + 1];
-            const double eJx = v[3 * j2] - v[3 * j1];
-            const double eJy = v[3 * j2 + 1] - v[3 * j1 + 1];
-            const double eJz = v[3 * j2 + 2] - v[3 * j1 + 2];
-            const double reJ = sqrt(eJx * eJx + eJy * eJy + eJz * eJz);
-            double eJhx = 0, eJhy = 0, eJhz = 0;
-            if (!(reJ < 1e-14)) {
-                const double inv = 1 / reJ;
-                eJhx = inv * eJx;
-                eJhy = inv * eJy;
-                eJhz = inv * eJz;
-            }
-            const double ell_J = reJ + eps;
-            const double dJ1x = -1 * eJhx, dJ1y = -1 * eJhy, dJ1z = -1 * eJhz;
-            const int ti[4] = {i1, i1, i2, i2};
-            const int tj[4] = {j1, j2, j1, j2};
-            kd_t t[4];
-            double sumF = 0;
-            for (int p = 0; p < 4; p++) {
-                const double dx = v[3 * ti[p]] - v[3 * tj[p]];
-                const double dy = v[3 * ti[p] + 1] - v[3 * tj[p] + 1];
-                const double dz = v[3 * ti[p] + 2] - v[3 * tj[p] + 2];
-                t[p] = kernel_derivs(eIx, eIy, eIz, dx, dy, dz, alpha, beta, eps);
-                sumF += t[p].f;
-            }
-            const double sI = 0.25 * ell_J * sumF;
-            add_to_grad(g, i1, sI * dP1x, sI * dP1y, sI * dP1z);
-            add_to_grad(g, i2, sI * dP2x, sI * dP2y, sI * dP2z);
-            const double sJ = 0.25 * ell_I_pow * sumF;
-            add_to_grad(g, j1, sJ * dJ1x, sJ * dJ1y, sJ * dJ1z);
-            add_to_grad(g, j2, sJ * eJhx, sJ * eJhy, sJ * eJhz);
-            const double base = 0.25 * ell_I_pow * ell_J;
-            for (int p = 0; p < 4; p++) {
-                add_to_grad(g, ti[p], base * t[p].dd[0], base * t[p].dd[1], base * t[p].dd[2]);
-                add_to_grad(g, tj[p], -base * t[p].dd[0], -base * t[p].dd[1], -base * t[p].dd[2]);
-                add_to_grad(g, i1, -base * t[p].de[0], -base * t[p].de[1], -base * t[p].de[2]);
-                add_to_grad(g, i2, base * t[p].de[0], base * t[p].de[1], base * t[p].de[2]);
-            }
-        }
+    const int nt = rco_threads();
+    double* buf = (double*)malloc(sizeof(double) * GR_Q * RCO_IB * (size_t)(E > 0 ? E : 1));
+    for (int I0 = 0; I0 < E; I0 += RCO_IB) {
+        const int nI = E - I0 < RCO_IB ? E - I0 : RCO_IB;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+        for (int r = 0; r < nI; r++)
+            gradient_row(v, edges, E, I0 + r, alpha, beta, eps, buf + (size_t)r * E * GR_Q);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+#endif
+        for (int t = 0; t < nt; t++)
+            gradient_scatter(v, edges, E, I0, nI, alpha, eps, buf, (int)((long)V * t / nt),
+                             (int)((long)V * (t + 1) / nt), g);
     }
+    free(buf);
     for (int i = 0; i < V; i++) {
         g[3 * i] *= 0.5;
         g[3 * i + 1] *= 0.5;
@@ -348,7 +438,13 @@ static void edge_geom_eps(const double* v, const int32_t* edges, int E, double e
     }
 }
 
+/* Explicit symmetrisation 0.5 (x + y) over off-diagonal pairs
+ * (innerProduct.ts:131-137). Row i writes the pairs (i, j > i): disjoint
+ * across rows, so the rows run in parallel. */
 static void symmetrize(double* B, int n) {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(rco_threads()) schedule(dynamic, 16)
+#endif
     for (int i = 0; i < n; i++)
         for (int j = i + 1; j < n; j++) {
             const double avg = 0.5 * (B[(size_t)i * n + j] + B[(size_t)j * n + i]);
@@ -357,101 +453,212 @@ static void symmetrize(double* B, int n) {
         }
 }
 
-/* assembleBHighFlat — innerProduct.ts:38-140. */
-void rco_assemble_b_high(int n, int E, const double* v, const int32_t* edges,
-                         double alpha, double beta, double eps, double* B) {
-    memset(B, 0, sizeof(double) * (size_t)n * n);
-    const double s = (beta - 1) / alpha;
-    const double sigma = s - 1;
-    const double distExp = 2 * sigma + 1;
-    double* ell = (double*)malloc(sizeof(double) * (size_t)E);
-    double* T = (double*)malloc(sizeof(double) * 3 * (size_t)E);
-    edge_geom_eps(v, edges, E, eps, ell, T);
-    for (int I = 0; I < E; I++) {
-        const int idxI[2] = {edges[2 * I], edges[2 * I + 1]};
-        const double ellI = ell[I];
-        const double* TI = T + 3 * I;
-        for (int J = 0; J < E; J++) {
-            if (!disjoint(edges, I, J)) continue;
-            const int idxJ[2] = {edges[2 * J], edges[2 * J + 1]};
-            const double ellJ = ell[J];
-            const double* TJ = T + 3 * J;
-            double sumDist = 0;
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
-                    const int ia = idxI[a], jb = idxJ[b];
-                    const double dx = v[3 * ia] - v[3 * jb];
-                    const double dy = v[3 * ia + 1] - v[3 * jb + 1];
-                    const double dz = v[3 * ia + 2] - v[3 * jb + 2];
-                    const double dNorm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
-                    sumDist += 1 / js_pow(dNorm, distExp);
-                }
-            const double w = 0.25 * ellI * ellJ * sumDist;
-            const double dotT = TI[0] * TJ[0] + TI[1] * TJ[1] + TI[2] * TJ[2];
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
-                    const double sign = (a + b) % 2 == 0 ? 1 : -1;
-                    const int ia = idxI[a], ib = idxI[b], ja = idxJ[a], jb = idxJ[b];
-                    B[(size_t)ia * n + ib] += (sign * w) / (ellI * ellI);
-                    B[(size_t)ia * n + jb] -= (sign * w * dotT) / (ellI * ellJ);
-                    B[(size_t)ja * n + jb] += (sign * w) / (ellJ * ellJ);
-                    B[(size_t)ja * n + ib] -= (sign * w * dotT) / (ellI * ellJ);
-                }
-        }
-    }
-    symmetrize(B, n);
-    free(ell);
-    free(T);
-}
-
-/* assembleBLowFlat — innerProduct.ts:177-285 (T_I only; unsigned table). */
-void rco_assemble_b_low(int n, int E, const double* v, const int32_t* edges,
-                        double alpha, double beta, double eps, double* B0) {
-    memset(B0, 0, sizeof(double) * (size_t)n * n);
-    const double s = (beta - 1) / alpha;
-    const double sigma = s - 1;
-    const double distExp = 2 * sigma + 1;
-    double* ell = (double*)malloc(sizeof(double) * (size_t)E);
-    double* T = (double*)malloc(sizeof(double) * 3 * (size_t)E);
-    edge_geom_eps(v, edges, E, eps, ell, T);
-    for (int I = 0; I < E; I++) {
-        const int idxI[2] = {edges[2 * I], edges[2 * I + 1]};
-        const double ellI = ell[I];
-        const double* TI = T + 3 * I;
-        for (int J = 0; J < E; J++) {
-            if (!disjoint(edges, I, J)) continue;
-            const int idxJ[2] = {edges[2 * J], edges[2 * J + 1]};
-            const double ellJ = ell[J];
-            double sumLow = 0;
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
-                    const int ia = idxI[a], jb = idxJ[b];
-                    const double dx = v[3 * ia] - v[3 * jb];
-                    const double dy = v[3 * ia + 1] - v[3 * jb + 1];
-                    const double dz = v[3 * ia + 2] - v[3 * jb + 2];
-                    const double dNorm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+/* Pair weights of row I of the metric, disjoint J only (w[J]):
+ * high order (innerProduct.ts:94-104) w = 1/4 ell_I ell_J sum 1/(d+eps)^(2 sigma+1);
+ * low order (innerProduct.ts:230-251) q = 1/4 (1/4 ell_I ell_J sum k2^4/(d+eps)^(2 sigma+1)),
+ * k2^4 = (||T_I x d|| + eps)^2 / (d+eps)^4 with T_I only. */
+static void gram_row(const double* v, const int32_t* edges, int E, int I, const double* ell,
+                     const double* T, double eps, double distExp, int low, double* w) {
+    const int idxI[2] = {edges[2 * I], edges[2 * I + 1]};
+    const double ellI = ell[I];
+    const double* TI = T + 3 * I;
+    for (int J = 0; J < E; J++) {
+        if (!disjoint(edges, I, J)) continue;
+        const int idxJ[2] = {edges[2 * J], edges[2 * J + 1]};
+        const double ellJ = ell[J];
+        double sum = 0;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const int ia = idxI[a], jb = idxJ[b];
+                const double dx = v[3 * ia] - v[3 * jb];
+                const double dy = v[3 * ia + 1] - v[3 * jb + 1];
+                const double dz = v[3 * ia + 2] - v[3 * jb + 2];
+                const double dNorm = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+                if (!low) {
+                    sum += 1 / js_pow(dNorm, distExp);
+                } else {
                     const double cx = TI[1] * dz - TI[2] * dy;
                     const double cy = TI[2] * dx - TI[0] * dz;
                     const double cz = TI[0] * dy - TI[1] * dx;
                     const double crossNorm = sqrt(cx * cx + cy * cy + cz * cz) + eps;
                     const double k24 = (crossNorm * crossNorm) / (dNorm * dNorm * dNorm * dNorm);
-                    sumLow += k24 / js_pow(dNorm, distExp);
+                    sum += k24 / js_pow(dNorm, distExp);
                 }
-            const double wLow = 0.25 * ellI * ellJ * sumLow;
-            const double q = 0.25 * wLow;
-            for (int a = 0; a < 2; a++)
-                for (int b = 0; b < 2; b++) {
-                    const int ia = idxI[a], ib = idxI[b], ja = idxJ[a], jb = idxJ[b];
-                    B0[(size_t)ia * n + ib] += q;
-                    B0[(size_t)ia * n + jb] -= q;
-                    B0[(size_t)ja * n + ib] -= q;
-                    B0[(size_t)ja * n + jb] += q;
-                }
+            }
+        if (!low) {
+            w[J] = 0.25 * ellI * ellJ * sum;
+        } else {
+            const double wLow = 0.25 * ellI * ellJ * sum;
+            w[J] = 0.25 * wLow;
         }
     }
-    symmetrize(B0, n);
+}
+
+/* The 16 updates of every pair of rows I0 .. I0+nI-1 in the reference's
+ * order (innerProduct.ts:111-124 signed, :259-270 unsigned), applied only to
+ * matrix rows lo <= row < hi. */
+static void gram_scatter(const int32_t* edges, int E, int I0, int nI, const double* ell,
+                         const double* T, int low, const double* buf, int lo, int hi, int n,
+                         double* B) {
+#define INR(x) ((x) >= lo && (x) < hi)
+    for (int r = 0; r < nI; r++) {
+        const int I = I0 + r;
+        const int idxI[2] = {edges[2 * I], edges[2 * I + 1]};
+        const int ihit = INR(idxI[0]) || INR(idxI[1]);
+        const double ellI = ell[I];
+        const double* TI = T + 3 * I;
+        for (int J = 0; J < E; J++) {
+            if (!disjoint(edges, I, J)) continue;
+            const int idxJ[2] = {edges[2 * J], edges[2 * J + 1]};
+            if (!ihit && !INR(idxJ[0]) && !INR(idxJ[1])) continue;
+            const double w = buf[(size_t)r * E + J];
+            const double ellJ = ell[J];
+            const double* TJ = T + 3 * J;
+            if (!low) {
+                const double dotT = TI[0] * TJ[0] + TI[1] * TJ[1] + TI[2] * TJ[2];
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) {
+                        const double sign = (a + b) % 2 == 0 ? 1 : -1;
+                        const int ia = idxI[a], ib = idxI[b], ja = idxJ[a], jb = idxJ[b];
+                        if (INR(ia)) B[(size_t)ia * n + ib] += (sign * w) / (ellI * ellI);
+                        if (INR(ia)) B[(size_t)ia * n + jb] -= (sign * w * dotT) / (ellI * ellJ);
+                        if (INR(ja)) B[(size_t)ja * n + jb] += (sign * w) / (ellJ * ellJ);
+                        if (INR(ja)) B[(size_t)ja * n + ib] -= (sign * w * dotT) / (ellI * ellJ);
+                    }
+            } else {
+                const double q = w;
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) {
+                        const int ia = idxI[a], ib = idxI[b], ja = idxJ[a], jb = idxJ[b];
+                        if (INR(ia)) B[(size_t)ia * n + ib] += q;
+                        if (INR(ia)) B[(size_t)ia * n + jb] -= q;
+                        if (INR(ja)) B[(size_t)ja * n + ib] -= q;
+                        if (INR(ja)) B[(size_t)ja * n + jb] += q;
+                    }
+            }
+        }
+    }
+#undef INR
+}
+
+/* assembleBHighFlat (innerProduct.ts:38-140, low = 0) and assembleBLowFlat
+ * (:177-285, low = 1): s = (beta-1)/alpha, sigma = s-1, distExp = 2 sigma+1
+ * (:52-54); no 1/2 on B; symmetrised. */
+static void assemble_b(int n, int E, const double* v, const int32_t* edges, double alpha,
+                       double beta, double eps, int low, double* B) {
+    memset(B, 0, sizeof(double) * (size_t)n * n);
+    const double s = (beta - 1) / alpha;
+    const double sigma = s - 1;
+    const double distExp = 2 * sigma + 1;
+    double* ell = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    double* T = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    edge_geom_eps(v, edges, E, eps, ell, T);
+    const int nt = rco_threads();
+    double* buf = (double*)malloc(sizeof(double) * RCO_IB * (size_t)(E > 0 ? E : 1));
+    for (int I0 = 0; I0 < E; I0 += RCO_IB) {
+        const int nI = E - I0 < RCO_IB ? E - I0 : RCO_IB;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+        for (int r = 0; r < nI; r++)
+            gram_row(v, edges, E, I0 + r, ell, T, eps, distExp, low, buf + (size_t)r * E);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+#endif
+        for (int t = 0; t < nt; t++)
+            gram_scatter(edges, E, I0, nI, ell, T, low, buf, (int)((long)n * t / nt),
+                         (int)((long)n * (t + 1) / nt), n, B);
+    }
+    symmetrize(B, n);
+    free(buf);
     free(ell);
     free(T);
+}
+
+/* A = B + B0 with every entry accumulated in long double (x87 80-bit) and
+ * rounded once at the end. Not a reference function: the pair weights are
+ * the reference's (FP64), only the accumulation over the ~2E contributions
+ * of the diagonal and edge entries is extra-precise. The checker uses it to
+ * separate the FP64 summation order's effect on the saddle solution from
+ * the solver's (kappa(A_hat) ~ 1e9 at V = 8192 amplifies 1e-14 entry errors). */
+void rco_assemble_a_ld(int n, int E, const double* v, const int32_t* edges, double alpha,
+                       double beta, double eps, double* A) {
+    long double* L = (long double*)calloc((size_t)n * n, sizeof(long double));
+    const double s = (beta - 1) / alpha;
+    const double sigma = s - 1;
+    const double distExp = 2 * sigma + 1;
+    double* ell = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    double* T = (double*)malloc(sizeof(double) * 3 * (size_t)(E > 0 ? E : 1));
+    edge_geom_eps(v, edges, E, eps, ell, T);
+    const int nt = rco_threads();
+    double* buf = (double*)malloc(sizeof(double) * 2 * RCO_IB * (size_t)(E > 0 ? E : 1));
+    for (int I0 = 0; I0 < E; I0 += RCO_IB) {
+        const int nI = E - I0 < RCO_IB ? E - I0 : RCO_IB;
+        double* bh = buf;
+        double* bl = buf + (size_t)RCO_IB * E;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+        for (int r = 0; r < 2 * nI; r++) {
+            const int low = r >= nI, rr = low ? r - nI : r;
+            gram_row(v, edges, E, I0 + rr, ell, T, eps, distExp, low,
+                     (low ? bl : bh) + (size_t)rr * E);
+        }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+#endif
+        for (int t = 0; t < nt; t++) {
+            const int lo = (int)((long)n * t / nt), hi = (int)((long)n * (t + 1) / nt);
+            for (int r = 0; r < nI; r++) {
+                const int I = I0 + r;
+                const int idxI[2] = {edges[2 * I], edges[2 * I + 1]};
+                const double ellI = ell[I];
+                const double* TI = T + 3 * I;
+                for (int J = 0; J < E; J++) {
+                    if (!disjoint(edges, I, J)) continue;
+                    const int idxJ[2] = {edges[2 * J], edges[2 * J + 1]};
+                    const double w = bh[(size_t)r * E + J], q = bl[(size_t)r * E + J];
+                    const double ellJ = ell[J];
+                    const double* TJ = T + 3 * J;
+                    const long double dotT = (long double)TI[0] * TJ[0] +
+                                             (long double)TI[1] * TJ[1] + (long double)TI[2] * TJ[2];
+                    for (int a = 0; a < 2; a++)
+                        for (int b = 0; b < 2; b++) {
+                            const long double sign = (a + b) % 2 == 0 ? 1 : -1;
+                            const int ia = idxI[a], ib = idxI[b], ja = idxJ[a], jb = idxJ[b];
+                            if (ia >= lo && ia < hi) {
+                                L[(size_t)ia * n + ib] += sign * w / ((long double)ellI * ellI) + q;
+                                L[(size_t)ia * n + jb] -= sign * w * dotT / ((long double)ellI * ellJ) + q;
+                            }
+                            if (ja >= lo && ja < hi) {
+                                L[(size_t)ja * n + jb] += sign * w / ((long double)ellJ * ellJ) + q;
+                                L[(size_t)ja * n + ib] -= sign * w * dotT / ((long double)ellI * ellJ) + q;
+                            }
+                        }
+                }
+            }
+        }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 16)
+#endif
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            A[(size_t)i * n + j] = (double)(0.5L * (L[(size_t)i * n + j] + L[(size_t)j * n + i]));
+    free(buf);
+    free(ell);
+    free(T);
+    free(L);
+}
+
+void rco_assemble_b_high(int n, int E, const double* v, const int32_t* edges,
+                         double alpha, double beta, double eps, double* B) {
+    assemble_b(n, E, v, edges, alpha, beta, eps, 0, B);
+}
+
+void rco_assemble_b_low(int n, int E, const double* v, const int32_t* edges,
+                        double alpha, double beta, double eps, double* B0) {
+    assemble_b(n, E, v, edges, alpha, beta, eps, 1, B0);
 }
 
 /* assembleAFlat — innerProduct.ts:325-350. */
@@ -460,7 +667,11 @@ void rco_assemble_a(int n, int E, const double* v, const int32_t* edges,
     double* B0 = (double*)malloc(sizeof(double) * (size_t)n * n);
     rco_assemble_b_high(n, E, v, edges, alpha, beta, eps, A);
     rco_assemble_b_low(n, E, v, edges, alpha, beta, eps, B0);
-    for (size_t i = 0; i < (size_t)n * n; i++) A[i] = A[i] + B0[i];
+    const long nn = (long)n * n;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(rco_threads()) schedule(static)
+#endif
+    for (long i = 0; i < nn; i++) A[i] = A[i] + B0[i];
     free(B0);
 }
 
@@ -1169,6 +1380,59 @@ int rco_solve_saddle_from_a(const double* A, int V, const double* C, int k,
     if (rc == 0) rc = frozen_solve(&f, rhs_top, rhs_bottom, x, lambda, residual);
     frozen_free(&f);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Residual of the reference saddle system K z = rhs (linsolve.ts:771-790,
+ * K = [[I3 (x) A, C^T],[C, 0]], z = [x; lambda]) in double-double
+ * arithmetic: every product a*x is split exactly (TwoProduct by fma), every
+ * sum is an error-free TwoSum carried in a (hi, lo) pair, and the row's
+ * result is rounded once. Not a reference function: it is the extra-precise
+ * residual with which the checker refines a saddle solution beyond FP64
+ * round-off (kappa(K) ~ 1e9 at V = 8192). A V x V row-major, C k x 3V. */
+typedef struct {
+    double hi, lo;
+} dd_t;
+
+static inline dd_t dd_add_d(dd_t a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double err = (a.hi - (s - bb)) + (b - bb);
+    const double lo = a.lo + err;
+    dd_t r;
+    r.hi = s + lo;
+    r.lo = lo - (r.hi - s);
+    return r;
+}
+
+static inline dd_t dd_sub_prod(dd_t a, double x, double y) {
+    const double p = x * y;
+    const double e = fma(x, y, -p);
+    a = dd_add_d(a, -p);
+    return dd_add_d(a, -e);
+}
+
+void rco_saddle_residual_dd(const double* A, int V, const double* C, int k,
+                            const double* r_top, const double* r_bot, const double* x,
+                            const double* lambda, double* rho_top, double* rho_bot) {
+    const int m = 3 * V;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(rco_threads()) schedule(dynamic, 16)
+#endif
+    for (int row = 0; row < m; row++) {
+        const int b = row / V, i = row - b * V;
+        dd_t s = {r_top ? r_top[row] : 0.0, 0.0};
+        const double* a = A + (size_t)i * V;
+        const double* xb = x + (size_t)b * V;
+        for (int j = 0; j < V; j++) s = dd_sub_prod(s, a[j], xb[j]);
+        for (int r = 0; r < k; r++) s = dd_sub_prod(s, C[(size_t)r * m + row], lambda[r]);
+        rho_top[row] = s.hi + s.lo;
+    }
+    for (int r = 0; r < k; r++) {
+        dd_t s = {r_bot ? r_bot[r] : 0.0, 0.0};
+        for (int j = 0; j < m; j++) s = dd_sub_prod(s, C[(size_t)r * m + j], x[j]);
+        rho_bot[r] = s.hi + s.lo;
+    }
 }
 
 /* ------------------------------------------------------------------------ */

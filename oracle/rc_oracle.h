@@ -90,6 +90,12 @@ typedef struct {
 /* Fills defaults matching the reference (optimizer.ts:18, lineSearch.ts:207-209,435-438). */
 void rco_default_opts(rco_step_opts* o);
 
+/* Threads of the pair loops (energy, gradient, Gram matrices); 0 = the
+ * OpenMP default. Results are bit-identical for every thread count: terms
+ * are computed in parallel and accumulated in the reference's order. */
+void rco_set_threads(int n);
+int rco_threads(void);
+
 /* ---- energy / gradient (src/core/tangentPointEnergy.ts) ---- */
 double rco_energy(int V, int E, const double* pos, const int32_t* edges,
                   double alpha, double beta, double eps);
@@ -137,6 +143,11 @@ void rco_assemble_b_low(int V, int E, const double* pos, const int32_t* edges,
 void rco_assemble_a(int V, int E, const double* pos, const int32_t* edges,
                     double alpha, double beta, double eps, double* A);
 
+/* Checker-only: A with each entry accumulated in long double and rounded
+ * once (same FP64 pair weights; not a reference function). */
+void rco_assemble_a_ld(int V, int E, const double* pos, const int32_t* edges,
+                       double alpha, double beta, double eps, double* A);
+
 /* ---- constraints (constraints.ts, constraintSet.ts, lineSearch.ts:86-109) ---- */
 void rco_barycenter_target(int V, int E, const double* pos, const int32_t* edges, double* x0);
 double rco_total_length(int V, int E, const double* pos, const int32_t* edges);
@@ -164,6 +175,14 @@ void rco_lu_solve(const double* m, const int32_t* piv, int n, const double* rhs,
 int rco_solve_saddle_from_a(const double* A, int V, const double* C, int k,
                             const double* rhs_top, const double* rhs_bottom,
                             int factor_lu, double* x, double* lambda, double* residual);
+
+/* rho = rhs - K z for K = [[I3 (x) A, C^T],[C, 0]], z = [x; lambda], in
+ * double-double arithmetic, rounded once per entry (the checker's
+ * extra-precise residual for refining saddle solutions; rhs parts nullable
+ * = 0). A V x V row-major, C k x 3V, x/rho_top coordinate-block 3V. */
+void rco_saddle_residual_dd(const double* A, int V, const double* C, int k,
+                            const double* r_top, const double* r_bot, const double* x,
+                            const double* lambda, double* rho_top, double* rho_bot);
 
 /* ---- L2h norm (lineSearch.ts:53-75) ---- */
 double rco_l2_curve_norm(int V, int E, const double* values, const double* pos,

@@ -132,7 +132,7 @@ def lib():
         L.rco_energy_bh.restype = d
         L.rco_energy_bh.argtypes = [i, i, _dp, _ip, d, d, d, d, d, i, C.POINTER(C.c_int64)]
         L.rco_step.argtypes = [i, i, _dp, _ip, d, d, d, i, d, d, _dp, _dp, _dp]
-        for nm in ("rco_assemble_b_high", "rco_assemble_b_low", "rco_assemble_a"):
+        for nm in ("rco_assemble_b_high", "rco_assemble_b_low", "rco_assemble_a", "rco_assemble_a_ld"):
             getattr(L, nm).argtypes = [i, i, _dp, _ip, d, d, d, _dp]
         L.rco_barycenter_target.argtypes = [i, i, _dp, _ip, _dp]
         L.rco_total_length.restype = d
@@ -159,8 +159,20 @@ def lib():
             i, i, _dp, _ip, i, C.POINTER(Block), C.POINTER(StepOpts), _dp, _dp,
             C.POINTER(StepStats)]
         L.rco_default_opts.argtypes = [C.POINTER(StepOpts)]
+        L.rco_saddle_residual_dd.argtypes = [_dp, i, _dp, i, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.rco_set_threads.argtypes = [i]
+        L.rco_threads.restype = i
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the pair loops (0 = OpenMP default); results do not depend on it."""
+    lib().rco_set_threads(int(n))
+
+
+def threads() -> int:
+    return lib().rco_threads()
 
 
 def _d(a):
@@ -260,7 +272,8 @@ def assemble(kind, v, e, alpha=3.0, beta=6.0, eps=1e-10):
     v, e = _verts(v), _edges(e)
     n = len(v)
     M = np.zeros((n, n))
-    getattr(lib(), {"B": "rco_assemble_b_high", "B0": "rco_assemble_b_low", "A": "rco_assemble_a"}[kind])(
+    getattr(lib(), {"B": "rco_assemble_b_high", "B0": "rco_assemble_b_low", "A": "rco_assemble_a",
+                    "A_ld": "rco_assemble_a_ld"}[kind])(
         n, len(e), _d(v), _i(e), alpha, beta, eps, _d(M))
     return M
 
@@ -360,6 +373,26 @@ def ldlt_solve(m, ipiv, b):
     x = np.ascontiguousarray(np.array(b, dtype=np.float64))
     lib().rco_ldlt_solve(_d(m), _i(ipiv), m.shape[0], _d(x))
     return x
+
+
+def saddle_residual_dd(A, Cm, r_top, r_bot, x, lam):
+    """rhs - K z of the reference saddle system in double-double arithmetic
+    (rounded once): (rho_top[3V], rho_bot[k]). A (V, V), Cm (k, 3V), x and
+    r_top coordinate-block 3V vectors."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    V = A.shape[0]
+    Cm = np.ascontiguousarray(np.asarray(Cm, dtype=np.float64).reshape(-1, 3 * V))
+    k = Cm.shape[0]
+    rt = np.ascontiguousarray(np.asarray(r_top, dtype=np.float64).reshape(3 * V))
+    rb = np.ascontiguousarray(np.asarray(r_bot, dtype=np.float64).reshape(max(k, 1))[:max(k, 1)]
+                              if k else np.zeros(1))
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3 * V))
+    lam = np.ascontiguousarray(np.asarray(lam, dtype=np.float64).reshape(-1)) if k else np.zeros(1)
+    ot = np.zeros(3 * V)
+    ob = np.zeros(max(k, 1))
+    lib().rco_saddle_residual_dd(_d(A), V, _d(Cm) if k else _d(np.zeros(1)), k, _d(rt), _d(rb),
+                                 _d(x), _d(lam), _d(ot), _d(ob))
+    return ot, ob[:k]
 
 
 def default_opts(**kw) -> StepOpts:

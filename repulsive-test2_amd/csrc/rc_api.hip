@@ -42,6 +42,8 @@ rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const De
 rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv,
                          const double* phi_g, double* solv, double* cur, LSCtl* d_ctl);
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
+rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
+                           int pcg_max_iter, double* d_out);
 
 namespace {
 
@@ -50,9 +52,11 @@ hipError_t dalloc(T** p, size_t n) {
     return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
 }
 
+// PCG diagnostics into the control block: residual of the gradient solve,
+// iterations of it and of the refinement corrections (mixed form).
 __global__ void k_copy_refine(LSCtl* c, const double* r) {
     c->refine_res = r[0];
-    c->refine_iters = (int)r[1];
+    c->refine_iters = (int)(r[1] + r[5]);
 }
 
 __global__ void k_pick_output(const double* __restrict__ pos, const double* __restrict__ cur,
@@ -87,6 +91,8 @@ extern "C" void rc_default_step_opts(rc_step_opts* o) {
     o->collect_timings = 0;
     o->gradient_mode = RC_MODE_ANALYTICAL;
     o->fd_h = 1e-6;
+    o->saddle_refine = 1;
+    o->saddle_refine_tol = 1e-6;
 }
 
 extern "C" const char* rc_last_error(const rc_ctx* ctx) {
@@ -108,7 +114,7 @@ static void free_all(rc_ctx* ctx) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     void* sp[] = {ctx->d_set, ctx->d_ctl, ctx->d_dots, ctx->d_bsum, ctx->d_ssol, ctx->d_Minv,
-                  ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_refine,
+                  ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_refine, ctx->d_ir,
                   ctx->d_stage, ctx->d_bh_f, ctx->d_bh_i, ctx->d_bh_stats, ctx->d_bct_f,
                   ctx->d_bct_i, ctx->d_bh_front, ctx->d_bh_part};
     for (void* p : sp)
@@ -247,6 +253,9 @@ static rc_status ensure_k_capacity(rc_ctx* ctx, int k) {
     if (ctx->d_tmp3) (void)hipFree(ctx->d_tmp3);
     ctx->d_tmp2 = nullptr;
     ctx->d_tmp3 = nullptr;
+    // so is the saddle refinement's
+    if (ctx->d_ir) (void)hipFree(ctx->d_ir);
+    ctx->d_ir = nullptr;
     for (int i = 0; i < 11; ++i) {
         hipError_t he = dalloc(bufs[i], sizes[i]);
         if (he != hipSuccess) {
@@ -684,6 +693,16 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_gt, ctx->d_dots, ctx->d_bsum, ctx->d_ssol,
                         ctx->d_Minv);
     if (s != RC_OK) return s;
+    // refinement with compensated residuals (k = 0 leaves A's null space
+    // unconstrained: no Schur system to correct through)
+    const int nref = (k > 0 && !ctx->has_isolated) ? (o ? o->saddle_refine : 1) : 0;
+    if (nref > 0) {
+        s = rc_saddle_refine(ctx, k, precision, nref, o ? o->saddle_refine_tol : 1e-6,
+                             o ? o->refine_max_iter : 200, ctx->d_refine + 2);
+        if (s != RC_OK) return s;
+    } else {
+        RC_HIP(hipMemsetAsync(ctx->d_refine + 2, 0, sizeof(double) * 4, ctx->stream));
+    }
     hipStream_t rst = ctx->stream;
     if (resid_go && resid_done) {
         RC_HIP(hipEventRecord(resid_go, ctx->stream));
@@ -984,6 +1003,8 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     int32_t* hst = (int32_t*)(ctx->d_host_stage + 16);
     RC_HIP(hipMemcpyAsync(hst, ctx->d_status, sizeof(int32_t) * 4, hipMemcpyDeviceToHost,
                           ctx->stream));
+    RC_HIP(hipMemcpyAsync(ctx->d_host_stage + 24, ctx->d_refine + 2, sizeof(double) * 3,
+                          hipMemcpyDeviceToHost, ctx->stream));
     if (gtilde_out) {
         hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
                            ctx->d_gt, ctx->V, Vp, ctx->d_tmp);
@@ -1022,6 +1043,10 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     stats->reason = is[3];
     stats->trials = is[4];
     stats->refine_iters = is[5];
+    stats->saddle_ir_residual = d[24];
+    stats->saddle_ir_steps = (int32_t)d[25];
+    stats->saddle_ir_correction = d[26];
+    stats->_pad2 = 0;
     if (tm) {
         float ms[9] = {};
         for (int i = 1; i <= 2; ++i) RC_HIP(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));

@@ -1192,6 +1192,137 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     publish(flags + r, epoch);
 }
 
+// Forward substitution L Y = B (FP64 factor) on nrhs columns of B
+// (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
+// grid row y (flags row y); block (b, y) solves the 64-row block r = b.
+// The mirror of k_trsm_bwd:
+//   Y_r = L_rr^-1 (B_r - sum_{c < r} L_rc Y_c)
+//       = Z_r - Q_r Y_{r-1},  Z_r = L_rr^-1 (B_r - sum_{c <= r-2} L_rc Y_c),
+//                             Q_r = L_rr^-1 L_{r,r-1},
+// so one 64 x 64 x nrhs product follows the wait for block r-1. Q_r^T tiles
+// are formed at the start as MFMA results: tile (nb, mb) holds
+// Q[16 mb + lr][16 nb + 4 q + lk] in register q, the A-operand layout of
+// Q_r for row block mb and k-block nb.
+__global__ __launch_bounds__(512) void k_trsm_fwd(const double* __restrict__ F, int Vp,
+                                                  const double* __restrict__ Linv128, double* B,
+                                                  int ldb, int nrhs, int nblk,
+                                                  int32_t* __restrict__ flags, int32_t epoch,
+                                                  int32_t* __restrict__ status) {
+    __shared__ double Lt[NB][NB + 1];         // A operand L_rc (Lt[i][m]); staging at start
+    __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (Y_c, the accumulator for Z)
+    B += (size_t)blockIdx.y * RHS_CHUNK * ldb;
+    nrhs = min(RHS_CHUNK, nrhs - (int)blockIdx.y * RHS_CHUNK);
+    flags += (size_t)blockIdx.y * nblk;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int r = (int)blockIdx.x;
+    const size_t rb = (size_t)r * NB;
+    const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
+    const bool live = wc < nrhs;
+    const double* Lrr = Linv128 + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
+    const int tr = t >> 3, tc = (t & 7) * 8;
+    double v[8];
+    auto gload = [&](int cb) {                // L_{r,cb} rows tr, columns tc..tc+7
+        const double* p = F + (rb + tr) * Vp + (size_t)cb * NB + tc;
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            const double2 x = *reinterpret_cast<const double2*>(p + q);
+            v[q] = x.x;
+            v[q + 1] = x.y;
+        }
+    };
+    auto sstore = [&]() {                     // Lt[i][m] = L[i][m]
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Lt[tr][tc + q] = v[q];
+    };
+    auto load_ys = [&](int cb) {              // Ys[m][c] = Y_cb[m][c]
+#pragma unroll
+        for (int q = 0; q < NB * RHS_CHUNK / 512; ++q) {
+            const int e = t + 512 * q, m = e % NB, c = e / NB;
+            const int cc = c < nrhs ? c : nrhs - 1;
+            const double y = B[(size_t)cc * ldb + (size_t)cb * NB + m];
+            Ys[m][c] = (c < nrhs) ? y : 0.0;
+        }
+    };
+    // this wave's rows of L_rr^-1 as A operands: A[m = wr + lr][k = 4kk + lk]
+    double li[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(wr + lr) * PB + 4 * kk + lk];
+    double qa[4][4];
+    const int ndep = r;
+    if (ndep > 0) {
+        // stage L_{r,r-1} in Lt (Lt[i][m]); Q^T tile (nb, mb) =
+        // sum_k L_{r,r-1}[k][16 nb + i] L_rr^-1[16 mb + j][k]
+        gload(r - 1);
+        sstore();
+        __syncthreads();
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[4 * kk + lk][16 * nb + lr], li[kk],
+                                                           acc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) qa[nb][q] = -acc[q];
+        }
+        __syncthreads();
+    }
+    d4 acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = wc + lr;
+        acc[q] = (c < nrhs) ? B[(size_t)c * ldb + rb + wr + 4 * q + lk] : 0.0;
+    }
+    // contributions of blocks c = 0 .. r-2
+    if (ndep > 1) gload(0);
+    for (int s = 0; s + 1 < ndep; ++s) {
+        const int cb = s;
+        wait_flag(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
+        load_ys(cb);
+        sstore();
+        if (s + 2 < ndep) gload(s + 1);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
+                                                           Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // Z = L_rr^-1 acc (acc through Ys as the B operand)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Ys[wr + 4 * q + lk][wc + lr] = acc[q];
+    __syncthreads();
+    d4 x = {0.0, 0.0, 0.0, 0.0};
+    if (live) {
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
+    }
+    if (ndep > 0) {
+        // Y_r = Z - Q Y_{r-1}
+        wait_flag(flags + r - 1, epoch, status);
+        load_ys(r - 1);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    x = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        qa[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr], x, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = wc + lr;
+        if (c < nrhs) B[(size_t)c * ldb + rb + wr + 4 * q + lk] = x[q];
+    }
+    publish(flags + r, epoch);
+}
+
 // Mixed form: 64-column panels on the FP32 factor.
 rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
     const int Vp = ctx->nblk * NB;
@@ -1430,6 +1561,25 @@ rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
     return RC_OK;
 }
 
+// L Y = B in place for nrhs columns of length Vp (column-major, ld = Vp)
+// with the FP64 factor (the solves of the saddle refinement; the gradient
+// solve's forward substitution happens inside the factorisation).
+rc_status rc_solve_fp64_forward(rc_ctx* ctx, double* d_B, int nrhs) {
+    const int Vp = ctx->nblk * NB;
+    const int per = RHS_CHUNK * ctx->bwd_chunks;
+    for (int c0 = 0; c0 < nrhs; c0 += per) {
+        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
+        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
+        const int32_t ef = ++ctx->epoch;
+        hipLaunchKernelGGL(k_trsm_fwd, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
+                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
+                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, ef,
+                           ctx->d_status);
+        RC_CHECK_LAUNCH();
+    }
+    return RC_OK;
+}
+
 // Solve S Y = B in place with the FP32 factor (caller applies the scaling).
 rc_status rc_solve_f32_factor(rc_ctx* ctx, double* d_B, int nrhs, const int32_t* skip) {
     return solve_impl<float>(ctx, ctx->d_F32, ctx->d_Linv32, d_B, nrhs, skip);
@@ -1448,10 +1598,13 @@ int rc_trsm_max_coresident(void) {
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_trsm<float, false>, 256, 0) != hipSuccess)
         return 0;
-    int per5 = 0;
+    int per5 = 0, per6 = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per5, k_trsm_bwd, 512, 0) != hipSuccess)
         return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per6, k_trsm_fwd, 512, 0) != hipSuccess)
+        return 0;
     if (per5 < per4) per4 = per5;
+    if (per6 < per4) per4 = per6;
     int m = per;
     if (per2 < m) m = per2;
     if (per3 < m) m = per3;

@@ -131,7 +131,9 @@ struct rc_ctx {
     double* d_Cg = nullptr;
     double* d_stats = nullptr;
     double* d_epart_ls = nullptr;
-    double* d_refine = nullptr;
+    double* d_refine = nullptr;   // [0..1] PCG (residual, iterations); [2..7] saddle refinement
+    double* d_ir = nullptr;       // saddle refinement workspace (rc_refine.hip), on first use
+    size_t ir_doubles = 0;
 
     // Barnes-Hut tree on the device (rc_energy_bh), grown on demand
     double* d_bh_f = nullptr;       // 6 doubles per node

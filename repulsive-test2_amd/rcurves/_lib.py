@@ -83,6 +83,9 @@ class StepOpts(C.Structure):
         ("collect_timings", C.c_int32),
         ("gradient_mode", C.c_int32),
         ("fd_h", C.c_double),
+        ("saddle_refine", C.c_int32),
+        ("_pad3", C.c_int32),
+        ("saddle_refine_tol", C.c_double),
     ]
 
 
@@ -102,6 +105,10 @@ class StepStats(C.Structure):
         ("reason", C.c_int32),
         ("trials", C.c_int32),
         ("refine_iters", C.c_int32),
+        ("saddle_ir_residual", C.c_double),
+        ("saddle_ir_correction", C.c_double),
+        ("saddle_ir_steps", C.c_int32),
+        ("_pad2", C.c_int32),
     ]
 
 

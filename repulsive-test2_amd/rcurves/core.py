@@ -408,6 +408,10 @@ class SobolevStepOptions:
     tau_min: float = 1e-12
     refine_tol: float = 1e-12
     refine_max_iter: int = 200
+    # double-double refinement of the saddle solution (not a reference knob:
+    # the FP64 solve alone stops ~kappa u_64 from the exact solution)
+    saddle_refine: int = 1
+    saddle_refine_tol: float = 1e-6
 
     def to_c(self) -> L.StepOpts:
         o = L.StepOpts()
@@ -425,17 +429,21 @@ class SobolevStepOptions:
         o.collect_timings = 1 if self.collect_timings else 0
         o.gradient_mode = _mode(self.mode)
         o.fd_h = float(self.h)
+        o.saddle_refine = int(self.saddle_refine)
+        o.saddle_refine_tol = float(self.saddle_refine_tol)
         return o
 
 
 def solve_constrained_gradient_set_frozen(vertices, edges, disjoint_pairs, alpha, beta, epsilon,
-                                          dE, cs, precision: str = "fp64"):
+                                          dE, cs, precision: str = "fp64", saddle_refine: int = 1,
+                                          refine_tol: float = 1e-12):
     """Returns (g_tilde (V,3), lambda (k,), residual)."""
     v = _verts(vertices)
     ctx = context_for(edges, len(v))
     _check_pairs(disjoint_pairs, ctx.E)
     arr, nb, keep = _blocks_array(cs)
-    opts = SobolevStepOptions(alpha=alpha, beta=beta, epsilon=epsilon, precision=precision).to_c()
+    opts = SobolevStepOptions(alpha=alpha, beta=beta, epsilon=epsilon, precision=precision,
+                              saddle_refine=saddle_refine, refine_tol=refine_tol).to_c()
     k = L.lib().rc_constraint_rows(ctx.handle, arr, nb)
     gt = np.zeros_like(v)
     lam = np.zeros(max(k, 1))
@@ -488,7 +496,10 @@ def sobolev_step_set(vertices, edges, disjoint_pairs, cs, opts: SobolevStepOptio
         "stats": _stats_dict(st),
         "diagnostics": {"trials": st.trials, "slope": st.slope, "phiNorm": st.phi_norm,
                         "energyBefore": st.energy_before, "refineIters": st.refine_iters,
-                        "refineResidual": st.refine_residual},
+                        "refineResidual": st.refine_residual,
+                        "saddleRefineSteps": st.saddle_ir_steps,
+                        "saddleResidualDD": st.saddle_ir_residual,
+                        "saddleCorrection": st.saddle_ir_correction},
     }
     if tm is not None:
         res["timings"] = {k: {"ms": getattr(tm, k), "calls": 1} for k, _ in L.Timings._fields_}

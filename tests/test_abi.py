@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from rcurves import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -41,10 +43,35 @@ def test_default_options_match_reference_defaults():
     assert (o.gradient_mode, o.fd_h) == (_lib.MODE_ANALYTICAL, 1e-6)  # optimizer.ts:18
 
 
-def test_struct_sizes_match_header_layout():
-    # rc_block: 2 ints, 3 + 1 + 3 doubles, pointer, 2 ints
-    assert ctypes.sizeof(_lib.Block) == 8 + 56 + 8 + 8
-    assert ctypes.sizeof(_lib.StepStats) == 8 * 8 + 6 * 4
-    assert ctypes.sizeof(_lib.KernelProfile) == 8 + 8 + 4 + 4
-    # rc_step_opts: params, 2 ints, 1 double, 2 ints, 7 doubles, 2 ints, int + pad, double
-    assert ctypes.sizeof(_lib.StepOpts) == 24 + 8 + 8 + 8 + 56 + 8 + 8 + 8
+def _c_layout(struct, fields):
+    """sizeof and offsetof of a header struct, from a C probe compiled here."""
+    import subprocess
+    import tempfile
+
+    body = [f'printf("%zu\\n", sizeof({struct}));']
+    body += [f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    src = ("#include <stddef.h>\n#include <stdio.h>\n#include \"rcurves.h\"\n"
+           "int main(void) {\n" + "\n".join(body) + "\nreturn 0;\n}\n")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        with open(c, "w") as f:
+            f.write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return [int(x) for x in out]
+
+
+@pytest.mark.parametrize("struct,cls", [
+    ("rc_block", "Block"), ("rc_penalties", "Penalties"), ("rc_step_opts", "StepOpts"),
+    ("rc_step_stats", "StepStats"), ("rc_timings", "Timings"),
+    ("rc_kernel_profile", "KernelProfile"), ("rc_params", "Params"),
+])
+def test_struct_layouts_match_header(struct, cls):
+    """Every ctypes mirror has the header struct's size and field offsets."""
+    py = getattr(_lib, cls)
+    names = [f[0] for f in py._fields_]
+    layout = _c_layout(struct, names)
+    assert layout[0] == ctypes.sizeof(py), (struct, layout[0], ctypes.sizeof(py))
+    for name, off in zip(names, layout[1:]):
+        assert getattr(py, name).offset == off, (struct, name, off)
