@@ -184,12 +184,9 @@ static rc_status build_topology(rc_ctx* ctx, const int32_t* edges) {
             for (int q = vOff[v]; q < vOff[v + 1]; ++q) lst.push_back(vEdge[q]);
         std::sort(lst.begin(), lst.end());
         lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
-        if ((int)lst.size() > RC_RMAX) {
-            ctx->err = "vertex tile " + std::to_string(t) + " touches " +
-                       std::to_string(lst.size()) + " edges (max " + std::to_string(RC_RMAX) +
-                       "); renumber vertices along the curve";
-            return RC_UNSUPPORTED;
-        }
+        // tiles touching more edges than the assembly stages in LDS are
+        // assembled in edge chunks (k_gram_tiles_wide)
+        if ((int)lst.size() > RC_RMAX) ctx->wide_tiles = 1;
         for (int v = t * RC_VTILE; v < std::min(V, (t + 1) * RC_VTILE); ++v)
             for (int q = vOff[v]; q < vOff[v + 1]; ++q)
                 vLocal[q] = (int32_t)(std::lower_bound(lst.begin(), lst.end(), vEdge[q]) -

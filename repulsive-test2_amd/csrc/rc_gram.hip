@@ -17,7 +17,8 @@
 // Ownership tiles: a block owns the A tile (U x V') of 32 x 32 vertices
 // (U <= V'), evaluates X, Y, 2w for every pair of the edges incident to U and
 // to V' into LDS, and gathers each A entry from its <= deg^2 incident edge
-// pairs. Every A entry has exactly one writer (no atomics); both orientations
+// pairs (tiles touching more edges than LDS stages at once are walked in
+// chunks by k_gram_tiles_wide). Every A entry has exactly one writer (no atomics); both orientations
 // are written from one value, so A is exactly symmetric. Row sums W_K, Q_K
 // come from per-tile partials [tile][K] (each slot has one writer) reduced in
 // fixed order, then k_gram_band adds the edge-own terms on the diagonal and
@@ -141,6 +142,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
     for (int p = blockIdx.x; p < ngram; p += gridDim.x) {
         int U, W;
         tri_decode(p, U, W);  // U <= W
+        // tiles beyond the staging bound go to k_gram_tiles_wide
+        if (rOff[U + 1] - rOff[U] > RC_RMAX || rOff[W + 1] - rOff[W] > RC_RMAX) continue;
         const bool diag = (U == W);
         int nU, nV;
         load_tile_edges(GU, rOff, rEdge, U, ed, owner, nU);
@@ -271,6 +274,149 @@ __global__ __launch_bounds__(256) void k_gram_tiles(EdgeSoA ed, P pw, double eps
     }
 }
 
+// Edges c0 .. c0+n-1 of tile t's incident-edge list into G (n <= RC_RMAX).
+__device__ __forceinline__ void load_edge_chunk(GEdge& G, const int32_t* __restrict__ rOff,
+                                                const int32_t* __restrict__ rEdge, int t, int c0,
+                                                int n, const EdgeSoA& ed,
+                                                const int32_t* __restrict__ owner) {
+    const int o = rOff[t] + c0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int K = rEdge[o + q];
+        G.P[q][0] = ed.ax[K]; G.P[q][1] = ed.ay[K]; G.P[q][2] = ed.az[K];
+        G.P[q][3] = ed.bx[K]; G.P[q][4] = ed.by[K]; G.P[q][5] = ed.bz[K];
+        G.u[q][0] = ed.ux[K]; G.u[q][1] = ed.uy[K]; G.u[q][2] = ed.uz[K];
+        G.ell[q] = ed.ell[K];
+        G.id[q] = K;
+        G.i1[q] = ed.i1[K];
+        G.i2[q] = ed.i2[K];
+        G.own[q] = owner[K];
+    }
+}
+
+// Tile pairs of which a tile touches more than RC_RMAX edges (dense
+// networks: a 32-vertex tile of a 3-D grid touches ~100): the same pair
+// weights, row-sum partials and gather as k_gram_tiles, over every pair of
+// RC_RMAX-edge chunks of the two tiles' edge lists. Each thread keeps its
+// four A entries in registers across the chunk pairs (the lower-index
+// vertex's incidences always in the U role, so both orientations of a
+// diagonal-tile entry add the same terms in the same order: A stays exactly
+// symmetric) and writes them once; the row-sum slots are accumulated in
+// place by their single writer (first chunk stores, later ones add).
+// Incidences come from global memory (their count per vertex is unbounded).
+template <class P>
+__global__ __launch_bounds__(256) void k_gram_tiles_wide(
+    EdgeSoA ed, P pw, double eps, int V, int E, int ngram, const int32_t* __restrict__ rOff,
+    const int32_t* __restrict__ rEdge, const int32_t* __restrict__ owner,
+    const int32_t* __restrict__ vOff, const int32_t* __restrict__ vSlot,
+    const int32_t* __restrict__ vLocal, double* __restrict__ A, int ldA,
+    double* __restrict__ rsp) {
+    __shared__ GEdge GU, GV;
+    __shared__ double Xt[RC_RMAX][RC_RMAX + 1];
+    __shared__ double Yt[RC_RMAX][RC_RMAX + 1];
+    __shared__ double Wt[RC_RMAX][RC_RMAX + 1];
+    const int t = threadIdx.x;
+    for (int p = blockIdx.x; p < ngram; p += gridDim.x) {
+        int U, W;
+        tri_decode(p, U, W);
+        const int nUt = rOff[U + 1] - rOff[U], nWt = rOff[W + 1] - rOff[W];
+        if (nUt <= RC_RMAX && nWt <= RC_RMAX) continue;   // k_gram_tiles' pair
+        const bool diag = (U == W);
+        double acc[RC_VTILE * RC_VTILE / 256];
+#pragma unroll
+        for (int j = 0; j < RC_VTILE * RC_VTILE / 256; ++j) acc[j] = 0.0;
+        // an edgeless tile still writes its row-sum slots (zero): one empty chunk
+        for (int cu0 = 0; cu0 < (nUt > 0 ? nUt : 1); cu0 += RC_RMAX) {
+            const int nu = min(RC_RMAX, nUt - cu0);
+            for (int cw0 = 0; cw0 < (nWt > 0 ? nWt : 1); cw0 += RC_RMAX) {
+                const int nv = min(RC_RMAX, nWt - cw0);
+                __syncthreads();   // the previous chunk pair's LDS reads are done
+                load_edge_chunk(GU, rOff, rEdge, U, cu0, nu, ed, owner);
+                load_edge_chunk(GV, rOff, rEdge, W, cw0, nv, ed, owner);
+                __syncthreads();
+                for (int q = t; q < nu * nv; q += blockDim.x) {
+                    const int li = q / nv, lj = q - (q / nv) * nv;
+                    const int I = GU.id[li], J = GV.id[lj];
+                    const bool dis = GU.i1[li] != GV.i1[lj] && GU.i1[li] != GV.i2[lj] &&
+                                     GU.i2[li] != GV.i1[lj] && GU.i2[li] != GV.i2[lj];
+                    double w2 = 0.0, X = 0.0, Y = 0.0;
+                    if (dis) {
+                        const bool ord = I <= J;
+                        pair_weights(pw, eps, ord ? GU.P[li] : GV.P[lj], ord ? GU.u[li] : GV.u[lj],
+                                     ord ? GU.ell[li] : GV.ell[lj], ord ? GV.P[lj] : GU.P[li],
+                                     ord ? GV.u[lj] : GU.u[li], ord ? GV.ell[lj] : GU.ell[li], w2,
+                                     X, Y);
+                    }
+                    Xt[li][lj] = X;
+                    Yt[li][lj] = Y;
+                    Wt[li][lj] = w2;
+                }
+                __syncthreads();
+                // row-sum partials of this chunk pair (as k_gram_tiles)
+                for (int side = 0; side < (diag ? 1 : 2); ++side) {
+                    const int nr = side ? nv : nu, nc = side ? nu : nv;
+                    const int row = t >> 2, part = t & 3;
+                    double sw = 0.0, sy = 0.0;
+                    bool live = false;
+                    if (row < nr) {
+                        live = side ? (GV.own[row] == W) : (GU.own[row] == U);
+                        for (int c = part; c < nc; c += 4) {
+                            const bool on = side ? (GU.own[c] == U) : (GV.own[c] == W);
+                            const double wv = side ? Wt[c][row] : Wt[row][c];
+                            const double yv = side ? Yt[c][row] : Yt[row][c];
+                            sw += on ? wv : 0.0;
+                            sy += on ? yv : 0.0;
+                        }
+                    }
+                    sw += __shfl_xor(sw, 1, 64);
+                    sy += __shfl_xor(sy, 1, 64);
+                    sw += __shfl_xor(sw, 2, 64);
+                    sy += __shfl_xor(sy, 2, 64);
+                    if (row < nr && part == 0 && live) {
+                        const int K = side ? GV.id[row] : GU.id[row];
+                        const int other = side ? U : W;
+                        const bool first = side ? (cu0 == 0) : (cw0 == 0);
+                        double* s0 = rsp + ((size_t)other * 2) * E + K;
+                        double* s1 = rsp + ((size_t)other * 2 + 1) * E + K;
+                        *s0 = first ? sw : *s0 + sw;
+                        *s1 = first ? sy : *s1 + sy;
+                    }
+                }
+                // gather this chunk pair's terms into the thread's entries
+#pragma unroll
+                for (int j = 0; j < RC_VTILE * RC_VTILE / 256; ++j) {
+                    const int q = t + 256 * j;
+                    const int u = U * RC_VTILE + q / RC_VTILE, v = W * RC_VTILE + q % RC_VTILE;
+                    if (u >= V || v >= V) continue;
+                    const bool swap = diag && v < u;
+                    const int lo = swap ? v : u, hi = swap ? u : v;
+                    double val = acc[j];
+                    for (int q1 = vOff[lo]; q1 < vOff[lo + 1]; ++q1) {
+                        const int li = vLocal[q1] - cu0;
+                        if (li < 0 || li >= nu) continue;
+                        const double si = vSlot[q1] ? -1.0 : 1.0;
+                        for (int q2 = vOff[hi]; q2 < vOff[hi + 1]; ++q2) {
+                            const int lj = vLocal[q2] - cw0;
+                            if (lj < 0 || lj >= nv) continue;
+                            const double sj = vSlot[q2] ? -1.0 : 1.0;
+                            val -= si * sj * Xt[li][lj] + Yt[li][lj];
+                        }
+                    }
+                    acc[j] = val;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RC_VTILE * RC_VTILE / 256; ++j) {
+            const int q = t + 256 * j;
+            const int u = U * RC_VTILE + q / RC_VTILE, v = W * RC_VTILE + q % RC_VTILE;
+            if (u >= V || v >= V) continue;
+            A[(size_t)u * ldA + v] = acc[j];
+            if (!diag) A[(size_t)v * ldA + u] = acc[j];
+        }
+        __syncthreads();
+    }
+}
+
 // Sums in tile order t = 0 .. nvt-1 (fixed order); the loads of 16 tiles are
 // issued ahead of their adds, and 64-thread blocks spread the edges over
 // E/64 workgroups.
@@ -340,6 +486,22 @@ rc_status rc_launch_gram(rc_ctx* ctx, const rc_params* p, hipStream_t st) {
                            ctx->d_vLocal, ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
     }
     RC_CHECK_LAUNCH();
+    if (ctx->wide_tiles) {
+        if (p->alpha == 3.0 && p->beta == 6.0) {
+            hipLaunchKernelGGL(k_gram_tiles_wide<Pow36>, dim3(grid), dim3(256), 0, st, ctx->ed,
+                               Pow36{}, p->epsilon, ctx->V, ctx->E, ctx->ngram, ctx->d_rOff,
+                               ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot,
+                               ctx->d_vLocal, ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
+        } else {
+            const double s = (p->beta - 1) / p->alpha;
+            PowGeneric pw{p->alpha, p->beta, 2 * (s - 1) + 1};
+            hipLaunchKernelGGL(k_gram_tiles_wide<PowGeneric>, dim3(grid), dim3(256), 0, st,
+                               ctx->ed, pw, p->epsilon, ctx->V, ctx->E, ctx->ngram, ctx->d_rOff,
+                               ctx->d_rEdge, ctx->d_owner, ctx->d_vOff, ctx->d_vSlot,
+                               ctx->d_vLocal, ctx->d_F, ctx->nblk * RC_NB, ctx->d_rsp);
+        }
+        RC_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(k_rowsum_reduce, dim3((ctx->E + 63) / 64), dim3(64), 0, st,
                        ctx->d_rsp, ctx->nvt, ctx->E, ctx->d_rs);
     RC_CHECK_LAUNCH();

@@ -12,7 +12,8 @@
 
 #define RC_TILE 64          // edges per tile of the pair kernels (one wave64 lane per edge)
 #define RC_VTILE 32         // vertices per ownership tile of the Gram assembly
-#define RC_RMAX 40          // max incident edges of one vertex tile (assembly LDS bound)
+#define RC_RMAX 40          // incident edges of one vertex tile staged in LDS by the assembly
+                            // (larger tiles are walked in chunks of this many)
 #define RC_NB 64            // Cholesky / triangular-solve block size
 #define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
 #define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
@@ -39,6 +40,7 @@ struct rc_ctx {
     int nblk = 0;             // Cholesky blocks of RC_NB rows (always even)
     int bwd_chunks = 1;       // right-hand-side chunks per backward-sweep launch
     int maxdeg = 0;
+    int wide_tiles = 0;       // some vertex tile touches more than RC_RMAX edges
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // factorisation lookahead (trailing updates)
     std::vector<hipEvent_t> dep_ev; // factorisation stream dependencies

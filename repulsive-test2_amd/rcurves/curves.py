@@ -13,6 +13,12 @@ Workloads (sizes as named by the step benchmark):
                      a_k, b_k in R^3 ~ N(0, 1/k^2), resampled to uniform arc length
   theta_network      components of two degree-3 junctions joined by three
                      Fourier arcs each
+  stress_grid        the first 200 points of a jittered 6 x 6 x 6 grid joined to
+                     their grid neighbours (the app's 'stress' preset,
+                     src/core/testConfigs.ts:102-150)
+  random_graph       random points in a ball of radius 2 and random distinct
+                     edges, isolated vertices joined to their nearest neighbour
+                     (the app's 'random' preset, src/core/testConfigs.ts:152-224)
 """
 from __future__ import annotations
 
@@ -129,3 +135,82 @@ def theta_network(total_edges: int = 16384, components: int = 8, seed: int = 1, 
 def seeded_pins(V: int, count: int, seed: int = 1) -> np.ndarray:
     rng = np.random.default_rng(seed + 1000)
     return np.sort(rng.choice(V, size=count, replace=False)).astype(np.int32)
+
+
+def stress_grid(seed: int = 1, grid: int = 6, spacing: float = 0.5, count: int = 200):
+    """testConfigs.ts:102-150: points (x, y, z) of a grid x grid x grid
+    lattice in x-major order up to `count`, jitter U(-0.1, 0.1) per
+    coordinate, an edge to each existing +x, +y, +z neighbour (in that order,
+    vertices in index order)."""
+    rng = np.random.default_rng(seed)
+    idx = {}
+    verts = []
+    for x in range(grid):
+        for y in range(grid):
+            for z in range(grid):
+                if len(verts) >= count:
+                    break
+                j = (rng.random(3) - 0.5) * 0.2
+                verts.append([(x - grid / 2) * spacing + j[0], (y - grid / 2) * spacing + j[1],
+                              (z - grid / 2) * spacing + j[2]])
+                idx[(x, y, z)] = len(verts) - 1
+    edges = []
+    for x in range(grid):
+        for y in range(grid):
+            for z in range(grid):
+                i = idx.get((x, y, z))
+                if i is None:
+                    continue
+                for nb in ((x + 1, y, z), (x, y + 1, z), (x, y, z + 1)):
+                    if nb in idx:
+                        edges.append((i, idx[nb]))
+    return (np.ascontiguousarray(np.asarray(verts, dtype=np.float64)),
+            np.ascontiguousarray(np.asarray(edges, dtype=np.int32)))
+
+
+def random_graph(num_vertices: int = 20, num_edges: int = 30, seed: int = 1):
+    """testConfigs.ts:152-224: points uniform in a ball of radius 2, up to
+    num_edges distinct undirected edges from 10 num_edges random draws,
+    then every vertex left without an edge joined to its nearest neighbour."""
+    rng = np.random.default_rng(seed)
+    th = rng.random(num_vertices) * 2 * np.pi
+    ph = np.arccos(2 * rng.random(num_vertices) - 1)
+    r = rng.random(num_vertices) ** (1 / 3) * 2
+    v = np.stack([r * np.sin(ph) * np.cos(th), r * np.sin(ph) * np.sin(th), r * np.cos(ph)], 1)
+    seen, edges = set(), []
+    attempts = 0
+    while len(edges) < num_edges and attempts < num_edges * 10:
+        a, b = (int(x) for x in rng.integers(0, num_vertices, 2))
+        if a != b:
+            key = (min(a, b), max(a, b))
+            if key not in seen:
+                seen.add(key)
+                edges.append((a, b))
+        attempts += 1
+    used = {x for ed in edges for x in ed}
+    for q in range(num_vertices):
+        if q in used:
+            continue
+        d = np.sum((v - v[q]) ** 2, axis=1)
+        d[q] = np.inf
+        nb = int(np.argmin(d))
+        edges.append((q, nb))
+        used.update((q, nb))
+    return (np.ascontiguousarray(v.astype(np.float64)),
+            np.ascontiguousarray(np.asarray(edges, dtype=np.int32)))
+
+
+def connected_components(V: int, e: np.ndarray) -> int:
+    parent = list(range(V))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for a, b in np.asarray(e):
+        ra, rb = find(int(a)), find(int(b))
+        if ra != rb:
+            parent[ra] = rb
+    return len({find(x) for x in range(V)})
