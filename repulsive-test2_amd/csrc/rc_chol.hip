@@ -30,9 +30,11 @@
 //                 into fp16 hi + lo (x = hi + lo to ~2^-22) and
 //                 hi*hi^T + hi*lo^T + lo*hi^T runs on
 //                 v_mfma_f32_16x16x32_f16 with FP32 accumulation
-// Triangular solves (k_trsm): one workgroup per 64-row block, dependency
-// flags in global memory (sync-free substitution), FP64 MFMA products on the
-// stored factor; the diagonal solve is a product with the stored L_kk^-1.
+// Triangular solves (k_trsm_fwd / k_trsm_bwd, on either factor): one
+// workgroup per 64-row block, dependency flags in global memory (sync-free
+// substitution), FP64 MFMA products on the stored factor, one 64 x 64
+// product on the critical path per block; the diagonal solve is a product
+// with the stored L_kk^-1.
 #include "rc_internal.h"
 #include "rc_math.h"
 #include <utility>
@@ -935,123 +937,47 @@ __global__ __launch_bounds__(256) void k_syrk_f16x3(float* __restrict__ F, int V
             }
 }
 
-// Sync-free blocked substitution on nrhs <= RHS_CHUNK columns of B
-// (column-major, leading dimension ldb), in place. FWD: L Y = B, block b
-// solves row block r = b after waiting for blocks 0..r-1. Backward:
-// L^T X = Y, block b solves row block r = nblk-1-b after blocks r+1..nblk-1.
-// Each block keeps its 64 x RHS_CHUNK accumulator in MFMA registers (wave w:
-// rows 16w..16w+15, two 16-column tiles) and subtracts L_{r,cb} Y_cb (or
-// L_{cb,r}^T X_cb) with v_mfma_f64_16x16x4_f64 as each dependency is
-// published; the next factor tile is prefetched into registers meanwhile
-// (the factor does not depend on the flags). The diagonal solve is the
-// product with the stored L_rr^-1 (or its transpose). skip (nullable): when
-// *skip != 0 every block returns at once.
-template <typename T, bool FWD>
-__global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
-                                              const T* __restrict__ Linv,
-                                              double* B, int ldb, int nrhs, int nblk,
-                                              int32_t* __restrict__ flags, int32_t epoch,
-                                              int32_t* __restrict__ status,
-                                              const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    __shared__ double Lt[NB][NB + 1];          // MFMA A operand, Lt[i][m]
-    __shared__ double Ys[NB][RHS_CHUNK + 1];   // MFMA B operand, Ys[m][c]
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int r = FWD ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
-    const size_t rb = (size_t)r * NB;
-    const int ncol = nrhs > 16 ? 2 : 1;        // 16-column MFMA tiles in use
-    // two partial accumulators per column tile (even / odd k-steps): two
-    // independent MFMA chains of half the length
-    d4 acc[2], acc2[2];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
-            acc[b][q] = (c < nrhs) ? B[(size_t)c * ldb + rb + i] : 0.0;
-            acc2[b][q] = 0.0;
-        }
-    // factor tile rows (t >> 2), 16 consecutive columns from (t & 3) * 16
-    const int tr = t >> 2, tc = (t & 3) * 16;
-    // the diagonal block's inverse does not depend on the flags: load it now
-    double li[16];
-    {
-        const T* Li = Linv + rb * NB + (size_t)tr * NB + tc;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) li[q] = (double)Li[q];
+// Inverses of the 64 x 64 diagonal blocks as the two factorisations store
+// them: the FP64 form keeps the 128 x 128 inverse of each panel (block r is
+// its diagonal 64 x 64 block r & 1, leading dimension 128), the FP32 form one
+// 64 x 64 inverse per block.
+template <typename T>
+struct InvLayout;
+template <>
+struct InvLayout<double> {
+    static constexpr int ld = PB;
+    __device__ static const double* block(const double* L, int r) {
+        return L + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
     }
-    double v[16];
-    auto gload = [&](int cb) {
-        const T* p = FWD ? F + (rb + tr) * Vp + (size_t)cb * NB + tc
-                         : F + ((size_t)cb * NB + tr) * Vp + rb + tc;
+};
+template <>
+struct InvLayout<float> {
+    static constexpr int ld = NB;
+    __device__ static const float* block(const float* L, int r) { return L + (size_t)r * NB * NB; }
+};
+
+// 8 consecutive factor entries as doubles
+__device__ __forceinline__ void load8(const double* p, double (&v)[8]) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = (double)p[q];
-    };
-    auto sstore = [&](const double (&x)[16]) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            if (FWD) Lt[tr][tc + q] = x[q];
-            else Lt[tc + q][tr] = x[q];
-        }
-    };
-    auto mfma_tile = [&](d4 (&d)[2], d4 (&d2)[2], bool neg) {
-#pragma unroll 2
-        for (int k = 0; k < NB; k += 8) {
-            const double a0 = neg ? -Lt[16 * w + lr][k + lk] : Lt[16 * w + lr][k + lk];
-            const double a1 = neg ? -Lt[16 * w + lr][k + 4 + lk] : Lt[16 * w + lr][k + 4 + lk];
-            d[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, Ys[k + lk][lr], d[0], 0, 0, 0);
-            d2[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, Ys[k + 4 + lk][lr], d2[0], 0, 0, 0);
-            if (ncol > 1) {
-                d[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, Ys[k + lk][16 + lr], d[1], 0, 0, 0);
-                d2[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, Ys[k + 4 + lk][16 + lr], d2[1], 0, 0, 0);
-            }
-        }
-    };
-    const int ndep = FWD ? r : nblk - 1 - r;
-    if (ndep > 0) gload(FWD ? 0 : nblk - 1);
-    for (int s = 0; s < ndep; ++s) {
-        const int cb = FWD ? s : nblk - 1 - s;
-        wait_flag(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
-#pragma unroll
-        for (int q = 0; q < NB * RHS_CHUNK / 256; ++q) {
-            const int e = t + 256 * q, m = e % NB, c = e / NB;
-            // plain loads: thread 0's acquire fence in wait_flag and the
-            // barrier after it order them after block cb's release
-            // (columns >= nrhs read column nrhs-1, in bounds, and store 0:
-            // no branch around the load, so the loads are issued together)
-            const int cc = c < nrhs ? c : nrhs - 1;
-            const double y = B[(size_t)cc * ldb + (size_t)cb * NB + m];
-            Ys[m][c] = (c < nrhs) ? y : 0.0;
-        }
-        sstore(v);
-        if (s + 1 < ndep) gload(FWD ? s + 1 : nblk - 2 - s);
-        __syncthreads();
-        mfma_tile(acc, acc2, true);
+    for (int q = 0; q < 8; q += 2) {
+        const double2 x = *reinterpret_cast<const double2*>(p + q);
+        v[q] = x.x;
+        v[q + 1] = x.y;
     }
-    __syncthreads();
-    // diagonal block: X_r = L_rr^-1 acc (forward) or L_rr^-T acc (backward)
+}
+__device__ __forceinline__ void load8(const float* p, double (&v)[8]) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Ys[16 * w + 4 * q + lk][16 * b + lr] = acc[b][q] + acc2[b][q];
-    sstore(li);
-    __syncthreads();
-    d4 xo[2], xo2[2];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) xo[b] = xo2[b] = (d4){0.0, 0.0, 0.0, 0.0};
-    mfma_tile(xo, xo2, false);
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = 16 * w + 4 * q + lk, c = 16 * b + lr;
-            if (c < nrhs) B[(size_t)c * ldb + rb + i] = xo[b][q] + xo2[b][q];
-        }
-    publish(flags + r, epoch);
+    for (int q = 0; q < 8; q += 4) {
+        const float4 x = *reinterpret_cast<const float4*>(p + q);
+        v[q] = x.x;
+        v[q + 1] = x.y;
+        v[q + 2] = x.z;
+        v[q + 3] = x.w;
+    }
 }
 
-// Backward substitution L^T X = Y (FP64 factor) on nrhs columns of Y
+// Backward substitution L^T X = Y (FP64 or FP32 factor, entries taken to
+// FP64 for the FP64 MFMA products) on nrhs columns of Y
 // (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
 // grid row y (flags row y: nblk flags); block (b, y) solves the 64-row block
 // r = nblk-1-b of its chunk. 8 waves, wave w owns the 16 x 16 output tile
@@ -1064,11 +990,14 @@ __global__ __launch_bounds__(256) void k_trsm(const T* __restrict__ F, int Vp,
 // as MFMA results (P tile (nb, mb): P[16nb + 4q + lk][16mb + lr] in register
 // q), which is exactly the A-operand layout of -P_r^T for row block mb and
 // k-block nb, so it stays in registers.
-__global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, int Vp,
-                                                  const double* __restrict__ Linv128, double* B,
+template <typename T>
+__global__ __launch_bounds__(512) void k_trsm_bwd(const T* __restrict__ F, int Vp,
+                                                  const T* __restrict__ Linv, double* B,
                                                   int ldb, int nrhs, int nblk,
                                                   int32_t* __restrict__ flags, int32_t epoch,
-                                                  int32_t* __restrict__ status) {
+                                                  int32_t* __restrict__ status,
+                                                  const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double Lt[NB][NB + 1];         // A operand L_cr^T (Lt[m][i]); staging at start
     __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (X_c, the accumulator for Z)
     // chunk blockIdx.y: columns 32 y .. of the launch's nrhs, its own flag row
@@ -1083,20 +1012,14 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     // a wave whose 16 output columns are all padding (nrhs <= 16: waves 4-7)
     // skips its MFMA products; it shares a SIMD with a wave that has work
     const bool live = wc < nrhs;
-    // L_rr^-1: diagonal 64 x 64 block (r & 1) of the 128 x 128 inverse
-    // (row-major, leading dimension PB) of panel r / 2
-    const double* Lrr = Linv128 + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
+    // L_rr^-1 (row-major, leading dimension ldl)
+    const int ldl = InvLayout<T>::ld;
+    const T* Lrr = InvLayout<T>::block(Linv, r);
     // factor tile rows tr, 8 consecutive columns from tc
     const int tr = t >> 3, tc = (t & 7) * 8;
     double v[8];
     auto gload = [&](int cb) {                // L_{cb,r} rows tr, columns tc..tc+7 of block r
-        const double* p = F + ((size_t)cb * NB + tr) * Vp + rb + tc;
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) {
-            const double2 x = *reinterpret_cast<const double2*>(p + q);
-            v[q] = x.x;
-            v[q + 1] = x.y;
-        }
+        load8(F + ((size_t)cb * NB + tr) * Vp + rb + tc, v);
     };
     auto sstore_t = [&]() {                   // Lt[m][i] = L[i][m]
 #pragma unroll
@@ -1114,7 +1037,7 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     // this wave's rows of L_rr^-T as A operands: A[m = wr + lr][k = 4kk + lk]
     double li[16];
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(4 * kk + lk) * PB + wr + lr];
+    for (int kk = 0; kk < 16; ++kk) li[kk] = (double)Lrr[(size_t)(4 * kk + lk) * ldl + wr + lr];
     // -P_r^T rows wr.. as A operands for k-blocks nb: pt[nb][q]
     double pt[4][4];
     const int ndep = nblk - 1 - r;
@@ -1129,9 +1052,8 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
             d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                    Lt[16 * nb + lr][4 * kk + lk], Lrr[(size_t)(4 * kk + lk) * PB + wr + lr], acc,
-                    0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[16 * nb + lr][4 * kk + lk], li[kk],
+                                                           acc, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) pt[nb][q] = -acc[q];
         }
@@ -1192,7 +1114,7 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
     publish(flags + r, epoch);
 }
 
-// Forward substitution L Y = B (FP64 factor) on nrhs columns of B
+// Forward substitution L Y = B (FP64 or FP32 factor) on nrhs columns of B
 // (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
 // grid row y (flags row y); block (b, y) solves the 64-row block r = b.
 // The mirror of k_trsm_bwd:
@@ -1203,11 +1125,14 @@ __global__ __launch_bounds__(512) void k_trsm_bwd(const double* __restrict__ F, 
 // are formed at the start as MFMA results: tile (nb, mb) holds
 // Q[16 mb + lr][16 nb + 4 q + lk] in register q, the A-operand layout of
 // Q_r for row block mb and k-block nb.
-__global__ __launch_bounds__(512) void k_trsm_fwd(const double* __restrict__ F, int Vp,
-                                                  const double* __restrict__ Linv128, double* B,
+template <typename T>
+__global__ __launch_bounds__(512) void k_trsm_fwd(const T* __restrict__ F, int Vp,
+                                                  const T* __restrict__ Linv, double* B,
                                                   int ldb, int nrhs, int nblk,
                                                   int32_t* __restrict__ flags, int32_t epoch,
-                                                  int32_t* __restrict__ status) {
+                                                  int32_t* __restrict__ status,
+                                                  const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double Lt[NB][NB + 1];         // A operand L_rc (Lt[i][m]); staging at start
     __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (Y_c, the accumulator for Z)
     B += (size_t)blockIdx.y * RHS_CHUNK * ldb;
@@ -1219,17 +1144,12 @@ __global__ __launch_bounds__(512) void k_trsm_fwd(const double* __restrict__ F, 
     const size_t rb = (size_t)r * NB;
     const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
     const bool live = wc < nrhs;
-    const double* Lrr = Linv128 + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
+    const int ldl = InvLayout<T>::ld;
+    const T* Lrr = InvLayout<T>::block(Linv, r);
     const int tr = t >> 3, tc = (t & 7) * 8;
     double v[8];
     auto gload = [&](int cb) {                // L_{r,cb} rows tr, columns tc..tc+7
-        const double* p = F + (rb + tr) * Vp + (size_t)cb * NB + tc;
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) {
-            const double2 x = *reinterpret_cast<const double2*>(p + q);
-            v[q] = x.x;
-            v[q + 1] = x.y;
-        }
+        load8(F + (rb + tr) * Vp + (size_t)cb * NB + tc, v);
     };
     auto sstore = [&]() {                     // Lt[i][m] = L[i][m]
 #pragma unroll
@@ -1247,7 +1167,7 @@ __global__ __launch_bounds__(512) void k_trsm_fwd(const double* __restrict__ F, 
     // this wave's rows of L_rr^-1 as A operands: A[m = wr + lr][k = 4kk + lk]
     double li[16];
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) li[kk] = Lrr[(size_t)(wr + lr) * PB + 4 * kk + lk];
+    for (int kk = 0; kk < 16; ++kk) li[kk] = (double)Lrr[(size_t)(wr + lr) * ldl + 4 * kk + lk];
     double qa[4][4];
     const int ndep = r;
     if (ndep > 0) {
@@ -1521,25 +1441,6 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
     return RC_OK;
 }
 
-template <typename T>
-static rc_status solve_impl(rc_ctx* ctx, const T* F, const T* Linv, double* d_B, int nrhs,
-                            const int32_t* skip) {
-    const int Vp = ctx->nblk * NB;
-    for (int c0 = 0; c0 < nrhs; c0 += RHS_CHUNK) {
-        const int nc = (nrhs - c0) < RHS_CHUNK ? (nrhs - c0) : RHS_CHUNK;
-        double* Bc = d_B + (size_t)c0 * Vp;
-        const int32_t ef = ++ctx->epoch;
-        hipLaunchKernelGGL((k_trsm<T, true>), dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp,
-                           Linv, Bc, Vp, nc, ctx->nblk, ctx->d_flags, ef, ctx->d_status, skip);
-        RC_CHECK_LAUNCH();
-        const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL((k_trsm<T, false>), dim3(ctx->nblk), dim3(256), 0, ctx->stream, F, Vp,
-                           Linv, Bc, Vp, nc, ctx->nblk, ctx->d_flags, eb, ctx->d_status, skip);
-        RC_CHECK_LAUNCH();
-    }
-    return RC_OK;
-}
-
 // L^T X = Y in place for nrhs columns of length Vp (column-major, ld = Vp)
 // with the FP64 factor; Y = L^-1 B comes out of the factorisation's
 // right-hand-side rows.
@@ -1552,10 +1453,10 @@ rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
         const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
         const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
         const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_bwd, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL(k_trsm_bwd<double>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
                            (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
                            d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
-                           ctx->d_status);
+                           ctx->d_status, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
     }
     return RC_OK;
@@ -1571,43 +1472,53 @@ rc_status rc_solve_fp64_forward(rc_ctx* ctx, double* d_B, int nrhs) {
         const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
         const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
         const int32_t ef = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_fwd, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
+        hipLaunchKernelGGL(k_trsm_fwd<double>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
                            (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
                            d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, ef,
-                           ctx->d_status);
+                           ctx->d_status, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
     }
     return RC_OK;
 }
 
-// Solve S Y = B in place with the FP32 factor (caller applies the scaling).
+// Solve S Y = B in place with the FP32 factor (caller applies the scaling):
+// the forward and backward sweeps of the FP64 form on FP32 entries.
 rc_status rc_solve_f32_factor(rc_ctx* ctx, double* d_B, int nrhs, const int32_t* skip) {
-    return solve_impl<float>(ctx, ctx->d_F32, ctx->d_Linv32, d_B, nrhs, skip);
+    const int Vp = ctx->nblk * NB;
+    const int per = RHS_CHUNK * ctx->bwd_chunks;
+    for (int c0 = 0; c0 < nrhs; c0 += per) {
+        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
+        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
+        const int32_t ef = ++ctx->epoch;
+        hipLaunchKernelGGL(k_trsm_fwd<float>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
+                           (const float*)ctx->d_F32, Vp, (const float*)ctx->d_Linv32,
+                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, ef,
+                           ctx->d_status, skip);
+        RC_CHECK_LAUNCH();
+        const int32_t eb = ++ctx->epoch;
+        hipLaunchKernelGGL(k_trsm_bwd<float>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
+                           (const float*)ctx->d_F32, Vp, (const float*)ctx->d_Linv32,
+                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
+                           ctx->d_status, skip);
+        RC_CHECK_LAUNCH();
+    }
+    return RC_OK;
 }
 
 int rc_trsm_max_coresident(void) {
-    int dev = 0, ncu = 0, per = 0, per2 = 0, per3 = 0, per4 = 0;
+    // every sweep kernel has the same shape (512 threads, the same LDS);
+    // the smallest per-CU residency of the four instantiations bounds them
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsm<double, true>, 256, 0) != hipSuccess)
+    int per[4] = {0, 0, 0, 0};
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_trsm_bwd<double>, 512, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_trsm_fwd<double>, 512, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_trsm_bwd<float>, 512, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[3], k_trsm_fwd<float>, 512, 0) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsm<double, false>, 256, 0) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, k_trsm<float, true>, 256, 0) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_trsm<float, false>, 256, 0) != hipSuccess)
-        return 0;
-    int per5 = 0, per6 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per5, k_trsm_bwd, 512, 0) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per6, k_trsm_fwd, 512, 0) != hipSuccess)
-        return 0;
-    if (per5 < per4) per4 = per5;
-    if (per6 < per4) per4 = per6;
-    int m = per;
-    if (per2 < m) m = per2;
-    if (per3 < m) m = per3;
-    if (per4 < m) m = per4;
+    int m = per[0];
+    for (int i = 1; i < 4; ++i) m = per[i] < m ? per[i] : m;
     return m * ncu;
 }

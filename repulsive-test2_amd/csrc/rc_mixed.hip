@@ -52,38 +52,49 @@ __global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ X, in
     if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// Q[c][u] = sum_j A[u][j] P[c][j] + mu * colsum[c], one row u per block,
-// up to GEMM_CHUNK columns per launch. A is read once per launch.
-__global__ __launch_bounds__(256) void k_ahat_gemm(const double* __restrict__ A, int V, int Vp,
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Q = A_hat P for nc <= 16 columns of P (ld Vp): Q[c][u] = sum_j A[j][u] P[c][j]
+// + mu colsum[c] (A symmetric, so A's column strip u0 .. u0+15 serves as
+// rows). One workgroup per 16-wide strip; its four waves split j in four and
+// add their partials in LDS in a fixed order. v_mfma_f64_16x16x4f64 with A
+// operand P^T[c = lane % 16][j + lane / 16] and B operand
+// A[j + lane / 16][u0 + lane % 16]: every B fetch is four 128-byte row
+// segments, A is read once per product.
+__global__ __launch_bounds__(256) void k_ahat_mfma(const double* __restrict__ A, int V, int Vp,
                                                    const double* __restrict__ mu,
                                                    const double* __restrict__ colsum,
                                                    const double* __restrict__ P, int nc,
                                                    double* __restrict__ Q,
                                                    const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double sh[4][GEMM_CHUNK];
-    const int u = blockIdx.x;
-    const double* row = A + (size_t)u * Vp;   // A in the FP64 factor storage (ld Vp)
-    double acc[GEMM_CHUNK];
-#pragma unroll
-    for (int c = 0; c < GEMM_CHUNK; ++c) acc[c] = 0.0;
-    for (int j = threadIdx.x; j < V; j += 256) {
-        const double a = row[j];
-#pragma unroll
-        for (int c = 0; c < GEMM_CHUNK; ++c)
-            if (c < nc) acc[c] = fma(a, P[(size_t)c * Vp + j], acc[c]);
+    __shared__ double red[4][16][17];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int u0 = blockIdx.x * 16;
+    const int span = (((V + 3) / 4) + 3) & ~3;    // a multiple of 4
+    const int j0 = w * span, j1 = min(V, j0 + span);
+    const bool cval = lr < nc;
+    const double* Pc = P + (size_t)(cval ? lr : 0) * Vp;
+    const bool uval = u0 + lr < V;
+    const double* Au = A + (size_t)(uval ? u0 + lr : 0);
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 16
+    for (int j = j0; j < j1; j += 4) {
+        const int jj = j + lk;
+        const bool ok = jj < V;
+        const double a = (cval && ok) ? Pc[jj] : 0.0;
+        const double b = (uval && ok) ? Au[(size_t)jj * Vp] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // acc[q] = partial Q[c = 4 q + lk][u = u0 + lr]
 #pragma unroll
-    for (int c = 0; c < GEMM_CHUNK; ++c) {
-        const double s = wsum(acc[c]);
-        if (lane == 0) sh[w][c] = s;
-    }
+    for (int q = 0; q < 4; ++q) red[w][4 * q + lk][lr] = acc[q];
     __syncthreads();
-    if ((int)threadIdx.x < nc) {
-        const int c = threadIdx.x;
-        Q[(size_t)c * Vp + u] = ((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c] + mu[0] * colsum[c];
-    }
+    const int c = t >> 4, uu = t & 15;
+    if (c < nc && u0 + uu < V)
+        Q[(size_t)c * Vp + u0 + uu] =
+            ((red[0][c][uu] + red[1][c][uu]) + red[2][c][uu]) + red[3][c][uu] + mu[0] * colsum[c];
 }
 
 // out[c] = <X[c], Y[c]> over v < V
@@ -219,8 +230,9 @@ static rc_status ahat_mult(rc_ctx* ctx, const double* X, double* Q, double* cs, 
     RC_CHECK_LAUNCH();
     for (int c0 = 0; c0 < nrhs; c0 += GEMM_CHUNK) {
         const int nc = (nrhs - c0) < GEMM_CHUNK ? (nrhs - c0) : GEMM_CHUNK;
-        hipLaunchKernelGGL(k_ahat_gemm, dim3(ctx->V), dim3(256), 0, ctx->stream, ctx->d_F, ctx->V,
-                           Vp, ctx->d_scal + 1, cs + c0, X + (size_t)c0 * Vp, nc,
+        hipLaunchKernelGGL(k_ahat_mfma, dim3((ctx->V + 15) / 16), dim3(256), 0, ctx->stream,
+                           (const double*)ctx->d_F, ctx->V, Vp, (const double*)(ctx->d_scal + 1),
+                           (const double*)(cs + c0), (const double*)(X + (size_t)c0 * Vp), nc,
                            Q + (size_t)c0 * Vp, skip);
         RC_CHECK_LAUNCH();
     }
