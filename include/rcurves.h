@@ -84,13 +84,17 @@ typedef struct {
     int32_t gradient_mode;      /* dE by RC_MODE_ANALYTICAL (default) | RC_MODE_FINITE_DIFF
                                  * (SobolevStepOptions.mode, optimizer.ts:96,278-282) */
     double fd_h;                /* forward-difference step h, default 1e-6 (optimizer.ts:18) */
-    int32_t saddle_refine;      /* refinement steps of the saddle solution with residuals
-                                 * formed in double-double (compensated) arithmetic, default 1;
-                                 * 0 = off. No counterpart in the reference, whose FP64 LDL^T
-                                 * stops at its own rounding: kappa(A) ~ 1e9 at V = 8192 leaves
-                                 * an FP64 solve ~2e-8 from the exact solution of the assembled
-                                 * system, the refined one at its rounding. */
-    int32_t _pad3;
+    int32_t saddle_refine;      /* refinement steps of the saddle solution [g~; lambda] with
+                                 * residuals formed in double-double (compensated) arithmetic,
+                                 * default 1; 0 = off. No counterpart in the reference, whose
+                                 * FP64 LDL^T stops at its own rounding: kappa(A) ~ 1e9 at
+                                 * V = 8192 leaves an FP64 solve ~2e-8 from the exact solution
+                                 * of the assembled system, the refined one at its rounding. */
+    int32_t refine_columns;     /* refinement steps, before the above, of all 3(k+1) solve
+                                 * columns A_hat^-1 [dE, C^T] (compensated residuals), default
+                                 * 0: makes the frozen projection's corrections exact too (they
+                                 * use A_hat^-1 C^T), which matters when they are large (length
+                                 * targets that grow, many pins) */
     double saddle_refine_tol;   /* mixed form: PCG relative residual of each refinement
                                  * correction, default 1e-6 */
 } rc_step_opts;
@@ -122,8 +126,8 @@ typedef struct {
     int32_t refine_iters;
     double saddle_ir_residual;   /* compensated ||rhs - K z|| / max(1, ||rhs||) of the returned
                                   * solution (0 when saddle_refine = 0) */
-    double saddle_ir_correction; /* ||dx|| / ||g~|| of the last refinement correction */
-    int32_t saddle_ir_steps;     /* refinement corrections applied */
+    double saddle_ir_correction; /* relative size of the last refinement correction */
+    int32_t saddle_ir_steps;     /* refinement steps applied (columns + saddle) */
     int32_t _pad2;
 } rc_step_stats;
 

@@ -44,6 +44,9 @@ rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
 rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
                            int pcg_max_iter, double* d_out);
+rc_status rc_refine_columns(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
+                            int pcg_max_iter, double* d_out);
+rc_status rc_refine_zero(rc_ctx* ctx, double* d_out);
 
 namespace {
 
@@ -92,6 +95,7 @@ extern "C" void rc_default_step_opts(rc_step_opts* o) {
     o->gradient_mode = RC_MODE_ANALYTICAL;
     o->fd_h = 1e-6;
     o->saddle_refine = 1;
+    o->refine_columns = 0;
     o->saddle_refine_tol = 1e-6;
 }
 
@@ -687,18 +691,25 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
         if (s != RC_OK) return s;
         RC_HIP(hipMemsetAsync(ctx->d_refine, 0, sizeof(double) * 2, ctx->stream));
     }
+    // refinement with compensated residuals (k = 0 leaves A's null space
+    // unconstrained: no Schur system to correct through): optionally of all
+    // the solve's columns first (A_hat^-1 C^T exact for the frozen
+    // projection), then of the saddle solution [g~; lambda]
+    const bool refinable = k > 0 && !ctx->has_isolated;
+    const int ncol = refinable ? (o ? o->refine_columns : 0) : 0;
+    const int nref = refinable ? (o ? o->saddle_refine : 1) : 0;
+    const double ptol = o ? o->saddle_refine_tol : 1e-6;
+    const int pmax = o ? o->refine_max_iter : 200;
+    s = rc_refine_zero(ctx, ctx->d_refine + 2);
+    if (s == RC_OK && ncol > 0)
+        s = rc_refine_columns(ctx, k, precision, ncol, ptol, pmax, ctx->d_refine + 2);
+    if (s != RC_OK) return s;
     s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_gt, ctx->d_dots, ctx->d_bsum, ctx->d_ssol,
                         ctx->d_Minv);
     if (s != RC_OK) return s;
-    // refinement with compensated residuals (k = 0 leaves A's null space
-    // unconstrained: no Schur system to correct through)
-    const int nref = (k > 0 && !ctx->has_isolated) ? (o ? o->saddle_refine : 1) : 0;
     if (nref > 0) {
-        s = rc_saddle_refine(ctx, k, precision, nref, o ? o->saddle_refine_tol : 1e-6,
-                             o ? o->refine_max_iter : 200, ctx->d_refine + 2);
+        s = rc_saddle_refine(ctx, k, precision, nref, ptol, pmax, ctx->d_refine + 2);
         if (s != RC_OK) return s;
-    } else {
-        RC_HIP(hipMemsetAsync(ctx->d_refine + 2, 0, sizeof(double) * 4, ctx->stream));
     }
     hipStream_t rst = ctx->stream;
     if (resid_go && resid_done) {

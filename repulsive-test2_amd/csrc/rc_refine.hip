@@ -261,6 +261,124 @@ __global__ __launch_bounds__(256) void k_ir_rhob(const double* __restrict__ C,
     }
 }
 
+// Compensated R = B - A_hat X for columns 0 .. nc-1 (nc <= 32) of X, B, R
+// (ld Vp), A_hat = A + mu 1 1^T with A from the strict upper triangle of the
+// factor storage and its diagonal d; mu (1^T X) is added in FP64 (the 1
+// direction is where A_hat is well conditioned). One workgroup per 16-row
+// block walks the 64-column tiles of its rows: upper entries read along the
+// stored rows, lower ones as the stored transpose (16 consecutive rows of a
+// stored column), both into LDS; thread (row t % 16, column pair t / 16)
+// accumulates its two (sum, error) pairs over every column of A.
+__global__ __launch_bounds__(256) void k_ir_cols(const double* __restrict__ F,
+                                                 const double* __restrict__ d, int V, int Vp,
+                                                 const double* __restrict__ X,
+                                                 const double* __restrict__ B,
+                                                 const double* __restrict__ colsum,
+                                                 const double* __restrict__ mu, int nc,
+                                                 double* __restrict__ R) {
+    __shared__ double As[16][65];
+    __shared__ double Xs[64][33];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * 16;
+    const int ii = t & 15, cg = t >> 4;
+    cpair acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const int ntile = (V + 63) / 64;
+    for (int J = 0; J < ntile; ++J) {
+        const int jb = J * 64;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            // upper entries (j > i): columns fastest
+            {
+                const int r = e >> 6, c = e & 63, i = i0 + r, j = jb + c;
+                if (i < V && j < V && j > i) As[r][c] = F[(size_t)i * Vp + j];
+            }
+            // lower entries (j < i) from the stored transpose: rows fastest
+            {
+                const int r = e & 15, c = e >> 4, i = i0 + r, j = jb + c;
+                if (i < V && j < V && j < i) As[r][c] = F[(size_t)j * Vp + i];
+                if (j == i && i < V) As[r][c] = d[i];
+                if (i >= V || j >= V) As[r][c] = 0.0;
+            }
+        }
+        for (int e = t; e < 64 * 32; e += 256) {
+            const int c = e >> 6, jj = e & 63, j = jb + jj;
+            Xs[jj][c] = (c < nc && j < V) ? X[(size_t)c * Vp + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int jj = 0; jj < 64; ++jj) {
+            const double a = As[ii][jj];
+            cp_fma(acc[0], a, Xs[jj][2 * cg]);
+            cp_fma(acc[1], a, Xs[jj][2 * cg + 1]);
+        }
+    }
+    const int i = i0 + ii;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int c = 2 * cg + q;
+        if (c >= nc || i >= Vp) continue;
+        double r = 0.0;
+        if (i < V) {
+            double h, tt;
+            two_sum(B[(size_t)c * Vp + i], -acc[q].s, h, tt);
+            r = h + ((tt - acc[q].e) - mu[0] * colsum[c]);
+        }
+        R[(size_t)c * Vp + i] = r;
+    }
+}
+
+// out[c] = sum_v X[c][v] over v < V, one workgroup per column.
+__global__ __launch_bounds__(256) void k_ir_colsum(const double* __restrict__ X, int V, int Vp,
+                                                   double* __restrict__ out) {
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int v = threadIdx.x; v < V; v += 256) s += X[(size_t)c * Vp + v];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// X += Z over every column (v < V); dn[blockIdx.x * 2 + 0 / 1] = this
+// block's sums of Z^2 and X^2 over the first three columns (dE's solution).
+__global__ __launch_bounds__(256) void k_ir_addcols(double* __restrict__ X,
+                                                    const double* __restrict__ Z, int V, int Vp,
+                                                    int ncols, double* __restrict__ dn) {
+    __shared__ double sh[2][4];
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (v < V) {
+        for (int c = 0; c < ncols; ++c) {
+            const size_t j = (size_t)c * Vp + v;
+            const double z = Z[j];
+            const double x = X[j] + z;
+            X[j] = x;
+            if (c < 3) {
+                a += z * z;
+                b += x * x;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dn[blockIdx.x * 2] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+        dn[blockIdx.x * 2 + 1] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    }
+}
+
 // Relative residual ||rho|| / max(1, ||[dE; 0]||) of the current iterate
 // (block partials rn, rho_lambda) into out[0].
 __global__ __launch_bounds__(256) void k_ir_resnorm(const double* __restrict__ rn, int nparts,
@@ -417,17 +535,22 @@ __global__ void k_ir_accum_iters(double* __restrict__ out) { out[3] += out[5]; }
 
 // Workspace of the refinement (allocated on first use, sized for the
 // context's constraint-row capacity; dropped when that grows).
-static rc_status ir_bufs(rc_ctx* ctx) {
-    if (ctx->d_ir) return RC_OK;
+static size_t ir_cols_offset(const rc_ctx* ctx) {
     const size_t Vp = (size_t)ctx->nblk * RC_NB;
     const size_t nb = (size_t)ctx->nblk;
     const size_t n = (size_t)ctx->kcap + 3;
-    const size_t total = 2 * nb * nb * 3 * RC_NB   // compensated tile partials
-                         + 2 * 3 * Vp              // rho, Z
-                         + 2 * n                   // rho_lambda, w
-                         + n                       // delta
-                         + 2 * 3 * nb              // residual norm partials
-                         + 2 * 3 * ((Vp + 255) / 256) + 64;  // correction norm partials
+    return 2 * nb * nb * 3 * RC_NB + 2 * 3 * Vp + 3 * n + 2 * 3 * nb +
+           2 * 3 * ((Vp + 255) / 256) + 64;
+}
+
+static rc_status ir_bufs(rc_ctx* ctx) {
+    if (ctx->d_ir) return RC_OK;
+    const size_t Vp = (size_t)ctx->nblk * RC_NB;
+    const size_t nr = (size_t)ctx->nrhs_cap;
+    const size_t total = ir_cols_offset(ctx)       // the saddle residual's workspace
+                         + 2 * nr * Vp             // column residuals R, corrections Z
+                         + nr                      // column sums
+                         + 2 * ((Vp + 255) / 256); // correction norm partials
     RC_HIP(hipMalloc((void**)&ctx->d_ir, sizeof(double) * total));
     ctx->ir_doubles = total;
     return RC_OK;
@@ -441,6 +564,12 @@ static rc_status ir_bufs(rc_ctx* ctx) {
 // iterate, out[1] = corrections applied, out[2] = ||dx|| / ||g~|| of the
 // last one, out[3] = PCG iterations of the corrections (mixed form),
 // out[4..5] the mixed solve's own (residual, iterations) slots.
+rc_status rc_refine_zero(rc_ctx* ctx, double* d_out) {
+    hipLaunchKernelGGL(k_ir_zero, dim3(1), dim3(64), 0, ctx->stream, d_out, 6);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
 rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
                            int pcg_max_iter, double* d_out) {
     rc_status s = ir_bufs(ctx);
@@ -459,8 +588,6 @@ rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double
     double* dn = rn + 2 * 3 * (size_t)nb;
     const int ngx = (Vp + 255) / 256;
     const double* Y = ctx->d_sol + 3 * (size_t)Vp;
-    hipLaunchKernelGGL(k_ir_zero, dim3(1), dim3(64), 0, ctx->stream, d_out, 6);
-    RC_CHECK_LAUNCH();
     for (int it = 0; it <= nsteps; ++it) {
         hipLaunchKernelGGL(k_ir_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, ctx->stream,
                            (const double*)ctx->d_F, ctx->V, Vp, nb, (const double*)ctx->d_gt,
@@ -507,6 +634,68 @@ rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_ir_corrnorm, dim3(1), dim3(256), 0, ctx->stream, (const double*)dn,
                            3 * ngx, d_out + 2, d_out + 1, (const int32_t*)nullptr);
+        RC_CHECK_LAUNCH();
+        if (precision == RC_PRECISION_MIXED) {
+            hipLaunchKernelGGL(k_ir_accum_iters, dim3(1), dim3(1), 0, ctx->stream, d_out);
+            RC_CHECK_LAUNCH();
+        }
+    }
+    return RC_OK;
+}
+
+// Refinement of the solve's columns X = A_hat^-1 B (ctx->d_sol, the 3 + 3k
+// columns [dE, C^T], ld Vp) with compensated residuals R = B - A_hat X
+// (B rebuilt in ctx->d_rhs) and corrections Z = A_hat^-1 R: the FP64
+// factor's forward and backward sweeps (plain form) or PCG with the FP32
+// factor to a relative residual pcg_tol (mixed form); X += Z. With exact
+// columns the Schur step that follows forms M, lambda, g~ and every frozen
+// projection correction from the exact A_hat^-1 C^T. out[1] = steps,
+// out[2] = ||Z|| / ||X|| over dE's three columns in the last step, out[3] =
+// PCG iterations of the corrections (out[4..5] the mixed solve's slots).
+rc_status rc_launch_build_rhs(rc_ctx* ctx, const double* dE, int k, double* B, int ncols);
+
+rc_status rc_refine_columns(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
+                            int pcg_max_iter, double* d_out) {
+    rc_status s = ir_bufs(ctx);
+    if (s != RC_OK) return s;
+    const int Vp = ctx->nblk * RC_NB;
+    const int nrhs = 3 + 3 * k;
+    double* R = ctx->d_ir + ir_cols_offset(ctx);
+    double* Z = R + (size_t)ctx->nrhs_cap * Vp;
+    double* cs = Z + (size_t)ctx->nrhs_cap * Vp;
+    double* dn = cs + ctx->nrhs_cap;
+    const int ngx = (Vp + 255) / 256;
+    s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs, nrhs);
+    if (s != RC_OK) return s;
+    for (int it = 0; it < nsteps; ++it) {
+        hipLaunchKernelGGL(k_ir_colsum, dim3(nrhs), dim3(256), 0, ctx->stream,
+                           (const double*)ctx->d_sol, ctx->V, Vp, cs);
+        RC_CHECK_LAUNCH();
+        for (int c0 = 0; c0 < nrhs; c0 += 32) {
+            const int nc = nrhs - c0 < 32 ? nrhs - c0 : 32;
+            hipLaunchKernelGGL(k_ir_cols, dim3(Vp / 16), dim3(256), 0, ctx->stream,
+                               (const double*)ctx->d_F, (const double*)ctx->d_Adiag, ctx->V, Vp,
+                               (const double*)(ctx->d_sol + (size_t)c0 * Vp),
+                               (const double*)(ctx->d_rhs + (size_t)c0 * Vp),
+                               (const double*)(cs + c0), (const double*)(ctx->d_scal + 1), nc,
+                               R + (size_t)c0 * Vp);
+            RC_CHECK_LAUNCH();
+        }
+        double* corr = R;
+        if (precision == RC_PRECISION_MIXED) {
+            s = rc_solve_mixed(ctx, R, Z, nrhs, pcg_tol, pcg_max_iter, d_out + 4);
+            if (s != RC_OK) return s;
+            corr = Z;
+        } else {
+            s = rc_solve_fp64_forward(ctx, R, nrhs);
+            if (s == RC_OK) s = rc_solve_fp64_backward(ctx, R, nrhs);
+            if (s != RC_OK) return s;
+        }
+        hipLaunchKernelGGL(k_ir_addcols, dim3(ngx), dim3(256), 0, ctx->stream, ctx->d_sol,
+                           (const double*)corr, ctx->V, Vp, nrhs, dn);
+        RC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_ir_corrnorm, dim3(1), dim3(256), 0, ctx->stream, (const double*)dn,
+                           ngx, d_out + 2, d_out + 1, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
         if (precision == RC_PRECISION_MIXED) {
             hipLaunchKernelGGL(k_ir_accum_iters, dim3(1), dim3(1), 0, ctx->stream, d_out);

@@ -84,7 +84,7 @@ class StepOpts(C.Structure):
         ("gradient_mode", C.c_int32),
         ("fd_h", C.c_double),
         ("saddle_refine", C.c_int32),
-        ("_pad3", C.c_int32),
+        ("refine_columns", C.c_int32),
         ("saddle_refine_tol", C.c_double),
     ]
 

@@ -411,6 +411,7 @@ class SobolevStepOptions:
     # double-double refinement of the saddle solution (not a reference knob:
     # the FP64 solve alone stops ~kappa u_64 from the exact solution)
     saddle_refine: int = 1
+    refine_columns: int = 0       # also refine A_hat^-1 C^T (exact frozen projection)
     saddle_refine_tol: float = 1e-6
 
     def to_c(self) -> L.StepOpts:
@@ -430,20 +431,22 @@ class SobolevStepOptions:
         o.gradient_mode = _mode(self.mode)
         o.fd_h = float(self.h)
         o.saddle_refine = int(self.saddle_refine)
+        o.refine_columns = int(self.refine_columns)
         o.saddle_refine_tol = float(self.saddle_refine_tol)
         return o
 
 
 def solve_constrained_gradient_set_frozen(vertices, edges, disjoint_pairs, alpha, beta, epsilon,
                                           dE, cs, precision: str = "fp64", saddle_refine: int = 1,
-                                          refine_tol: float = 1e-12):
+                                          refine_tol: float = 1e-12, refine_columns: int = 0):
     """Returns (g_tilde (V,3), lambda (k,), residual)."""
     v = _verts(vertices)
     ctx = context_for(edges, len(v))
     _check_pairs(disjoint_pairs, ctx.E)
     arr, nb, keep = _blocks_array(cs)
     opts = SobolevStepOptions(alpha=alpha, beta=beta, epsilon=epsilon, precision=precision,
-                              saddle_refine=saddle_refine, refine_tol=refine_tol).to_c()
+                              saddle_refine=saddle_refine, refine_tol=refine_tol,
+                              refine_columns=refine_columns).to_c()
     k = L.lib().rc_constraint_rows(ctx.handle, arr, nb)
     gt = np.zeros_like(v)
     lam = np.zeros(max(k, 1))

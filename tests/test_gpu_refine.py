@@ -96,6 +96,13 @@ def test_refined_solve_is_exact(V, maker, precision):
     assert rel(g1, gx) <= 1e-13
     assert rel(l1, lx) <= 1e-12
     assert res <= 1e-9
+    # with the solve's columns refined first (exact A_hat^-1 C^T), the same
+    g2, l2, _ = core.solve_constrained_gradient_set_frozen(
+        v, e, None, 3.0, 6.0, 1e-10, dE, cs, precision=precision, saddle_refine=1,
+        refine_columns=1)
+    log("  columns refined too: dir err %.3e lam %.3e" % (rel(g2, gx), rel(l2, lx)))
+    assert rel(g2, gx) <= 1e-13
+    assert rel(l2, lx) <= 1e-12
 
 
 def test_fp64_factor_forward_error_v8192():
