@@ -149,6 +149,19 @@ const char* rc_last_error(const rc_ctx* ctx);
 
 /* calculateEnergy (tangentPointEnergy.ts:50-111). */
 rc_status rc_energy(rc_ctx* ctx, const double* xyz, const rc_params* p, double* out);
+/* The energy of rc_energy without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Device pointers only:
+ * d_xyz = SoA x[V], y[V], z[V]; d_edges = int32[E][2]; *d_energy receives the
+ * energy; work is enqueued on `stream` (a hipStream_t, NULL = the default
+ * stream) and the call returns without waiting. The edge table and partial
+ * sums live in a static device array of the library (2 MiB per GPU, held
+ * until the process ends): E <= 16384 (RC_UNSUPPORTED beyond). Calls that
+ * overlap on one GPU share that array and race. Endpoints outside [0, V) or
+ * self-loops give a NaN energy (they are not read). Same kernels and partial
+ * order as rc_energy: the same bits. */
+rc_status rc_energy_noalloc(const double* d_xyz, int32_t V, const int32_t* d_edges, int32_t E,
+                            const rc_params* p, double* d_energy, void* stream);
 /* gradientAnalytical (tangentPointEnergy.ts:139-392): adjoint of rc_energy,
  * AoS double[V][3] out, ascent orientation. Optionally also the energy. */
 rc_status rc_gradient(rc_ctx* ctx, const double* xyz, const rc_params* p, double* grad,

@@ -17,6 +17,7 @@
 #define RC_NB 64            // Cholesky / triangular-solve block size
 #define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
 #define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
+#define RC_NOALLOC_EMAX 16384  // largest E of rc_energy_noalloc (its static scratch)
 
 // Structure-of-arrays per-edge geometry refreshed by k_edge_prep for the
 // current positions. Index = edge id; padded entries (id >= E) are unused.

@@ -13,6 +13,9 @@
 // step s) so the J-side accumulators can travel with their edge through a
 // lane shuffle: every accumulator has one owner at every step, so there are
 // no atomics and the sum order is fixed (bit-reproducible run to run).
+#include <cmath>
+#include <mutex>
+
 #include "rc_internal.h"
 #include "rc_math.h"
 #include "rc_pair_energy.h"
@@ -34,13 +37,20 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Edge table of the current positions. Endpoints i1[K * is], i2[K * is]
+// (is = 1 for the context's tables, 2 for a caller's int32[E][2]). check:
+// endpoints are not trusted (a caller's device table): an endpoint outside
+// [0, V) or a self-loop reads no position and turns the edge's geometry into
+// NaN, so every energy of it is NaN.
 __global__ void k_edge_prep(const double* __restrict__ pos, int V, const int32_t* __restrict__ i1,
-                            const int32_t* __restrict__ i2, int E, int Epad, double alpha,
-                            double eps, EdgeSoA ed) {
+                            const int32_t* __restrict__ i2, int is, int check, int E, int Epad,
+                            double alpha, double eps, EdgeSoA ed) {
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= Epad) return;
     if (K < E) {
-        const int a = i1[K], b = i2[K];
+        int a = i1[(size_t)K * is], b = i2[(size_t)K * is];
+        const bool bad = check && (a < 0 || a >= V || b < 0 || b >= V || a == b);
+        if (bad) a = b = 0;
         const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
         const double bx = pos[b], by = pos[V + b], bz = pos[2 * V + b];
         const double ex = bx - ax, ey = by - ay, ez = bz - az;
@@ -62,6 +72,13 @@ __global__ void k_edge_prep(const double* __restrict__ pos, int V, const int32_t
         ed.dpc[K] = (1.0 - alpha) * pow(ell, -alpha);
         ed.i1[K] = a;
         ed.i2[K] = b;
+        if (bad) {
+            const double nan = __builtin_nan("");
+            ed.ax[K] = ed.ay[K] = ed.az[K] = ed.bx[K] = ed.by[K] = ed.bz[K] = nan;
+            ed.ell[K] = ed.rlen[K] = ed.lpow[K] = ed.dpc[K] = nan;
+            ed.i1[K] = -3;
+            ed.i2[K] = -4;
+        }
     } else {
         ed.ax[K] = ed.ay[K] = ed.az[K] = 0;
         ed.bx[K] = ed.by[K] = ed.bz[K] = 0;
@@ -545,8 +562,8 @@ int pair_grid(int npairs) {
 
 rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p) {
     hipLaunchKernelGGL(k_edge_prep, dim3((ctx->Epad + 255) / 256), dim3(256), 0, ctx->stream, pos,
-                       ctx->V, ctx->d_i1, ctx->d_i2, ctx->E, ctx->Epad, p->alpha, p->epsilon,
-                       ctx->ed);
+                       ctx->V, (const int32_t*)ctx->d_i1, (const int32_t*)ctx->d_i2, 1, 0, ctx->E,
+                       ctx->Epad, p->alpha, p->epsilon, ctx->ed);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
@@ -639,4 +656,84 @@ rc_status rc_download_positions_aos(rc_ctx* ctx, const double* d_src, double* xy
                           ctx->stream));
     RC_HIP(hipStreamSynchronize(ctx->stream));
     return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Energy without allocation (rc_energy_noalloc): the edge table and the
+// tile-pair partials of rc_energy live in this file-scope device array
+// instead of context buffers, sized for E <= RC_NOALLOC_EMAX. Each GPU holds
+// its own copy until the process ends; its device address is looked up once
+// per device (first call with that device current) and kept.
+
+namespace {
+
+constexpr int NA_EPAD = RC_NOALLOC_EMAX;                  // a multiple of RC_TILE
+constexpr int NA_TILES = NA_EPAD / RC_TILE;
+constexpr int NA_PAIRS = NA_TILES * (NA_TILES + 1) / 2;
+constexpr size_t NA_DOUBLES = 14 * (size_t)NA_EPAD + NA_PAIRS;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+namespace {
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mu;
+double* na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (the array's name is a
+// host-side stand-in and must never reach a kernel).
+rc_status scratch_on_current_device(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) return RC_HIP_ERROR;
+    std::lock_guard<std::mutex> lock(na_mu);
+    if (!na_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RC_HIP_ERROR;
+        na_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = na_ptr[dev];
+    return RC_OK;
+}
+
+}  // namespace
+
+extern "C" rc_status rc_energy_noalloc(const double* d_xyz, int32_t V, const int32_t* d_edges,
+                                       int32_t E, const rc_params* p, double* d_energy,
+                                       void* stream) {
+    if (!d_xyz || !d_edges || !d_energy || !p || V < 2 || E < 1) return RC_BAD_ARG;
+    if (!std::isfinite(p->alpha) || !std::isfinite(p->beta) || !std::isfinite(p->epsilon) ||
+        p->alpha == 0.0)
+        return RC_BAD_ARG;
+    if (E > RC_NOALLOC_EMAX) return RC_UNSUPPORTED;
+    double* base = nullptr;
+    rc_status s = scratch_on_current_device(&base);
+    if (s != RC_OK) return s;
+    const int Epad = ((E + RC_TILE - 1) / RC_TILE) * RC_TILE;
+    const int ntiles = Epad / RC_TILE;
+    const int npairs = ntiles * (ntiles + 1) / 2;
+    EdgeSoA ed;
+    double** fields[] = {&ed.ax, &ed.ay, &ed.az, &ed.bx, &ed.by, &ed.bz, &ed.ell,
+                         &ed.rlen, &ed.ux, &ed.uy, &ed.uz, &ed.lpow, &ed.dpc};
+    for (int i = 0; i < 13; ++i) *fields[i] = base + (size_t)i * Epad;
+    ed.i1 = reinterpret_cast<int32_t*>(base + 13 * (size_t)Epad);
+    ed.i2 = ed.i1 + Epad;
+    double* part = base + 14 * (size_t)Epad;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_edge_prep, dim3((Epad + 255) / 256), dim3(256), 0, st, d_xyz, V, d_edges,
+                       d_edges + 1, 2, 1, E, Epad, p->alpha, p->epsilon, ed);
+    const int grid = pair_grid(npairs);
+    if (is36(p)) {
+        hipLaunchKernelGGL(k_energy<Pow36>, dim3(grid), dim3(256), 0, st, ed, E, npairs, Pow36{},
+                           p->epsilon, part);
+    } else {
+        const double sx = (p->beta - 1) / p->alpha;
+        PowGeneric pw{p->alpha, p->beta, 2 * (sx - 1) + 1};
+        hipLaunchKernelGGL(k_energy<PowGeneric>, dim3(grid), dim3(256), 0, st, ed, E, npairs, pw,
+                           p->epsilon, part);
+    }
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, st, (const double*)part, npairs, 0.5,
+                       d_energy);
+    return hipGetLastError() == hipSuccess ? RC_OK : RC_HIP_ERROR;
 }

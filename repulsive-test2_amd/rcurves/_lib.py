@@ -129,6 +129,7 @@ EXPORTS = (
     "rc_bct_apply_b", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
     "rc_constraint_rows", "rc_sobolev_step", "rc_set_positions", "rc_get_positions",
     "rc_sobolev_step_resident", "rc_kernel_profile_enable", "rc_kernel_profile_read",
+    "rc_energy_noalloc",
 )
 
 _lib = None
@@ -152,6 +153,8 @@ def lib():
     L.rc_last_error.argtypes = [vp]
     L.rc_last_error.restype = C.c_char_p
     L.rc_energy.argtypes = [vp, _dp, C.POINTER(Params), _dp]
+    L.rc_energy_noalloc.restype = C.c_int
+    L.rc_energy_noalloc.argtypes = [vp, C.c_int32, vp, C.c_int32, C.POINTER(Params), vp, vp]
     L.rc_gradient.argtypes = [vp, _dp, C.POINTER(Params), _dp, _dp]
     L.rc_gradient_fd.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, _dp]
     L.rc_energy_bh.argtypes = [vp, _dp, C.POINTER(Params), C.c_double, C.c_double, C.c_int32,
