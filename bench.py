@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--V", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time only the mixed form (profiling; the line reports it as measured)")
     ap.add_argument("--no-kernel-profile", action="store_true",
                     help="time the FP64 steps without the per-launch events of the roofline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -292,6 +294,15 @@ def main():
     log("building workload")
     v, e, cs, pen = make_workload(args.V)
     ctx = L.Context(e, len(v))
+    if args.mixed_only:
+        rmx = run_form(ctx, v, cs, pen, "mixed", args.steps, args.warmup)
+        print(json.dumps({"metric": METRIC, "form": "mixed", "value": 1.0 / rmx["t"],
+                          "unit": "steps/s", "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": rmx["t"] * 1e3,
+                          "phase_ms": {k: round(x, 4) for k, x in rmx["phase"].items()}}),
+              flush=True)
+        ctx.close()
+        return
     r64 = run_form(ctx, v, cs, pen, "fp64", args.steps, args.warmup,
                    kprof=not args.no_kernel_profile)
     rmx = None
