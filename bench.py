@@ -85,7 +85,7 @@ def run_form(ctx, v0, cs, pen, precision, steps, warmup, kprof=False, projection
         e_prev = st.energy
         if record:
             for k, _ in L.Timings._fields_:
-                phase[k] += getattr(tm, k)
+                phase[k] += getattr(tm, k).ms
             trials.append(st.trials)
             accepted.append(st.accepted)
 

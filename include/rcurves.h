@@ -131,9 +131,29 @@ typedef struct {
     int32_t _pad2;
 } rc_step_stats;
 
-/* Per-phase milliseconds, keys of phaseTimings.ts:9-20 (sub-phases overlap). */
+/* PhaseSample (phaseTimings.ts:22-31): accumulated device time of one phase
+ * over a step (HIP events on the stream the phase runs on) and how often it
+ * fired; calls = 0 means the phase did not fire (absent from the reference's
+ * Partial ledger). */
 typedef struct {
-    double dE, energy, assembleA, saddle, factor, projection, lineSearch, step;
+    double ms;
+    int32_t calls;
+    int32_t _pad;
+} rc_phase;
+
+/* SobolevStepTimings (phaseTimings.ts:9-20, 32-36); sub-phases overlap their
+ * parents as in the reference: factor inside saddle, energy and projection
+ * inside lineSearch, the projection's correction solves are counted in saddle
+ * (and, reassembled, in assembleA and factor) as the reference's wraps count
+ * them. bHigh and bLow are one fused kernel here (both Gram matrices in one
+ * pass over the pair tiles), reported as assembleA; expand never fires (nor
+ * does it in the reference's flat path, gradient.ts:95-99). The frozen
+ * projection's correction solves run inside the projection kernel: their
+ * count is in saddle.calls, their time in projection.ms only. A singular
+ * system records neither factor nor saddle (the reference's factor throws
+ * inside both wraps). */
+typedef struct {
+    rc_phase dE, energy, assembleA, saddle, factor, projection, lineSearch, step;
 } rc_timings;
 
 /* Fills defaults (optimizer.ts:18, lineSearch.ts:207-209,435-438). */
@@ -219,8 +239,8 @@ rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xyz, const do
                                         double* residual);
 int32_t rc_constraint_rows(const rc_ctx* ctx, const rc_block* blocks, int32_t nblocks);
 
-/* sobolevStepSet (optimizer.ts:231-440), projectionMode 'frozen', dE by
- * o->gradient_mode. Positions in/out are host AoS; gtilde_out
+/* sobolevStepSet (optimizer.ts:231-440), projectionMode by
+ * o->projection_frozen, dE by o->gradient_mode. Positions in/out are host AoS; gtilde_out
  * (collectField -> descentField) and timings are nullable. Never fails on a
  * singular saddle: that folds into stats->reason like the reference. */
 rc_status rc_sobolev_step(rc_ctx* ctx, const double* xyz_in, const rc_block* blocks,

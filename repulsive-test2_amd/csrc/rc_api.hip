@@ -29,10 +29,9 @@ rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, cons
 rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
                              const double* e0tpe, const double* e0pen, int has_e0,
                              double e0_given, double tau0, LSCtl* d_ctl);
-rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
-                             const rc_params* p, const double* pos0, const double* dir,
-                             double* cur, const double* Y, const double* Minv,
-                             double* d_epart_ls);
+rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
+                               const rc_params* p, const double* pos0, const double* dir,
+                               double* cur, const double* Y, const double* Minv);
 rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
                                 int update, double* d_stats);
 rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const DevSet* d_set,
@@ -134,6 +133,7 @@ extern "C" void rc_ctx_destroy(rc_ctx* ctx) {
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->dep_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->led_ev) (void)hipEventDestroy(e);
     if (ctx->asm_go) (void)hipEventDestroy(ctx->asm_go);
     if (ctx->asm_done) (void)hipEventDestroy(ctx->asm_done);
     if (ctx->res_go) (void)hipEventDestroy(ctx->res_go);
@@ -639,9 +639,82 @@ __global__ void k_blocks_to_aos(const double* __restrict__ in, int V, int S,
 // resid_done (nullable): run the residual on the second stream and record
 // this event there when done (the caller waits for it before reading the
 // residual); otherwise it runs on the main stream.
+// Phase ledger of one step (phaseTimings.ts:9-78): event pairs on the
+// stream each phase runs on, summed per key after the step's final
+// synchronisation. A pair tagged with a trial index counts only when the line
+// search examined that trial (the frozen search launches trials in batches;
+// those past the decision return at once). Call counts the host cannot know
+// (trial energies, projection correction solves) come from device counters.
+enum LedgerKey { LK_DE, LK_ENERGY, LK_ASM, LK_SADDLE, LK_FACTOR, LK_PROJ, LK_LS, LK_STEP, LK_N };
+
+struct Ledger {
+    struct Mark {
+        int key, trial;
+        hipEvent_t a, b;
+    };
+    rc_ctx* ctx = nullptr;
+    std::vector<Mark> marks;
+    size_t used = 0;
+    int calls[LK_N] = {};
+
+    explicit Ledger(rc_ctx* c) : ctx(c) {}
+    hipError_t event(hipEvent_t& e) {
+        if (used == ctx->led_ev.size()) {
+            hipEvent_t n;
+            const hipError_t r = hipEventCreateWithFlags(&n, hipEventDisableSystemFence);
+            if (r != hipSuccess) return r;
+            ctx->led_ev.push_back(n);
+        }
+        e = ctx->led_ev[used++];
+        return hipSuccess;
+    }
+    // Opens a pair on stream st; *id receives its index for close().
+    hipError_t open(int key, hipStream_t st, int* id, int trial = -1) {
+        Mark m{key, trial, nullptr, nullptr};
+        hipError_t r = event(m.a);
+        if (r == hipSuccess) r = hipEventRecord(m.a, st);
+        if (r != hipSuccess) return r;
+        marks.push_back(m);
+        *id = (int)marks.size() - 1;
+        return hipSuccess;
+    }
+    hipError_t close(int id, hipStream_t st) {
+        hipError_t r = event(marks[id].b);
+        return r == hipSuccess ? hipEventRecord(marks[id].b, st) : r;
+    }
+    // Sums the pairs into t (trials counted: 0 .. ntrials - 1).
+    hipError_t sum(rc_timings* t, int ntrials) const {
+        rc_phase* ph[LK_N] = {&t->dE, &t->energy, &t->assembleA, &t->saddle,
+                              &t->factor, &t->projection, &t->lineSearch, &t->step};
+        for (const Mark& m : marks) {
+            if (m.trial >= ntrials || !m.b) continue;
+            float ms = 0.0f;
+            const hipError_t r = hipEventElapsedTime(&ms, m.a, m.b);
+            if (r != hipSuccess) return r;
+            ph[m.key]->ms += ms;
+        }
+        for (int k = 0; k < LK_N; ++k) ph[k]->calls += calls[k];
+        return hipSuccess;
+    }
+};
+
+// Records a ledger pair around a block of work when a ledger is armed.
+#define LED_OPEN(led, key, st, id, ...)                                              \
+    do {                                                                         \
+        if (led) RC_HIP((led)->open((key), (st), &(id), ##__VA_ARGS__));         \
+    } while (0)
+#define LED_CLOSE(led, id, st)                                                   \
+    do {                                                                         \
+        if (led) RC_HIP((led)->close((id), (st)));                               \
+    } while (0)
+
+// The gradient solve (gradient.ts:88-120): assembly unless the caller ran it
+// (gram_done), the constraint Jacobian, the factorisation, the solve and its
+// refinement and the Schur system; the structured residual follows on the
+// second stream when resid_go / resid_done are given. led: the step's ledger
+// (saddle and factor; null = not timed).
 static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, int precision,
-                             const rc_step_opts* o, hipEvent_t* ev_asm0, hipEvent_t* ev_asm1,
-                             hipEvent_t* ev_fac1, hipEvent_t gram_done = nullptr,
+                             const rc_step_opts* o, Ledger* led, hipEvent_t gram_done = nullptr,
                              hipEvent_t resid_go = nullptr, hipEvent_t resid_done = nullptr) {
     const int k = ds.k;
     RC_HIP(hipMemcpyAsync(ctx->d_set, &ds, sizeof(DevSet), hipMemcpyHostToDevice, ctx->stream));
@@ -649,11 +722,11 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     if (gram_done) {
         RC_HIP(hipStreamWaitEvent(ctx->stream, gram_done, 0));
     } else {
-        if (ev_asm0) RC_HIP(hipEventRecord(*ev_asm0, ctx->stream));
         s = rc_launch_gram(ctx, p);
         if (s != RC_OK) return s;
     }
-    if (ev_asm1) RC_HIP(hipEventRecord(*ev_asm1, ctx->stream));
+    int id_saddle = -1, id_factor = -1;
+    LED_OPEN(led, LK_SADDLE, ctx->stream, id_saddle);
     if (k > 0) {
         s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
         if (s != RC_OK) return s;
@@ -665,19 +738,22 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     const int nx = (nrhs + RC_NB - 1) / RC_NB;
     double* Frhs = ctx->d_F + (size_t)Vp * Vp;
     if (precision == RC_PRECISION_MIXED) {
+        LED_OPEN(led, LK_FACTOR, ctx->stream, id_factor);
         s = rc_factor_mixed(ctx);
     } else {
         s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
-        if (s == RC_OK) s = rc_factor_fp64(ctx, nx);
+        if (s != RC_OK) return s;
+        LED_OPEN(led, LK_FACTOR, ctx->stream, id_factor);
+        s = rc_factor_fp64(ctx, nx);
     }
     if (s != RC_OK) return s;
+    LED_CLOSE(led, id_factor, ctx->stream);
     if (ctx->has_isolated) {
         // isolated vertex: zero rows of A and C -> singular saddle (optimizer-sobolev.test.ts:83-105)
         int32_t one = 1;
         RC_HIP(hipMemcpyAsync(ctx->d_status, &one, sizeof(int32_t), hipMemcpyHostToDevice,
                               ctx->stream));
     }
-    if (ev_fac1) RC_HIP(hipEventRecord(*ev_fac1, ctx->stream));
     if (precision == RC_PRECISION_MIXED) {
         s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs, nrhs);
         if (s != RC_OK) return s;
@@ -710,6 +786,13 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     if (nref > 0) {
         s = rc_saddle_refine(ctx, k, precision, nref, ptol, pmax, ctx->d_refine + 2);
         if (s != RC_OK) return s;
+    }
+    // the structured residual, a diagnostic the line search does not read,
+    // runs beside it on the second stream: outside the saddle pair
+    LED_CLOSE(led, id_saddle, ctx->stream);
+    if (led) {
+        led->calls[LK_SADDLE] += 1;
+        led->calls[LK_FACTOR] += 1;
     }
     hipStream_t rst = ctx->stream;
     if (resid_go && resid_done) {
@@ -747,7 +830,7 @@ extern "C" rc_status rc_solve_constrained_gradient(rc_ctx* ctx, const double* xy
     hipLaunchKernelGGL(k_aos_to_blocks, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream,
                        ctx->d_work, ctx->V, Vp, ctx->d_dE);
     RC_CHECK_LAUNCH();
-    s = solve_phase(ctx, &o->params, ds, o->precision, o, nullptr, nullptr, nullptr);
+    s = solve_phase(ctx, &o->params, ds, o->precision, o, nullptr);
     if (s != RC_OK) return s;
     hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
                        ctx->d_gt, ctx->V, Vp, ctx->d_work);
@@ -796,7 +879,7 @@ static int trial_count(double tau0, double shrink, double tau_min) {
 // is reused, so the residual of the gradient solve (second stream) must be
 // done first.
 static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const DevSet& ds,
-                                        const DevPen& dp, int ntr, int max_iter) {
+                                        const DevPen& dp, int ntr, int max_iter, Ledger* led) {
     const int Vp = ctx->nblk * RC_NB;
     const int k = ds.k;
     const int nrhs = 3 + 3 * k;
@@ -807,6 +890,8 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
     int32_t* flag = (int32_t*)ctx->d_host_stage;
     rc_status s = RC_OK;
     for (int trial = 0; trial < ntr; ++trial) {
+        int id_proj = -1, id_e = -1;
+        LED_OPEN(led, LK_PROJ, ctx->stream, id_proj, trial);
         s = rc_launch_ls_project_re(ctx, 0, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
                                     ctx->d_work, Y, ctx->d_Minv);
         if (s != RC_OK) return s;
@@ -816,12 +901,21 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
             RC_HIP(hipStreamSynchronize(ctx->stream));
             if (!*flag) break;
             // A(cur), C(cur) and the factored saddle at the iterate
+            int id_asm = -1, id_sad = -1, id_fac = -1;
+            LED_OPEN(led, LK_ASM, ctx->stream, id_asm, trial);
             s = rc_launch_edge_prep(ctx, ctx->d_work, p);
             if (s == RC_OK) s = rc_launch_gram(ctx, p);
-            if (s == RC_OK && k > 0) s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
-            if (s == RC_OK) s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
-            if (s == RC_OK) s = rc_factor_fp64(ctx, nx);
             if (s != RC_OK) return s;
+            LED_CLOSE(led, id_asm, ctx->stream);
+            if (led) led->calls[LK_ASM] += 1;
+            LED_OPEN(led, LK_SADDLE, ctx->stream, id_sad, trial);
+            if (k > 0) s = rc_launch_constraint_jacobian(ctx, ctx->d_set, k);
+            if (s == RC_OK) s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
+            if (s != RC_OK) return s;
+            LED_OPEN(led, LK_FACTOR, ctx->stream, id_fac, trial);
+            s = rc_factor_fp64(ctx, nx);
+            if (s != RC_OK) return s;
+            LED_CLOSE(led, id_fac, ctx->stream);
             RC_HIP(hipMemcpyAsync(ctx->d_sol, Frhs, sizeof(double) * (size_t)nrhs * Vp,
                                   hipMemcpyDeviceToDevice, ctx->stream));
             s = rc_solve_fp64_backward(ctx, ctx->d_sol, nrhs);
@@ -829,13 +923,17 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
             if (s == RC_OK)
                 s = rc_launch_schur(ctx, ctx->d_sol, ctx->d_dE, k, ctx->d_tmp, ctx->d_dots,
                                     ctx->d_bsum, ctx->d_ssol, ctx->d_Minv);
-            if (s == RC_OK)
-                s = rc_launch_ls_project_re(ctx, 1, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos,
-                                            ctx->d_dir, ctx->d_work, Y, ctx->d_Minv);
+            if (s != RC_OK) return s;
+            LED_CLOSE(led, id_sad, ctx->stream);
+            s = rc_launch_ls_project_re(ctx, 1, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos,
+                                        ctx->d_dir, ctx->d_work, Y, ctx->d_Minv);
             if (s != RC_OK) return s;
         }
+        LED_CLOSE(led, id_proj, ctx->stream);
+        LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, trial);
         s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
         if (s != RC_OK) return s;
+        LED_CLOSE(led, id_e, ctx->stream);
         RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
                               hipMemcpyDeviceToHost, ctx->stream));
         RC_HIP(hipStreamSynchronize(ctx->stream));
@@ -850,7 +948,7 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
 // streaming M^{-1} and Y per correction was the single-workgroup kernel's
 // bound). Same checks, iteration count and failure rules as k_ls_project.
 static rc_status line_search_frozen_wide(rc_ctx* ctx, const rc_params* p, const DevPen& dp,
-                                         int k, int ntr, int max_iter) {
+                                         int k, int ntr, int max_iter, Ledger* led) {
     const int Vp = ctx->nblk * RC_NB;
     const double* Y = ctx->d_sol + 3 * (size_t)Vp;
     // solv (k + 3 doubles): the Schur workspace, idle once M^{-1} is formed
@@ -858,6 +956,8 @@ static rc_status line_search_frozen_wide(rc_ctx* ctx, const rc_params* p, const 
     int32_t* flag = (int32_t*)ctx->d_host_stage;
     rc_status s = RC_OK;
     for (int trial = 0; trial < ntr; ++trial) {
+        int id_proj = -1, id_e = -1;
+        LED_OPEN(led, LK_PROJ, ctx->stream, id_proj, trial);
         s = rc_launch_ls_project_re(ctx, 0, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
                                     ctx->d_work, Y, ctx->d_Minv);
         if (s != RC_OK) return s;
@@ -872,8 +972,11 @@ static rc_status line_search_frozen_wide(rc_ctx* ctx, const rc_params* p, const 
                                             ctx->d_dir, ctx->d_work, Y, ctx->d_Minv);
             if (s != RC_OK) return s;
         }
+        LED_CLOSE(led, id_proj, ctx->stream);
+        LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, trial);
         s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
         if (s != RC_OK) return s;
+        LED_CLOSE(led, id_e, ctx->stream);
         RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
                               hipMemcpyDeviceToHost, ctx->stream));
         RC_HIP(hipStreamSynchronize(ctx->stream));
@@ -908,11 +1011,10 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     const DevPen dp = resolve_pen(pen);
     const rc_params* p = &o->params;
     const int Vp = ctx->nblk * RC_NB;
-    const bool tm = o->collect_timings && timings;
-    hipEvent_t ev[10] = {};
-    if (tm)
-        for (auto& e : ev) RC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-    if (tm) RC_HIP(hipEventRecord(ev[0], ctx->stream));
+    Ledger ledger(ctx);
+    Ledger* led = (o->collect_timings && timings) ? &ledger : nullptr;
+    int id_step = -1, id_asm = -1, id_e0 = -1, id_de = -1, id_ls = -1;
+    LED_OPEN(led, LK_STEP, ctx->stream, id_step);
 
     s = rc_launch_edge_prep(ctx, ctx->d_pos, p);
     if (s != RC_OK) return s;
@@ -927,27 +1029,32 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     }
     RC_HIP(hipEventRecord(ctx->asm_go, ctx->stream));
     RC_HIP(hipStreamWaitEvent(ctx->stream2, ctx->asm_go, 0));
-    if (tm) RC_HIP(hipEventRecord(ev[3], ctx->stream2));
+    LED_OPEN(led, LK_ASM, ctx->stream2, id_asm);
     s = rc_launch_gram(ctx, p, ctx->stream2);
     if (s != RC_OK) return s;
-    if (tm) RC_HIP(hipEventRecord(ev[9], ctx->stream2));
+    LED_CLOSE(led, id_asm, ctx->stream2);
+    if (led) led->calls[LK_ASM] += 1;
     RC_HIP(hipEventRecord(ctx->asm_done, ctx->stream2));
     // E0 = objective at gamma0 unless the caller supplies it (optimizer.ts:117-132)
     if (!o->has_energy_before) {
+        LED_OPEN(led, LK_ENERGY, ctx->stream, id_e0);
         s = rc_launch_energy(ctx, p, ctx->d_scal + 2);
         if (s == RC_OK) s = rc_launch_penalty_energy(ctx, dp, ctx->d_scal + 3);
         if (s != RC_OK) return s;
+        LED_CLOSE(led, id_e0, ctx->stream);
+        if (led) led->calls[LK_ENERGY] += 1;
     }
-    if (tm) RC_HIP(hipEventRecord(ev[1], ctx->stream));
     // dE = gradient (analytic or forward difference) + penalty gradient
     // (optimizer.ts:278-294), stride Vp
+    LED_OPEN(led, LK_DE, ctx->stream, id_de);
     s = o->gradient_mode == RC_MODE_FINITE_DIFF ? rc_launch_gradient_fd(ctx, p, o->fd_h, ctx->d_grad)
                                                 : rc_launch_gradient(ctx, p, ctx->d_grad);
     if (s != RC_OK) return s;
     RC_HIP(hipMemsetAsync(ctx->d_dE, 0, sizeof(double) * 3 * Vp, ctx->stream));
     s = rc_launch_penalty_grad(ctx, dp, ctx->d_grad, ctx->V, ctx->d_dE, Vp);
     if (s != RC_OK) return s;
-    if (tm) RC_HIP(hipEventRecord(ev[2], ctx->stream));
+    LED_CLOSE(led, id_de, ctx->stream);
+    if (led) led->calls[LK_DE] += 1;
 
     // control block
     LSCtl hc;
@@ -962,10 +1069,9 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
 
     // the residual (a diagnostic the line search does not read) runs on the
     // second stream beside the line search
-    s = solve_phase(ctx, p, ds, o->precision, o, nullptr, tm ? &ev[4] : nullptr,
-                    tm ? &ev[5] : nullptr, ctx->asm_done, ctx->res_go, ctx->res_done);
+    s = solve_phase(ctx, p, ds, o->precision, o, led, ctx->asm_done, ctx->res_go, ctx->res_done);
     if (s != RC_OK) return s;
-    if (tm) RC_HIP(hipEventRecord(ev[6], ctx->stream));
+    LED_OPEN(led, LK_LS, ctx->stream, id_ls);
     s = rc_launch_ls_setup(ctx, ctx->d_gt, ctx->d_dE, ctx->d_dir, ctx->d_scal + 2,
                            ctx->d_scal + 3, o->has_energy_before, o->energy_before, o->tau0,
                            ctx->d_ctl);
@@ -980,19 +1086,25 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     const bool wide = o->projection_frozen && ds.k > RC_KSMALL;
     int done_trials = (o->projection_frozen && !wide) ? 0 : ntr;
     if (!o->projection_frozen) {
-        s = line_search_reassemble(ctx, p, ds, dp, ntr, o->max_iter);
+        s = line_search_reassemble(ctx, p, ds, dp, ntr, o->max_iter, led);
         if (s != RC_OK) return s;
     } else if (wide) {
-        s = line_search_frozen_wide(ctx, p, dp, ds.k, ntr, o->max_iter);
+        s = line_search_frozen_wide(ctx, p, dp, ds.k, ntr, o->max_iter, led);
         if (s != RC_OK) return s;
     }
     for (int bi = 0; bi < 3 && done_trials < ntr; ++bi) {
         const int nb = std::min(batches[bi], ntr - done_trials);
         for (int i = 0; i < nb; ++i) {
-            s = rc_launch_ls_trial(ctx, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
-                                   ctx->d_work, ctx->d_sol + 3 * (size_t)Vp, ctx->d_Minv,
-                                   ctx->d_epart_ls);
+            int id_proj = -1, id_e = -1;
+            LED_OPEN(led, LK_PROJ, ctx->stream, id_proj, done_trials + i);
+            s = rc_launch_ls_project(ctx, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
+                                     ctx->d_work, ctx->d_sol + 3 * (size_t)Vp, ctx->d_Minv);
             if (s != RC_OK) return s;
+            LED_CLOSE(led, id_proj, ctx->stream);
+            LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, done_trials + i);
+            s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
+            if (s != RC_OK) return s;
+            LED_CLOSE(led, id_e, ctx->stream);
         }
         done_trials += nb;
         if (done_trials < ntr) {
@@ -1002,7 +1114,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
             if (*(int32_t*)ctx->d_host_stage) break;
         }
     }
-    if (tm) RC_HIP(hipEventRecord(ev[7], ctx->stream));
+    LED_CLOSE(led, id_ls, ctx->stream);
     RC_HIP(hipStreamWaitEvent(ctx->stream, ctx->res_done, 0));
     s = rc_launch_ls_finalize(ctx, ctx->d_ctl, ctx->d_pos, ctx->d_work, update, ctx->d_stats);
     if (s != RC_OK) return s;
@@ -1029,7 +1141,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         RC_HIP(hipMemcpyAsync(xyz_out, ctx->d_stage, sizeof(double) * 3 * ctx->V,
                               hipMemcpyDeviceToHost, ctx->stream));
     }
-    if (tm) RC_HIP(hipEventRecord(ev[8], ctx->stream));
+    LED_CLOSE(led, id_step, ctx->stream);
     RC_HIP(hipStreamSynchronize(ctx->stream));
     if (hst[0] & 2) {
         ctx->err = "device dependency wait timed out (triangular solve or factorisation)";
@@ -1055,27 +1167,24 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     stats->saddle_ir_steps = (int32_t)d[25];
     stats->saddle_ir_correction = d[26];
     stats->_pad2 = 0;
-    if (tm) {
-        float ms[9] = {};
-        for (int i = 1; i <= 2; ++i) RC_HIP(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
+    if (led) {
+        // device-counted calls (trial energies, correction solves); the
+        // phases that did not fire report calls = 0 and no time
+        led->calls[LK_STEP] += 1;
+        led->calls[LK_ENERGY] += is[6];
+        led->calls[LK_SADDLE] += is[7];
+        if (!o->projection_frozen) led->calls[LK_FACTOR] += is[7];
+        led->calls[LK_PROJ] += stats->trials;
+        const bool singular = stats->reason == RC_REASON_SINGULAR_SYSTEM;
+        if (!singular && !stats->converged) led->calls[LK_LS] += 1;
+        if (singular) led->calls[LK_SADDLE] = led->calls[LK_FACTOR] = 0;
         std::memset(timings, 0, sizeof(*timings));
-        timings->energy = ms[1];
-        timings->dE = ms[2];
-        // ev[3]->ev[9]: assembly (second stream, beside energy and dE);
-        // ev[4]: the main stream has the assembled matrix
-        float asm_ms = 0, fac_ms = 0, sad_ms = 0, ls_ms = 0, step_ms = 0;
-        RC_HIP(hipEventElapsedTime(&asm_ms, ev[3], ev[9]));
-        RC_HIP(hipEventElapsedTime(&fac_ms, ev[4], ev[5]));
-        RC_HIP(hipEventElapsedTime(&sad_ms, ev[4], ev[6]));
-        RC_HIP(hipEventElapsedTime(&ls_ms, ev[6], ev[7]));
-        RC_HIP(hipEventElapsedTime(&step_ms, ev[0], ev[8]));
-        timings->assembleA = asm_ms;
-        timings->factor = fac_ms;
-        timings->saddle = sad_ms;
-        timings->lineSearch = ls_ms;
-        timings->projection = 0.0;
-        timings->step = step_ms;
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        RC_HIP(led->sum(timings, stats->trials));
+        rc_phase* ph[LK_N] = {&timings->dE, &timings->energy, &timings->assembleA,
+                              &timings->saddle, &timings->factor, &timings->projection,
+                              &timings->lineSearch, &timings->step};
+        for (rc_phase* q : ph)
+            if (q->calls == 0) q->ms = 0.0;
     }
     return RC_OK;
 }

@@ -157,6 +157,7 @@ struct rc_ctx {
     std::vector<hipEvent_t> kev;   // event pool, pairs around trailing-update launches
     std::vector<double> kev_flop;  // algorithmic flops of each recorded launch
     size_t kev_used = 0;           // events in use since the last read
+    std::vector<hipEvent_t> led_ev; // event pool of the step's phase ledger
     double kprof_ms = 0.0, kprof_flop = 0.0;
     int kprof_launches = 0;
 };

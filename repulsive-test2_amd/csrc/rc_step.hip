@@ -761,6 +761,8 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
         ctl->accepted = 0;
         ctl->converged = 0;
         ctl->trials = 0;
+        ctl->energy_evals = 0;
+        ctl->proj_solves = 0;
         ctl->proj_iters = 0;
         ctl->proj_iters_acc = -1;
         ctl->tau_acc = 0.0;
@@ -1055,7 +1057,7 @@ __global__ __launch_bounds__(CT) void k_ls_project(
         return;
     }
     trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
-    int ok = 0, it = 0;
+    int ok = 0, it = 0, nsolve = 0;
     double phin = INFINITY;
     const int maxit = ctl->max_iter;
     for (it = 0; it <= maxit; ++it) {
@@ -1065,10 +1067,12 @@ __global__ __launch_bounds__(CT) void k_ls_project(
         }
         if (it == maxit) break;
         // frozen correction with Y and M^{-1} of gamma0
+        ++nsolve;
         if (project_correct(set->k, Y, Minv, V, Vp, cur, phi, solv, sh))
             break;  // non-finite step -> ok = false (lineSearch.ts:283-288)
     }
     if (it > maxit) it = maxit;
+    if (threadIdx.x == 0) ctl->proj_solves += nsolve;
     project_finish(ctl, ok, it, phin, cur, ed, V, E, pen, vOff, vEdge, alpha, eps, sh);
 }
 
@@ -1115,10 +1119,15 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
             // frozen correction already applied to cur by k_corr_solv / k_corr_apply
             bad = ctl->corr_bad != 0;
             __syncthreads();
-            if (t == 0) ctl->corr_bad = 0;
+            if (t == 0) {
+                ctl->corr_bad = 0;
+                ctl->proj_solves += 1;
+            }
         } else {
             bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
             if (!bad) {
+                // a solve that completed (the reference's throws on a singular system)
+                if (t == 0) ctl->proj_solves += 1;
                 for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
                 __syncthreads();
                 bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
@@ -1228,6 +1237,8 @@ __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restri
         is[3] = ctl->reason;
         is[4] = ctl->trials;
         is[5] = ctl->refine_iters;
+        is[6] = ctl->energy_evals;
+        is[7] = ctl->proj_solves;
     }
 }
 
@@ -1284,6 +1295,7 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
     if (threadIdx.x == 0) {
         const double e1 = 0.5 * s + ctl->penE;
         ctl->trials += 1;
+        ctl->energy_evals += 1;
         ctl->counter = 0;
         const double tau = ctl->tau;
         if (e1 - e1 == 0.0 && e1 <= ctl->e0 - ctl->c1 * tau * ctl->slope) {
@@ -1440,16 +1452,16 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
 
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
 
-rc_status rc_launch_ls_trial(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
-                             const rc_params* p, const double* pos0, const double* dir,
-                             double* cur, const double* Y, const double* Minv,
-                             double* d_epart_ls) {
+// One trial's candidate and frozen projection; rc_launch_ls_energy follows.
+rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
+                               const rc_params* p, const double* pos0, const double* dir,
+                               double* cur, const double* Y, const double* Minv) {
     const int Vp = ctx->nblk * RC_NB;
     hipLaunchKernelGGL(k_ls_project, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, d_set, ctx->d_ell0,
                        pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv, ctx->d_vOff,
                        ctx->d_vEdge, p->alpha, p->epsilon);
     RC_CHECK_LAUNCH();
-    return rc_launch_ls_energy(ctx, d_ctl, p, d_epart_ls);
+    return RC_OK;
 }
 
 rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const DevSet* d_set,

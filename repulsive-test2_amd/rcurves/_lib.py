@@ -112,8 +112,12 @@ class StepStats(C.Structure):
     ]
 
 
+class Phase(C.Structure):
+    _fields_ = [("ms", C.c_double), ("calls", C.c_int32), ("_pad", C.c_int32)]
+
+
 class Timings(C.Structure):
-    _fields_ = [(k, C.c_double) for k in
+    _fields_ = [(k, Phase) for k in
                 ("dE", "energy", "assembleA", "saddle", "factor", "projection", "lineSearch", "step")]
 
 

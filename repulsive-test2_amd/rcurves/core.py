@@ -505,7 +505,9 @@ def sobolev_step_set(vertices, edges, disjoint_pairs, cs, opts: SobolevStepOptio
                         "saddleCorrection": st.saddle_ir_correction},
     }
     if tm is not None:
-        res["timings"] = {k: {"ms": getattr(tm, k), "calls": 1} for k, _ in L.Timings._fields_}
+        # only the phases that fired, like the reference's Partial ledger
+        res["timings"] = {k: {"ms": getattr(tm, k).ms, "calls": getattr(tm, k).calls}
+                          for k, _ in L.Timings._fields_ if getattr(tm, k).calls > 0}
     if opts.collect_field:
         res["descentField"] = None if res["stats"].get("reason") == "singular_system" else gt
     return res

@@ -61,8 +61,8 @@ def compute_arrow_field(vertices, edges, mode: str, descent_mode: str, x0) -> Op
 
 def dispatch_descent_step(args: dict) -> dict:
     """dispatchDescentStep (dispatch.ts:185-290): 'sobolev' -> the constrained
-    Sobolev step, 'raw' -> step() with the frame loop's arguments."""
-    if args.get("descentMode", "sobolev") != "sobolev":
+    Sobolev step, anything else -> step() with the frame loop's arguments."""
+    if args.get("descentMode") != "sobolev":
         r = core.step(args["vertices"], args["edges"], args.get("disjointPairs"),
                       core.StepOptions(mode=args.get("mode", "analytical"),
                                        step_size=args["stepSize"],
@@ -76,11 +76,23 @@ def dispatch_descent_step(args: dict) -> dict:
             "timings": None,
             "descentField": r.get("field"),
         }
-    if ("barycenterConstraint" not in args and "lengthConstraint" not in args
-            and "lengthMode" not in args):
-        cs = [core.barycenter_block(args["x0"])]
-    else:
-        cs = build_constraint_set(args)
+    if (args.get("barycenterConstraint") is None and args.get("lengthConstraint") is None
+            and args.get("lengthMode") is None):
+        # pre-toggle call shape (dispatch.ts:187-210): the barycenter-only
+        # sobolevStep with the mode alone (default projection, no penalties,
+        # no E0 reuse), never timings or a descent field
+        r = core.sobolev_step(args["vertices"], args["edges"], args.get("disjointPairs"),
+                              args["x0"], core.SobolevStepOptions(mode=args.get("mode", "analytical")))
+        return {
+            "vertices": r["vertices"],
+            "energy": r["energy"],
+            "accepted": r["accepted"],
+            "converged": r["converged"],
+            "stats": r["stats"],
+            "timings": None,
+            "descentField": None,
+        }
+    cs = build_constraint_set(args)
     pen = args.get("penalties")
     if isinstance(pen, dict):
         f = pen.get("field") or {}
@@ -126,7 +138,7 @@ class SolverWorker:
                 a = msg["args"]
                 fld = compute_arrow_field(a["vertices"], self.topology["edges"],
                                           a.get("mode", "analytical"),
-                                          a.get("descentMode", "sobolev"), a["x0"])
+                                          a.get("descentMode"), a["x0"])
                 return {"type": "fieldResult", "graphVersion": msg["graphVersion"], "field": fld}
             if self.topology is None:
                 raise RuntimeError("solverWorker: step received before topology")

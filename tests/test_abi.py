@@ -64,7 +64,7 @@ def _c_layout(struct, fields):
 
 @pytest.mark.parametrize("struct,cls", [
     ("rc_block", "Block"), ("rc_penalties", "Penalties"), ("rc_step_opts", "StepOpts"),
-    ("rc_step_stats", "StepStats"), ("rc_timings", "Timings"),
+    ("rc_step_stats", "StepStats"), ("rc_phase", "Phase"), ("rc_timings", "Timings"),
     ("rc_kernel_profile", "KernelProfile"), ("rc_params", "Params"),
 ])
 def test_struct_layouts_match_header(struct, cls):
@@ -75,3 +75,52 @@ def test_struct_layouts_match_header(struct, cls):
     assert layout[0] == ctypes.sizeof(py), (struct, layout[0], ctypes.sizeof(py))
     for name, off in zip(names, layout[1:]):
         assert getattr(py, name).offset == off, (struct, name, off)
+
+
+def test_phase_ledger_keys():
+    """SobolevPhaseKey (phaseTimings.ts:9-20) minus the keys that do not fire
+    on the device: bHigh and bLow are one fused Gram kernel timed as
+    assembleA, and expand never fires (as in the reference's flat path)."""
+    reference = ["dE", "energy", "bHigh", "bLow", "assembleA", "expand", "saddle", "factor",
+                 "projection", "lineSearch", "step"]
+    fused = {"bHigh", "bLow", "expand"}
+    assert [k for k, _ in _lib.Timings._fields_] == [
+        "dE", "energy", "assembleA", "saddle", "factor", "projection", "lineSearch", "step"]
+    assert {k for k, _ in _lib.Timings._fields_} == set(reference) - fused
+    # PhaseSample {ms, calls} per key
+    assert [f[0] for f in _lib.Phase._fields_][:2] == ["ms", "calls"]
+    assert all(t is _lib.Phase for _, t in _lib.Timings._fields_)
+
+
+def test_ts_binding_layouts_match_header():
+    """ts/sobolevFfi.ts packs rc_step_opts, rc_step_stats, rc_timings and
+    rc_block by byte offset (no Bun runtime here to run it): its constants
+    must equal a compiled probe of the header."""
+    import re
+
+    src = open(os.path.join(ROOT, "repulsive-test2_amd", "ts", "sobolevFfi.ts")).read()
+
+    def const(name):
+        return int(re.search(rf"const {name} = (\d+);", src).group(1))
+
+    def table(name):
+        body = re.search(rf"const {name} = \{{(.*?)\}} as const;", src, re.S).group(1)
+        return {k: int(v) for k, v in re.findall(r"(\w+): (\d+)", body)}
+
+    sizes = {"rc_step_opts": "OPTS_BYTES", "rc_step_stats": "STATS_BYTES",
+             "rc_timings": "TIMINGS_BYTES", "rc_phase": "PHASE_BYTES", "rc_block": "BLOCK_BYTES"}
+    for struct, name in sizes.items():
+        assert _c_layout(struct, [])[0] == const(name), struct
+    params = {"alpha", "beta", "epsilon"}
+    for struct, name in (("rc_step_opts", "OPT"), ("rc_step_stats", "STAT"), ("rc_block", "BLK")):
+        tab = table(name)
+        fields = [f for f in tab if f not in params]
+        lay = _c_layout(struct, fields)
+        for f, off in zip(fields, lay[1:]):
+            assert tab[f] == off, (struct, f, tab[f], off)
+        if name == "OPT":
+            p = _c_layout("rc_params", ["alpha", "beta", "epsilon"])
+            assert [tab[k] for k in ("alpha", "beta", "epsilon")] == p[1:]
+    # the ledger keys in the order of rc_timings
+    keys = re.search(r"const PHASE_KEYS = \[(.*?)\] as const;", src, re.S).group(1)
+    assert re.findall(r"'(\w+)'", keys) == [k for k, _ in _lib.Timings._fields_]

@@ -81,3 +81,77 @@ def test_raw_step_with_unknown_gradient_mode_is_an_error():
                     "args": {"vertices": v, "x0": [0, 0, 0], "descentMode": "raw",
                              "mode": "centralDiff", "stepSize": 1e-3}})
     assert out["type"] == "error" and "unknown gradient mode" in out["message"]
+
+
+def test_legacy_branch_passes_the_mode_alone(monkeypatch):
+    """dispatch.ts:187-210: with no M-set toggle the barycenter-only
+    sobolevStep runs with {mode} only (default projection, no penalties, no
+    E0 reuse, nothing collected) and returns null timings and field."""
+    seen = {}
+
+    def fake_step(vertices, edges, disjoint_pairs, x0, opts):
+        seen["x0"], seen["opts"] = x0, opts
+        return {"vertices": vertices, "energy": 1.5, "accepted": True, "converged": False,
+                "stats": {"tau": 1.0}, "timings": {"step": {"ms": 1.0, "calls": 1}},
+                "descentField": np.zeros_like(vertices)}
+
+    monkeypatch.setattr(core, "sobolev_step", fake_step)
+    monkeypatch.setattr(core, "sobolev_step_set", lambda *a, **k: pytest.fail("toggle path taken"))
+    v, e = fixture()
+    out = dispatch.dispatch_descent_step({
+        "descentMode": "sobolev", "vertices": v, "edges": e, "x0": [0.5, 0.5, 0.0],
+        "mode": "finiteDiff", "collectTimings": True, "collectField": True,
+        "energyBefore": 3.0, "projectionMode": "frozen", "penalties": {"totalLength": 2.0},
+        "precision": "mixed"})
+    o = seen["opts"]
+    default = core.SobolevStepOptions()
+    assert o.mode == "finiteDiff"
+    assert o.projection_mode == default.projection_mode == "reassemble"
+    assert o.penalties is None and o.energy_before is None
+    assert not o.collect_timings and not o.collect_field
+    assert o.precision == default.precision
+    assert seen["x0"] == [0.5, 0.5, 0.0]
+    assert out["timings"] is None and out["descentField"] is None
+    assert out["energy"] == 1.5 and out["accepted"] is True and out["stats"] == {"tau": 1.0}
+
+
+def test_toggle_present_takes_the_set_path(monkeypatch):
+    """Any one toggle (even False / 'none') selects the ConstraintSet path."""
+    calls = []
+    monkeypatch.setattr(core, "sobolev_step", lambda *a, **k: pytest.fail("legacy path taken"))
+    monkeypatch.setattr(core, "sobolev_step_set",
+                        lambda v, e, dp, cs, opts: calls.append(([b.kind for b in cs], opts)) or
+                        {"vertices": v, "energy": 0.0, "accepted": False, "converged": False,
+                         "stats": {}})
+    v, e = fixture()
+    base = {"descentMode": "sobolev", "vertices": v, "edges": e, "x0": [0, 0, 0],
+            "mode": "analytical", "sobolevL0": 4.0}
+    for extra, kinds in (({"barycenterConstraint": False}, ["totalLength"]),
+                         ({"lengthConstraint": False}, ["barycenter"]),
+                         ({"lengthMode": "none"}, ["barycenter"])):
+        out = dispatch.dispatch_descent_step(dict(base, **extra))
+        assert calls[-1][0] == kinds
+        assert out["timings"] is None and out["descentField"] is None
+
+
+def test_absent_descent_mode_is_the_raw_step(monkeypatch):
+    """dispatch.ts:185-188 takes the Sobolev branch only for 'sobolev'."""
+    seen = {}
+
+    def fake_raw(vertices, edges, dp, opts):
+        seen["opts"] = opts
+        return {"vertices": vertices, "energy": 2.0}
+
+    monkeypatch.setattr(core, "step", fake_raw)
+    v, e = fixture()
+    out = dispatch.dispatch_descent_step({"vertices": v, "edges": e, "x0": [0, 0, 0],
+                                          "mode": "analytical", "stepSize": 1e-3})
+    assert seen["opts"].step_size == 1e-3
+    assert out["stats"] is None and out["timings"] is None and out["descentField"] is None
+    assert out["accepted"] is True and out["converged"] is False
+
+
+def test_tangent_point_energy_backward_needs_forward():
+    tpe = core.TangentPointEnergy(fixture()[1], 4)
+    with pytest.raises(RuntimeError, match="before forward"):
+        tpe.backward()

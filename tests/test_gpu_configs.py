@@ -144,6 +144,7 @@ def test_worker_field_is_the_constrained_gradient(goldens):
                                                           [core.barycenter_block(x0)])
     w = dispatch.SolverWorker()
     w.handle({"type": "topology", "graphVersion": 555, "edges": e})
-    resp = w.handle({"type": "field", "graphVersion": 555, "args": {"vertices": v, "x0": x0}})
+    resp = w.handle({"type": "field", "graphVersion": 555,
+                     "args": {"vertices": v, "x0": x0, "descentMode": "sobolev"}})
     assert resp["type"] == "fieldResult" and resp["graphVersion"] == 555
     assert rel(resp["field"], gt) <= 1e-12
