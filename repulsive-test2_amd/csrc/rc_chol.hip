@@ -18,10 +18,10 @@
 //   k_panel_trsm  L_ik = A_ik L_kk^-T as a GEMM with the stored inverse
 //   k_syrk128     A_ij -= L_ik L_jk^T on 128x64 lower blocks, K = 128,
 //                 v_mfma_f64_16x16x4_f64, XCD-grouped block order, on a second
-//                 stream; the next panel's column is updated separately
-//                 (k_col_diag for its diagonal tile on the critical path,
-//                 k_col_rows below it on the second stream), so the next
-//                 panel's factorisation overlaps the big update
+//                 stream; the next panel's column goes first in the same
+//                 launch (its diagonal tile is updated by k_col_diag on the
+//                 critical path), so the next panel's factorisation overlaps
+//                 the bulk of the update
 // Mixed form, 64-column panels:
 //   k_potrf_panel<float>  every block factors the 64x64 diagonal block in
 //                 FP64 in LDS and inverts it; block 0 stores L_kk^-1, the
@@ -647,7 +647,7 @@ __device__ __forceinline__ void signal_done(int32_t* cnt) {
 
 // Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
 // workgroup, in place. wcnt (nullable): the previous panel's column update
-// (k_col_rows, other stream) must have counted `target` workgroups first.
+// (k_syrk128 column blocks, other stream) must have counted `target` first.
 // A_hat = A + mu on operand values of the V x V part (first panel).
 struct ShiftA {
     double mu;
@@ -689,35 +689,85 @@ __global__ __launch_bounds__(256) void k_panel_trsm(double* __restrict__ F, int 
                     acc[a][b][r];
 }
 
-// Trailing update A_ij -= L_ik L_jk^T right of the first trailing column
-// (that column, the next panel, is updated by k_col_diag / k_col_rows), over
-// 128 x 64 blocks of the lower triangle: row tile ti (128 rows) and column
-// half hj (64 columns), 0 <= hj <= 2 ti + 1, numbered p = ti (ti + 1) + hj,
-// ti, hj counted from the second trailing 128-column tile. Per workgroup:
-// 4 waves of 64 x 32, K = 128 (the panel) on v_mfma_f64_16x16x4_f64, the
-// block's 32 C values per lane prefetched into registers at the start.
-// Workgroup b runs on XCD b % 8, so each XCD is given a contiguous range of
-// block numbers (blocks of one row share their A panel in that XCD's L2).
-__device__ __forceinline__ void decode_half(int p, int& ti, int& hj) {
-    int t = (int)((__builtin_sqrt(4.0 * (double)p + 1.0) - 1.0) * 0.5);
-    while ((t + 1) * (t + 2) <= p) ++t;
-    while (t * (t + 1) > p) --t;
+// Trailing update with panel kb: A_ij -= L_ik L_jk^T for everything right
+// of column kb except the next diagonal tile (kb+1, kb+1), which k_col_diag
+// updates on the critical path. One launch per panel on the second stream,
+// its workgroups in three groups, dispatched in this order:
+//   A  column kb+1 below its diagonal tile (2 (nt - 1) blocks of 64 x 128:
+//      two per 128-row tile) and the right-hand-side rows of column kb+1
+//      (nx blocks of 64 x 128): the next panel's TRSM waits for all of them
+//      (col_done);
+//   B  the diagonal tile (kb+2, kb+2) as two 128 x 64 blocks, which
+//      k_col_diag of the next panel waits for (diag_done); then the
+//      right-hand-side rows of columns kb+2 .. (nx (nt - 1) blocks of
+//      64 x 128), needed only by later panels;
+//   C  from the next multiple of 8: the 128 x 64 blocks of the lower triangle
+//      right of column kb+1 below tile (kb+2, kb+2) (row tile ti >= 1 and
+//      column half hj, 0 <= hj <= 2 ti + 1, both counted from tile kb+2).
+//      Workgroup b runs on XCD b % 8, and each XCD is given a contiguous
+//      range of them in row order (blocks of one row share their A panel in
+//      that XCD's L2).
+// A and B are small and go first, so the next panel's factorisation and
+// TRSM start while C (the bulk) is still running, with nothing waiting in
+// between: the next panel's TRSM is only launched after its diagonal
+// factorisation, by when group A is long done. 64 x 128 blocks: 4 waves of
+// 32 x 64; 128 x 64 blocks: 4 waves of 64 x 32; K = 128 (the panel) on
+// v_mfma_f64_16x16x4_f64, the block's 32 C values per lane prefetched into
+// registers at the start. The first panel's update reads A itself:
+// A_hat = A + mu on the V x V part.
+__device__ __forceinline__ void decode_rest(int q, int& ti, int& hj) {
+    // q-th 128 x 64 block of the rows ti >= 1 in row order (row ti holds
+    // 2 ti + 2 blocks, (ti - 1)(ti + 2) before it)
+    int t = (int)((__builtin_sqrt(9.0 + 4.0 * (double)q) - 1.0) * 0.5);
+    while (t > 1 && (t - 1) * (t + 2) > q) --t;
+    while (t * (t + 3) <= q) ++t;
     ti = t;
-    hj = p - t * (t + 1);
+    hj = q - (t - 1) * (t + 2);
 }
 
-__global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
-                                                 int nblocks, const double* __restrict__ mu, int V,
-                                                 int32_t* __restrict__ diag_done) {
-    __shared__ double As[2][KC_SYRK][LDT];
-    __shared__ double Bs[2][KC_SYRK][LDT];
-    const int per = (nblocks + 7) >> 3;
-    const int p = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (p >= nblocks) return;
-    int ti, hj;
-    decode_half(p, ti, hj);
-    const size_t k0 = (size_t)kb * PB;
-    const size_t ri = k0 + 2 * PB + (size_t)ti * PB, rj = k0 + 2 * PB + (size_t)hj * 64;
+// C_rows(ri, rj) (64 x 128) -= P_i P_j^T with the panel columns k0 ..
+__device__ __forceinline__ void update_64x128(double* __restrict__ F, int Vp, size_t k0,
+                                              size_t ri, size_t rj, double mu0, int vi, int vj,
+                                              double (*As)[KC_TRSM][LDT],
+                                              double (*Bs)[KC_TRSM][LDT]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
+    double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
+    d4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    double cv[2][4][4];
+    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc, [&](int ch) {
+        if (ch == 0) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
+        }
+    });
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = wr + 16 * a + 4 * r + (lane >> 4), j = wc + 16 * b + (lane & 15);
+                const double c = (i < vi && j < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
+                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
+            }
+}
+
+// C(ri, rj) (128 x 64) -= P_i P_j^T; diag: the block lies on a diagonal
+// tile (only j <= i is stored), jo its column offset in that tile
+__device__ __forceinline__ void update_128x64(double* __restrict__ F, int Vp, size_t k0,
+                                              size_t ri, size_t rj, bool diag, int jo, double mu0,
+                                              int vi, int vj, double (*As)[KC_SYRK][LDT],
+                                              double (*Bs)[KC_SYRK][LDT]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
     double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
@@ -739,11 +789,6 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
                         cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
         }
     });
-    const bool diag = (hj >> 1) == ti;
-    const int jo = (hj & 1) * 64;
-    // the first panel's update reads A itself: A_hat = A + mu on the V x V part
-    const double mu0 = (kb == 0) ? mu[0] : 0.0;
-    const int vi = (kb == 0) ? V - (int)ri : 0, vj = (kb == 0) ? V - (int)rj : 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -756,69 +801,70 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
                 if (!diag || j <= i)
                     C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
             }
-    // blocks 0 and 1 (scheduled first) are the diagonal tile of the panel
-    // after next, which k_col_diag of the next panel waits for
-    if (p < 2) signal_done(diag_done);
 }
 
-// Update of the next panel's column (the first trailing column) with panel
-// kb, below its diagonal tile: tiles (i, kb+1), i > kb+1, as 64 x 128 blocks
-// (two per 128-row tile, nt - 1 tiles; workgroups 0 .. 2(nt-1)-1). Runs
-// beside the next panel's diagonal factorisation, which only needs the
-// diagonal tile (k_col_diag). The workgroups after those update the
-// right-hand-side rows below the factor (rows Vp ..): 64-row block x, all nt
-// trailing columns tj, workgroup 2(nt-1) + x nt + tj.
-__global__ __launch_bounds__(256, 2) void k_col_rows(double* __restrict__ F, int Vp, int kb,
-                                                     int nt, const double* __restrict__ mu, int V,
-                                                     int32_t* __restrict__ done) {
-    __shared__ double As[2][KC_TRSM][LDT];
-    __shared__ double Bs[2][KC_TRSM][LDT];
-    const size_t k0 = (size_t)kb * PB;
-    size_t ri, rj;
-    const int nreg = 2 * (nt - 1);
-    if ((int)blockIdx.x < nreg) {
-        const int ti = 1 + (blockIdx.x >> 1), h = blockIdx.x & 1;
-        ri = k0 + PB + (size_t)ti * PB + 64 * h;
-        rj = k0 + PB;
-    } else {
-        const int q = (int)blockIdx.x - nreg;
-        ri = (size_t)Vp + 64 * (size_t)(q / nt);
-        rj = k0 + PB + (size_t)(q % nt) * PB;
+// Workgroup counts of the trailing update of panel kb (nt = trailing
+// 128-column tiles, nx right-hand-side row blocks).
+struct TrailShape {
+    int na, nb, rest0, nrest, grid;
+    __host__ __device__ TrailShape(int nt, int nx) {
+        const int nc = nt - 1;
+        na = 2 * (nt - 1) + nx;
+        nb = nc > 0 ? 2 + nx * nc : 0;
+        rest0 = (na + nb + 7) & ~7;
+        nrest = nc > 0 ? nc * (nc + 1) - 2 : 0;
+        grid = rest0 + 8 * ((nrest + 7) / 8);
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
-    double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
-    d4 acc[2][4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    double cv[2][4][4];
-    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc, [&](int ch) {
-        if (ch == 0) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        cv[a][b][r] = C[(size_t)(16 * a + 4 * r) * Vp + 16 * b];
-        }
-    });
-    // first panel: A_hat = A + mu on the V x V part (not the right-hand sides)
+};
+
+__global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int Vp, int kb,
+                                                 int nt, int nx, const double* __restrict__ mu,
+                                                 int V, int32_t* __restrict__ col_done,
+                                                 int32_t* __restrict__ diag_done) {
+    __shared__ double As[2][KC_SYRK][LDT];
+    __shared__ double Bs[2][KC_SYRK][LDT];
+    const TrailShape sh(nt, nx);
+    const int wg = blockIdx.x;
+    const size_t k0 = (size_t)kb * PB;
+    const size_t c1 = k0 + PB;       // column kb+1 (the next panel)
+    const size_t c2 = k0 + 2 * PB;   // column kb+2
     const double mu0 = (kb == 0) ? mu[0] : 0.0;
-    const int vi = (kb == 0) ? V - (int)ri : 0, vj = (kb == 0) ? V - (int)rj : 0;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = wr + 16 * a + 4 * r + (lane >> 4), j = wc + 16 * b + (lane & 15);
-                const double c = (i < vi && j < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
-                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
-            }
-    signal_done(done);
+    if (wg < sh.na) {
+        // A: column kb+1 below its diagonal tile, then its right-hand-side rows
+        const int nreg = 2 * (nt - 1);
+        const size_t ri = wg < nreg ? c1 + PB + (size_t)(wg >> 1) * PB + 64 * (wg & 1)
+                                    : (size_t)Vp + 64 * (size_t)(wg - nreg);
+        const bool vv = kb == 0 && wg < nreg;
+        update_64x128(F, Vp, k0, ri, c1, vv ? mu0 : 0.0, vv ? V - (int)ri : 0,
+                      vv ? V - (int)c1 : 0, As, Bs);
+        signal_done(col_done);
+        return;
+    }
+    if (wg < sh.na + sh.nb) {
+        const int q = wg - sh.na;
+        if (q < 2) {
+            // B: the diagonal tile (kb+2, kb+2), column half q
+            const size_t rj = c2 + 64 * (size_t)q;
+            update_128x64(F, Vp, k0, c2, rj, true, 64 * q, mu0, kb == 0 ? V - (int)c2 : 0,
+                          kb == 0 ? V - (int)rj : 0, As, Bs);
+            signal_done(diag_done);
+        } else {
+            // B: right-hand-side rows of columns kb+2 ..
+            const int r = q - 2, nc = nt - 1;
+            update_64x128(F, Vp, k0, (size_t)Vp + 64 * (size_t)(r / nc),
+                          c2 + (size_t)(r % nc) * PB, 0.0, 0, 0, As, Bs);
+        }
+        return;
+    }
+    if (wg < sh.rest0) return;
+    const int r = wg - sh.rest0, per = (sh.nrest + 7) >> 3;
+    const int q = (r & 7) * per + (r >> 3);
+    if (q >= sh.nrest) return;
+    int ti, hj;
+    decode_rest(q, ti, hj);
+    const size_t ri = c2 + (size_t)ti * PB, rj = c2 + (size_t)hj * 64;
+    update_128x64(F, Vp, k0, ri, rj, (hj >> 1) == ti, (hj & 1) * 64, mu0,
+                  kb == 0 ? V - (int)ri : 0, kb == 0 ? V - (int)rj : 0, As, Bs);
 }
 
 // Update of the next panel's 128 x 128 diagonal tile with panel kb, on the
@@ -1266,23 +1312,24 @@ rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
 // GEMM with the inverse, MFMA trailing update) with a one-panel lookahead on
 // two streams. Main stream, per panel kb: potrf(kb) -> panel TRSM(kb) ->
 // update of the next diagonal tile (k_col_diag), the critical path. Second
-// stream: the rest of column kb+1 (k_col_rows) and then the rest of the
-// trailing matrix (k_syrk128), both with panel kb, overlapping the next
-// panel's diagonal factorisation. Ordering: the event T (panel kb final)
-// before the second stream's updates of kb; device counters (d_fcnt, cleared
-// per factorisation) for the main stream's waits, which a stream event would
-// turn into a command-processor barrier of ~5-10 us on the critical path:
-// panel TRSM(kb+1) polls for all k_col_rows(kb) workgroups, k_col_diag(kb+1)
-// for the two k_syrk128(kb) blocks of its tile (scheduled first). Both
-// consumers are small and their producers were launched earlier on the other
-// stream, so a poll always ends; a bound on it sets status bit 2. (Waiting
-// for the whole previous update with a stream event while it is large was
-// measured slower on the same box: 87.6-88.6 against 90.9-91.2 steps/s.)
+// stream: the trailing update with panel kb (k_syrk128), whose first
+// workgroups update column kb+1 and the diagonal tile after next, so the
+// next panel's diagonal factorisation and TRSM overlap its bulk. Ordering:
+// the event T (panel kb final) before the second stream's update of kb;
+// device counters (d_fcnt, cleared per factorisation) for the main stream's
+// waits, which a stream event would turn into a command-processor barrier of
+// ~5-10 us on the critical path: panel TRSM(kb+1) polls for the column
+// blocks of k_syrk128(kb), k_col_diag(kb+1) for its two diagonal-tile blocks.
+// Both consumers are small, their producers were launched earlier on the
+// other stream and are dispatched first, so a poll always ends; a bound on
+// it sets status bit 2. (Waiting for the whole previous update with a stream
+// event while it is large was measured slower on the same box: 87.6-88.6
+// against 90.9-91.2 steps/s.)
 // The main stream waits for the last rest-update before returning.
 // A non-positive pivot sets status bit 1 (singular saddle system).
 // Right-hand sides: nx blocks of 64 rows stored below the factor (rows Vp ..,
 // B^T) are carried along as extra matrix rows: the panel TRSM of every panel
-// (the last one included) and the column updates of k_col_rows cover them,
+// (the last one included) and the trailing updates of k_syrk128 cover them,
 // which leaves Y^T = (L^-1 B)^T in their place, the forward substitution of
 // L L^T X = B done inside the factorisation.
 rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
@@ -1300,7 +1347,7 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     hipEvent_t* T = ctx->dep_ev.data();   // panel kb final
     hipEvent_t start = ctx->dep_ev[np];
     hipEvent_t fin = ctx->dep_ev[np + 1];
-    int32_t* colcnt = ctx->d_fcnt;        // [kb]: k_col_rows(kb) workgroups done
+    int32_t* colcnt = ctx->d_fcnt;        // [kb]: k_syrk128(kb) column-kb+1 blocks done
     int32_t* diagcnt = ctx->d_fcnt + np;  // [kb]: k_syrk128(kb) blocks of tile kb+2 done
     const double* mu = ctx->d_scal + 1;
     RC_HIP(hipMemsetAsync(ctx->d_fcnt, 0, sizeof(int32_t) * 2 * (size_t)np, ctx->stream));
@@ -1332,10 +1379,9 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
         const int nt = np - kb - 1;
         if (nt == 0 && nx == 0) break;
         // panel kb below its diagonal block (and in the right-hand-side
-        // rows): needs column kb below the diagonal tile (k_col_rows of
-        // panel kb-1, second stream: 2 (nt - 1) + nx nt workgroups of it
-        // with nt = np - kb)
-        const int prev_cols = (kb > 0) ? 2 * (np - kb - 1) + nx * (np - kb) : 0;
+        // rows): needs column kb below the diagonal tile, updated by the
+        // first workgroups of k_syrk128(kb-1) on the second stream
+        const int prev_cols = (kb > 0) ? TrailShape(np - kb, nx).na : 0;
         hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nt + nx), dim3(256), 0, ctx->stream, ctx->d_F, Vp,
                            kb, (const double*)ctx->d_Linv128,
                            (const int32_t*)(prev_cols > 0 ? colcnt + kb - 1 : nullptr), prev_cols,
@@ -1348,21 +1394,17 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
                            (const int32_t*)(kb > 0 ? diagcnt + kb - 1 : nullptr), mu, ctx->V,
                            ctx->d_status);
         RC_CHECK_LAUNCH();
-        // second stream: the rest of column kb+1, then the rest of the
-        // trailing matrix, both with panel kb
+        // second stream: the trailing update with panel kb (column kb+1
+        // and the next diagonal tile first)
         RC_HIP(hipStreamWaitEvent(ctx->stream2, T[kb], 0));
-        if (2 * (nt - 1) + nx * nt > 0) {
-            hipLaunchKernelGGL(k_col_rows, dim3(2 * (nt - 1) + nx * nt), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nt, mu, ctx->V, colcnt + kb);
-            RC_CHECK_LAUNCH();
-        }
-        // 128 x 64 blocks of the lower triangle right of column kb+1
-        const int nblocks = (nt - 1) * nt;
-        if (nblocks > 0) {
+        const TrailShape ts(nt, nx);
+        if (ts.grid > 0) {
             const double mr = (double)(nt - 1) * PB;
-            hipEvent_t* ev = prof_begin(ctx->stream2, (double)PB * mr * (mr + 1.0));
-            hipLaunchKernelGGL(k_syrk128, dim3(8 * ((nblocks + 7) / 8)), dim3(256), 0, ctx->stream2,
-                               ctx->d_F, Vp, kb, nblocks, mu, ctx->V, diagcnt + kb);
+            const double flop = (double)PB * mr * (mr + 1.0) +
+                                2.0 * PB * PB * 64.0 * (2.0 * (nt - 1) + (double)nx * nt);
+            hipEvent_t* ev = prof_begin(ctx->stream2, flop);
+            hipLaunchKernelGGL(k_syrk128, dim3(ts.grid), dim3(256), 0, ctx->stream2, ctx->d_F, Vp,
+                               kb, nt, nx, mu, ctx->V, colcnt + kb, diagcnt + kb);
             RC_CHECK_LAUNCH();
             if (ev) RC_HIP(hipEventRecord(ev[1], ctx->stream2));
         }
