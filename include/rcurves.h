@@ -27,6 +27,7 @@ typedef enum {
     RC_BAD_ARG = 3,
     RC_HIP_ERROR = 4,
     RC_UNSUPPORTED = 5,
+    RC_DIVERGED = 6,     /* multigrid: MGDivergenceError (tpe_stage2_oracle.py:1061-1081) */
 } rc_status;
 
 typedef struct rc_ctx rc_ctx;
@@ -257,6 +258,51 @@ rc_status rc_get_positions(rc_ctx* ctx, double* xyz);
 rc_status rc_sobolev_step_resident(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
                                    const rc_penalties* pen, const rc_step_opts* o,
                                    rc_step_stats* stats, rc_timings* timings);
+
+/* Multigrid projected solver of the saddle system: the reference's stage-2
+ * oracle mg_projected_solve / mg_saddles (oracle/tpe_stage2_oracle.py:796-1198).
+ * The caller builds the level hierarchy on the host (curve coarsening,
+ * per-level constraint Jacobians and (C C^T)^+, as the reference does in
+ * Python); the solve runs on the device. Level 0 is the context's curve at
+ * xyz (its Gram matrix is assembled); level l >= 1 gives, for every vertex of
+ * level l-1, its (up to two) parents in level l and their weights (-1 = none),
+ * and its operator is the Galerkin product P^T A_{l-1} P unless A is given
+ * (the reference's 'rediscretize' option). Vectors (rhs, x) are 3n doubles in
+ * coordinate blocks [x.., y.., z..]. blocks: the level-0 set, used by the
+ * dense cleanup (the library's saddle solve). RC_DIVERGED when the residual
+ * grows past 10x the best one or a cleanup fails its certificate. */
+#define RC_MG_MAXLEVELS 16
+#define RC_MG_MAXHIST 64
+typedef struct {
+    int32_t n;             /* vertices of this level */
+    int32_t k;             /* constraint rows (<= 256) */
+    const double* C;       /* k x 3n, row-major */
+    const double* Gp;      /* k x k: pinv(C C^T) */
+    const int32_t* parent; /* levels >= 1: 2 per vertex of level l-1 */
+    const double* weight;  /* levels >= 1: 2 per vertex of level l-1 */
+    const double* A;       /* optional n x n operator of this level (NULL: Galerkin) */
+} rc_mg_level;
+typedef struct {
+    double tol;            /* projected residual target (1e-10) */
+    int32_t max_cycles;    /* V-cycles (50) */
+    int32_t nu_pre, nu_post;  /* projected-CG smoothing steps (2, 2) */
+    int32_t init_nested;   /* 1: nested-iteration initial guess, 0: zero */
+} rc_mg_opts;
+typedef struct {
+    int32_t iterations;    /* residual evaluations - 1, as the reference counts */
+    int32_t used_direct_cleanup;
+    int32_t nhist;
+    int32_t _pad;
+    double residual;       /* final projected residual */
+    double history[RC_MG_MAXHIST];
+} rc_mg_stats;
+rc_status rc_mg_solve(rc_ctx* ctx, const double* xyz, const rc_params* p, const rc_block* blocks,
+                      int32_t nblocks, const rc_mg_level* levels, int32_t nlevels,
+                      const double* rhs, const rc_mg_opts* o, double* x, rc_mg_stats* stats);
+/* y = A u per coordinate block with the Gram matrix of the curve at xyz
+ * (u, y: 3V doubles in coordinate blocks). */
+rc_status rc_gram_apply(rc_ctx* ctx, const double* xyz, const rc_params* p, const double* u,
+                        double* y);
 
 /* Kernel-level profile of the dominant kernel of the FP64 step, the
  * trailing update of the Cholesky factorisation (k_syrk128, all of the

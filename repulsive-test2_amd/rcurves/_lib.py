@@ -13,8 +13,10 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("RCURVES_LIB", os.path.join(PKG_DIR, "librcurves.so"))
 
-RC_OK, RC_SINGULAR, RC_NONFINITE, RC_BAD_ARG, RC_HIP_ERROR, RC_UNSUPPORTED = range(6)
-STATUS_NAMES = {0: "ok", 1: "singular", 2: "nonfinite", 3: "bad_arg", 4: "hip_error", 5: "unsupported"}
+RC_OK, RC_SINGULAR, RC_NONFINITE, RC_BAD_ARG, RC_HIP_ERROR, RC_UNSUPPORTED, RC_DIVERGED = range(7)
+STATUS_NAMES = {0: "ok", 1: "singular", 2: "nonfinite", 3: "bad_arg", 4: "hip_error", 5: "unsupported",
+                6: "diverged"}
+MG_MAXLEVELS, MG_MAXHIST = 16, 64
 BLOCK_BARYCENTER, BLOCK_TOTAL_LENGTH, BLOCK_EDGE_LENGTHS, BLOCK_POINT = range(4)
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 MODE_ANALYTICAL, MODE_FINITE_DIFF = 0, 1
@@ -126,6 +128,23 @@ class KernelProfile(C.Structure):
                 ("syrk_launches", C.c_int32), ("_pad", C.c_int32)]
 
 
+class MgLevel(C.Structure):
+    _fields_ = [("n", C.c_int32), ("k", C.c_int32), ("C", C.POINTER(C.c_double)),
+                ("Gp", C.POINTER(C.c_double)), ("parent", C.POINTER(C.c_int32)),
+                ("weight", C.POINTER(C.c_double)), ("A", C.POINTER(C.c_double))]
+
+
+class MgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int32), ("nu_pre", C.c_int32),
+                ("nu_post", C.c_int32), ("init_nested", C.c_int32)]
+
+
+class MgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("used_direct_cleanup", C.c_int32),
+                ("nhist", C.c_int32), ("_pad", C.c_int32), ("residual", C.c_double),
+                ("history", C.c_double * MG_MAXHIST)]
+
+
 EXPORTS = (
     "rc_default_step_opts", "rc_ctx_create", "rc_ctx_destroy", "rc_last_error", "rc_energy",
     "rc_gradient", "rc_gradient_fd", "rc_step", "rc_energy_bh", "rc_gradient_bh",
@@ -133,7 +152,7 @@ EXPORTS = (
     "rc_bct_apply_b", "rc_assemble_a", "rc_penalties_eval", "rc_solve_constrained_gradient",
     "rc_constraint_rows", "rc_sobolev_step", "rc_set_positions", "rc_get_positions",
     "rc_sobolev_step_resident", "rc_kernel_profile_enable", "rc_kernel_profile_read",
-    "rc_energy_noalloc",
+    "rc_energy_noalloc", "rc_mg_solve", "rc_gram_apply",
 )
 
 _lib = None
@@ -185,6 +204,10 @@ def lib():
     L.rc_sobolev_step_resident.argtypes = [vp, C.POINTER(Block), C.c_int32, C.POINTER(Penalties),
                                            C.POINTER(StepOpts), C.POINTER(StepStats),
                                            C.POINTER(Timings)]
+    L.rc_mg_solve.argtypes = [vp, _dp, C.POINTER(Params), C.POINTER(Block), C.c_int32,
+                              C.POINTER(MgLevel), C.c_int32, _dp, C.POINTER(MgOpts), _dp,
+                              C.POINTER(MgStats)]
+    L.rc_gram_apply.argtypes = [vp, _dp, C.POINTER(Params), _dp, _dp]
     L.rc_kernel_profile_enable.argtypes = [vp, C.c_int32]
     L.rc_kernel_profile_read.argtypes = [vp, C.POINTER(KernelProfile)]
     for name in ("rc_ctx_create", "rc_energy", "rc_gradient", "rc_gradient_fd", "rc_step",
@@ -192,7 +215,7 @@ def lib():
                  "rc_assemble_a", "rc_penalties_eval",
                  "rc_solve_constrained_gradient", "rc_sobolev_step", "rc_set_positions",
                  "rc_get_positions", "rc_sobolev_step_resident", "rc_kernel_profile_enable",
-                 "rc_kernel_profile_read"):
+                 "rc_kernel_profile_read", "rc_mg_solve", "rc_gram_apply"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

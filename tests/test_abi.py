@@ -66,6 +66,7 @@ def _c_layout(struct, fields):
     ("rc_block", "Block"), ("rc_penalties", "Penalties"), ("rc_step_opts", "StepOpts"),
     ("rc_step_stats", "StepStats"), ("rc_phase", "Phase"), ("rc_timings", "Timings"),
     ("rc_kernel_profile", "KernelProfile"), ("rc_params", "Params"),
+    ("rc_mg_level", "MgLevel"), ("rc_mg_opts", "MgOpts"), ("rc_mg_stats", "MgStats"),
 ])
 def test_struct_layouts_match_header(struct, cls):
     """Every ctypes mirror has the header struct's size and field offsets."""
