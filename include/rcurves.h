@@ -129,7 +129,9 @@ typedef struct {
                                   * solution (0 when saddle_refine = 0) */
     double saddle_ir_correction; /* relative size of the last refinement correction */
     int32_t saddle_ir_steps;     /* refinement steps applied (columns + saddle) */
-    int32_t _pad2;
+    int32_t fp64_fallback;       /* mixed form: 1 when the PCG missed refine_tol within
+                                  * refine_max_iter and the solve was redone with the FP64
+                                  * factorisation (refine_residual / refine_iters keep the PCG's) */
 } rc_step_stats;
 
 /* PhaseSample (phaseTimings.ts:22-31): accumulated device time of one phase

@@ -110,7 +110,7 @@ static void free_all(rc_ctx* ctx) {
                     ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_Adiag,
                     ctx->d_symv,
                     ctx->d_F, ctx->d_F32, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
-                    ctx->d_flags, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
+                    ctx->d_flags, ctx->d_ticket, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
                     ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
                     ctx->d_dE, ctx->d_gt, ctx->d_Y};
@@ -331,6 +331,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_symv, (size_t)ctx->nblk * ctx->nblk * 3 * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
     ALLOC(ctx->d_flags, (size_t)ctx->nblk * RC_BWD_MAXCH);
+    ALLOC(ctx->d_ticket, 1);
     ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
     ALLOC(ctx->d_status, 4);
     ALLOC(ctx->d_tmp, 3 * (size_t)Vp);
@@ -354,6 +355,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ctx->host_stage_bytes = 4096;
     he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk * RC_BWD_MAXCH);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
+    if (he == hipSuccess) he = hipMemset(ctx->d_ticket, 0, sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
     // the inverse's zero blocks above the diagonal are never written
     if (he == hipSuccess)
@@ -373,11 +375,6 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         return fail(RC_HIP_ERROR);
     }
     const int cores = rc_trsm_max_coresident();
-    if (cores < ctx->nblk) {
-        ctx->err = "triangular solve needs " + std::to_string(ctx->nblk) +
-                   " co-resident workgroups, device holds " + std::to_string(cores);
-        return fail(RC_UNSUPPORTED);
-    }
     ctx->bwd_chunks = std::max(1, std::min(RC_BWD_MAXCH, cores / ctx->nblk));
     *out = ctx;
     return RC_OK;
@@ -717,6 +714,7 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
                              const rc_step_opts* o, Ledger* led, hipEvent_t gram_done = nullptr,
                              hipEvent_t resid_go = nullptr, hipEvent_t resid_done = nullptr) {
     const int k = ds.k;
+    ctx->fp64_fallback = 0;
     RC_HIP(hipMemcpyAsync(ctx->d_set, &ds, sizeof(DevSet), hipMemcpyHostToDevice, ctx->stream));
     rc_status s = RC_OK;
     if (gram_done) {
@@ -757,10 +755,36 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     if (precision == RC_PRECISION_MIXED) {
         s = rc_launch_build_rhs(ctx, ctx->d_dE, k, ctx->d_rhs, nrhs);
         if (s != RC_OK) return s;
-        s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, o ? o->refine_tol : 1e-12,
+        const double ptol = o ? o->refine_tol : 1e-12;
+        s = rc_solve_mixed(ctx, ctx->d_rhs, ctx->d_sol, nrhs, ptol,
                            o ? o->refine_max_iter : 200, ctx->d_refine);
         if (s != RC_OK) return s;
-    } else {
+        // a PCG that misses its tolerance within its iteration budget (the
+        // FP32 preconditioner of a badly conditioned A_hat, kappa u_32 >> 1
+        // on large curves) leaves an inaccurate solve: redo it with the
+        // FP64 factorisation, on the device, and report it in the stats
+        RC_HIP(hipMemcpyAsync(ctx->d_host_stage + 40, ctx->d_refine, sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        RC_HIP(hipStreamSynchronize(ctx->stream));
+        if (!(ctx->d_host_stage[40] <= ptol)) {
+            ctx->fp64_fallback = 1;
+            precision = RC_PRECISION_FP64;
+        }
+    }
+    if (precision != RC_PRECISION_MIXED && ctx->fp64_fallback) {
+        s = rc_launch_build_rhs(ctx, ctx->d_dE, k, Frhs, nx * RC_NB);
+        if (s != RC_OK) return s;
+        int id_f64 = -1;
+        LED_OPEN(led, LK_FACTOR, ctx->stream, id_f64);
+        s = rc_factor_fp64(ctx, nx);
+        if (s != RC_OK) return s;
+        LED_CLOSE(led, id_f64, ctx->stream);
+        if (led) led->calls[LK_FACTOR] += 1;
+        RC_HIP(hipMemcpyAsync(ctx->d_sol, Frhs, sizeof(double) * (size_t)nrhs * Vp,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        s = rc_solve_fp64_backward(ctx, ctx->d_sol, nrhs);
+        if (s != RC_OK) return s;
+    } else if (precision != RC_PRECISION_MIXED) {
         RC_HIP(hipMemcpyAsync(ctx->d_sol, Frhs, sizeof(double) * (size_t)nrhs * Vp,
                               hipMemcpyDeviceToDevice, ctx->stream));
         s = rc_solve_fp64_backward(ctx, ctx->d_sol, nrhs);
@@ -1166,7 +1190,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     stats->saddle_ir_residual = d[24];
     stats->saddle_ir_steps = (int32_t)d[25];
     stats->saddle_ir_correction = d[26];
-    stats->_pad2 = 0;
+    stats->fp64_fallback = ctx->fp64_fallback;
     if (led) {
         // device-counted calls (trial energies, correction solves); the
         // phases that did not fire report calls = 0 and no time

@@ -92,9 +92,12 @@ struct rc_ctx {
     float* d_Linv32 = nullptr;
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
     double mixed_shift = 0.0;     // diagonal shift of the last FP32 factorisation
+    int32_t fp64_fallback = 0;    // the last mixed solve was redone with the FP64 factor
     int32_t* d_flags = nullptr;   // triangular-solve block flags
     int32_t* d_fcnt = nullptr;    // factorisation completion counters [2][nblk]
     int32_t epoch = 0;
+    uint32_t* d_ticket = nullptr; // triangular-solve ticket counter (only grows)
+    uint32_t tickets = 0;         // tickets issued so far (mod 2^32)
     int32_t* d_status = nullptr;  // device status words (factor breakdown, spin timeouts)
 
     // solve workspace

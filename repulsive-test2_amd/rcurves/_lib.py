@@ -110,7 +110,7 @@ class StepStats(C.Structure):
         ("saddle_ir_residual", C.c_double),
         ("saddle_ir_correction", C.c_double),
         ("saddle_ir_steps", C.c_int32),
-        ("_pad2", C.c_int32),
+        ("fp64_fallback", C.c_int32),
     ]
 
 

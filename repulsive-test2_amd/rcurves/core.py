@@ -502,7 +502,8 @@ def sobolev_step_set(vertices, edges, disjoint_pairs, cs, opts: SobolevStepOptio
                         "refineResidual": st.refine_residual,
                         "saddleRefineSteps": st.saddle_ir_steps,
                         "saddleResidualDD": st.saddle_ir_residual,
-                        "saddleCorrection": st.saddle_ir_correction},
+                        "saddleCorrection": st.saddle_ir_correction,
+                        "fp64Fallback": bool(st.fp64_fallback)},
     }
     if tm is not None:
         # only the phases that fired, like the reference's Partial ledger
