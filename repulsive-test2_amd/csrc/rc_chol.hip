@@ -630,12 +630,6 @@ __device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t epoch, in
     __syncthreads();
 }
 
-__device__ __forceinline__ void publish(int32_t* flag, int32_t epoch) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Write-through hand-off between the triangular sweeps' blocks. A block's
 // result tile is stored with agent-scope atomic stores (coherent at the
 // device level: never held dirty in one XCD's L2), every wave waits for its
@@ -688,13 +682,18 @@ __device__ __forceinline__ int take_ticket(uint32_t* ctr, uint32_t base) {
     return tk;
 }
 
-// Completion counter of a producer kernel: every workgroup releases its
-// stores device-wide, then one thread adds 1 (consumers poll with wait_flag
-// for the number of producer workgroups).
-__device__ __forceinline__ void signal_done(int32_t* cnt) {
-    __threadfence();
+// Completion counter of a producer kernel whose workgroups wrote their
+// results with store_wt: every wave waits for its stores to complete, then
+// one thread adds 1 (consumers poll with wait_flag for the number of
+// producer workgroups; its acquire drops their stale L2 lines). A release
+// here would write back the XCD's L2, which the rest of the trailing update
+// keeps full of dirty lines.
+__device__ __forceinline__ void signal_done_wt(int32_t* cnt) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
@@ -778,6 +777,8 @@ __device__ __forceinline__ void decode_rest(int q, int& ti, int& hj) {
 }
 
 // C_rows(ri, rj) (64 x 128) -= P_i P_j^T with the panel columns k0 ..
+// (WT: write-through stores, for blocks another stream's kernel waits for)
+template <bool WT>
 __device__ __forceinline__ void update_64x128(double* __restrict__ F, int Vp, size_t k0,
                                               size_t ri, size_t rj, double mu0, int vi, int vj,
                                               double (*As)[KC_TRSM][LDT],
@@ -810,12 +811,16 @@ __device__ __forceinline__ void update_64x128(double* __restrict__ F, int Vp, si
             for (int r = 0; r < 4; ++r) {
                 const int i = wr + 16 * a + 4 * r + (lane >> 4), j = wc + 16 * b + (lane & 15);
                 const double c = (i < vi && j < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
-                C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
+                if (WT)
+                    store_wt(C + (size_t)(16 * a + 4 * r) * Vp + 16 * b, c - acc[a][b][r]);
+                else
+                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
             }
 }
 
 // C(ri, rj) (128 x 64) -= P_i P_j^T; diag: the block lies on a diagonal
 // tile (only j <= i is stored), jo its column offset in that tile
+template <bool WT>
 __device__ __forceinline__ void update_128x64(double* __restrict__ F, int Vp, size_t k0,
                                               size_t ri, size_t rj, bool diag, int jo, double mu0,
                                               int vi, int vj, double (*As)[KC_SYRK][LDT],
@@ -850,8 +855,12 @@ __device__ __forceinline__ void update_128x64(double* __restrict__ F, int Vp, si
                 const int i = wr + 16 * a + 4 * r + (lane >> 4);
                 const int jl = wc + 16 * b + (lane & 15), j = jo + jl;
                 const double c = (i < vi && jl < vj) ? cv[a][b][r] + mu0 : cv[a][b][r];
-                if (!diag || j <= i)
-                    C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
+                if (!diag || j <= i) {
+                    if (WT)
+                        store_wt(C + (size_t)(16 * a + 4 * r) * Vp + 16 * b, c - acc[a][b][r]);
+                    else
+                        C[(size_t)(16 * a + 4 * r) * Vp + 16 * b] = c - acc[a][b][r];
+                }
             }
 }
 
@@ -887,9 +896,9 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
         const size_t ri = wg < nreg ? c1 + PB + (size_t)(wg >> 1) * PB + 64 * (wg & 1)
                                     : (size_t)Vp + 64 * (size_t)(wg - nreg);
         const bool vv = kb == 0 && wg < nreg;
-        update_64x128(F, Vp, k0, ri, c1, vv ? mu0 : 0.0, vv ? V - (int)ri : 0,
-                      vv ? V - (int)c1 : 0, As, Bs);
-        signal_done(col_done);
+        update_64x128<true>(F, Vp, k0, ri, c1, vv ? mu0 : 0.0, vv ? V - (int)ri : 0,
+                            vv ? V - (int)c1 : 0, As, Bs);
+        signal_done_wt(col_done);
         return;
     }
     if (wg < sh.na + sh.nb) {
@@ -897,13 +906,13 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
         if (q < 2) {
             // B: the diagonal tile (kb+2, kb+2), column half q
             const size_t rj = c2 + 64 * (size_t)q;
-            update_128x64(F, Vp, k0, c2, rj, true, 64 * q, mu0, kb == 0 ? V - (int)c2 : 0,
-                          kb == 0 ? V - (int)rj : 0, As, Bs);
-            signal_done(diag_done);
+            update_128x64<true>(F, Vp, k0, c2, rj, true, 64 * q, mu0, kb == 0 ? V - (int)c2 : 0,
+                                kb == 0 ? V - (int)rj : 0, As, Bs);
+            signal_done_wt(diag_done);
         } else {
             // B: right-hand-side rows of columns kb+2 ..
             const int r = q - 2, nc = nt - 1;
-            update_64x128(F, Vp, k0, (size_t)Vp + 64 * (size_t)(r / nc),
+            update_64x128<false>(F, Vp, k0, (size_t)Vp + 64 * (size_t)(r / nc),
                           c2 + (size_t)(r % nc) * PB, 0.0, 0, 0, As, Bs);
         }
         return;
@@ -915,7 +924,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
     int ti, hj;
     decode_rest(q, ti, hj);
     const size_t ri = c2 + (size_t)ti * PB, rj = c2 + (size_t)hj * 64;
-    update_128x64(F, Vp, k0, ri, rj, (hj >> 1) == ti, (hj & 1) * 64, mu0,
+    update_128x64<false>(F, Vp, k0, ri, rj, (hj >> 1) == ti, (hj & 1) * 64, mu0,
                   kb == 0 ? V - (int)ri : 0, kb == 0 ? V - (int)rj : 0, As, Bs);
 }
 
