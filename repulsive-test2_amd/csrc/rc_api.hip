@@ -46,6 +46,7 @@ rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double
 rc_status rc_refine_columns(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
                             int pcg_max_iter, double* d_out);
 rc_status rc_refine_zero(rc_ctx* ctx, double* d_out);
+rc_status rc_saddle_final_residual(rc_ctx* ctx, int k, hipStream_t st, double* d_out);
 
 namespace {
 
@@ -826,6 +827,8 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     }
     s = rc_launch_residual(ctx, rst, ctx->d_gt, ctx->d_dE, k, ctx->d_ssol, ctx->d_Ax, ctx->d_Cg,
                            ctx->d_scal + RC_SCAL_RESID);
+    // the refined solution's compensated residual: a diagnostic as well
+    if (s == RC_OK && nref > 0) s = rc_saddle_final_residual(ctx, k, rst, ctx->d_refine + 2);
     if (s == RC_OK && rst == ctx->stream2) RC_HIP(hipEventRecord(resid_done, ctx->stream2));
     return s;
 }

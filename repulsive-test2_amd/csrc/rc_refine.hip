@@ -560,8 +560,8 @@ static rc_status ir_bufs(rc_ctx* ctx) {
 // [lambda; t] (ctx->d_ssol), with Y = X columns 3.. (ctx->d_sol + 3 Vp) and
 // M^-1 (ctx->d_Minv) of the Schur reduction. A in the strict upper triangle
 // of the factor storage and ctx->d_Adiag (both forms keep it there). nsteps
-// corrections; out[0] = relative compensated residual of the returned
-// iterate, out[1] = corrections applied, out[2] = ||dx|| / ||g~|| of the
+// corrections; out[0] = relative compensated residual (of the returned
+// iterate once rc_saddle_final_residual has run), out[1] = corrections applied, out[2] = ||dx|| / ||g~|| of the
 // last one, out[3] = PCG iterations of the corrections (mixed form),
 // out[4..5] the mixed solve's own (residual, iterations) slots.
 rc_status rc_refine_zero(rc_ctx* ctx, double* d_out) {
@@ -570,69 +570,93 @@ rc_status rc_refine_zero(rc_ctx* ctx, double* d_out) {
     return RC_OK;
 }
 
+namespace {
+
+struct IrBufs {
+    double *part_s, *part_e, *rho, *Z, *rhob, *wv, *delta, *rn, *dn;
+};
+
+IrBufs ir_layout(rc_ctx* ctx) {
+    const int Vp = ctx->nblk * RC_NB;
+    const int nb = ctx->nblk;
+    const size_t np = (size_t)nb * nb * 3 * RC_NB;
+    IrBufs b;
+    b.part_s = ctx->d_ir;
+    b.part_e = b.part_s + np;
+    b.rho = b.part_e + np;
+    b.Z = b.rho + 3 * (size_t)Vp;
+    b.rhob = b.Z + 3 * (size_t)Vp;
+    b.wv = b.rhob + (ctx->kcap + 3);
+    b.delta = b.wv + (ctx->kcap + 3);
+    b.rn = b.delta + (ctx->kcap + 3);
+    b.dn = b.rn + 2 * 3 * (size_t)nb;
+    return b;
+}
+
+// Compensated saddle residual of the current [g~; lambda] on stream st:
+// rho (top, coordinate blocks), rhob (constraint rows), d_out[0] = relative
+// norm.
+rc_status ir_residual(rc_ctx* ctx, int k, hipStream_t st, const IrBufs& b, double* d_out) {
+    const int Vp = ctx->nblk * RC_NB;
+    const int nb = ctx->nblk;
+    hipLaunchKernelGGL(k_ir_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, st,
+                       (const double*)ctx->d_F, ctx->V, Vp, nb, (const double*)ctx->d_gt,
+                       b.part_s, b.part_e, (const int32_t*)nullptr);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ir_rho, dim3(nb, 3), dim3(64), 0, st, (const double*)b.part_s,
+                       (const double*)b.part_e, (const double*)ctx->d_Adiag,
+                       (const double*)ctx->d_gt, (const double*)ctx->d_C,
+                       (const double*)ctx->d_ssol, k, (const double*)ctx->d_dE, ctx->V, Vp, nb,
+                       b.rho, b.Z, b.rn, (const int32_t*)nullptr);
+    RC_CHECK_LAUNCH();
+    if (k > 0) {
+        hipLaunchKernelGGL(k_ir_rhob, dim3(k), dim3(256), 0, st, (const double*)ctx->d_C,
+                           (const double*)ctx->d_gt, ctx->V, Vp, b.rhob,
+                           (const int32_t*)nullptr);
+        RC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ir_resnorm, dim3(1), dim3(256), 0, st, (const double*)b.rn, 3 * nb,
+                       (const double*)b.rhob, k, d_out, (const int32_t*)nullptr);
+    RC_CHECK_LAUNCH();
+    return RC_OK;
+}
+
+}  // namespace
+
 rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
                            int pcg_max_iter, double* d_out) {
     rc_status s = ir_bufs(ctx);
     if (s != RC_OK) return s;
     const int Vp = ctx->nblk * RC_NB;
-    const int nb = ctx->nblk;
-    const size_t np = (size_t)nb * nb * 3 * RC_NB;
-    double* part_s = ctx->d_ir;
-    double* part_e = part_s + np;
-    double* rho = part_e + np;
-    double* Z = rho + 3 * (size_t)Vp;
-    double* rhob = Z + 3 * (size_t)Vp;
-    double* wv = rhob + (ctx->kcap + 3);
-    double* delta = wv + (ctx->kcap + 3);
-    double* rn = delta + (ctx->kcap + 3);
-    double* dn = rn + 2 * 3 * (size_t)nb;
+    const IrBufs b = ir_layout(ctx);
     const int ngx = (Vp + 255) / 256;
     const double* Y = ctx->d_sol + 3 * (size_t)Vp;
-    for (int it = 0; it <= nsteps; ++it) {
-        hipLaunchKernelGGL(k_ir_tiles, dim3(nb * (nb + 1) / 2), dim3(256), 0, ctx->stream,
-                           (const double*)ctx->d_F, ctx->V, Vp, nb, (const double*)ctx->d_gt,
-                           part_s, part_e, (const int32_t*)nullptr);
-        RC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_ir_rho, dim3(nb, 3), dim3(64), 0, ctx->stream,
-                           (const double*)part_s, (const double*)part_e,
-                           (const double*)ctx->d_Adiag, (const double*)ctx->d_gt,
-                           (const double*)ctx->d_C, (const double*)ctx->d_ssol, k,
-                           (const double*)ctx->d_dE, ctx->V, Vp, nb, rho, Z, rn,
-                           (const int32_t*)nullptr);
-        RC_CHECK_LAUNCH();
-        if (k > 0) {
-            hipLaunchKernelGGL(k_ir_rhob, dim3(k), dim3(256), 0, ctx->stream,
-                               (const double*)ctx->d_C, (const double*)ctx->d_gt, ctx->V, Vp, rhob,
-                               (const int32_t*)nullptr);
-            RC_CHECK_LAUNCH();
-        }
-        hipLaunchKernelGGL(k_ir_resnorm, dim3(1), dim3(256), 0, ctx->stream, (const double*)rn,
-                           3 * nb, (const double*)rhob, k, d_out, (const int32_t*)nullptr);
-        RC_CHECK_LAUNCH();
-        if (it == nsteps) break;
+    for (int it = 0; it < nsteps; ++it) {
+        s = ir_residual(ctx, k, ctx->stream, b, d_out);
+        if (s != RC_OK) return s;
         // Z = A_hat^-1 rho (three columns)
         if (precision == RC_PRECISION_MIXED) {
             // PCG writes its (max residual, iterations) pair into d_out[4..5]
-            s = rc_solve_mixed(ctx, rho, Z, 3, pcg_tol, pcg_max_iter, d_out + 4);
+            s = rc_solve_mixed(ctx, b.rho, b.Z, 3, pcg_tol, pcg_max_iter, d_out + 4);
             if (s != RC_OK) return s;
         } else {
-            s = rc_solve_fp64_forward(ctx, Z, 3);
-            if (s == RC_OK) s = rc_solve_fp64_backward(ctx, Z, 3);
+            s = rc_solve_fp64_forward(ctx, b.Z, 3);
+            if (s == RC_OK) s = rc_solve_fp64_backward(ctx, b.Z, 3);
             if (s != RC_OK) return s;
         }
         hipLaunchKernelGGL(k_ir_dots, dim3(k + 3), dim3(256), 0, ctx->stream,
-                           (const double*)ctx->d_C, (const double*)Z, (const double*)rho, k,
-                           ctx->V, Vp, wv, (const int32_t*)nullptr);
+                           (const double*)ctx->d_C, (const double*)b.Z, (const double*)b.rho, k,
+                           ctx->V, Vp, b.wv, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_ir_small, dim3(k + 3), dim3(64), 0, ctx->stream,
-                           (const double*)ctx->d_Minv, (const double*)wv, (const double*)rhob, k,
-                           delta, ctx->d_ssol, (const int32_t*)nullptr);
+                           (const double*)ctx->d_Minv, (const double*)b.wv, (const double*)b.rhob,
+                           k, b.delta, ctx->d_ssol, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_ir_update, dim3(ngx, 3), dim3(256), 0, ctx->stream,
-                           (const double*)Z, Y, (const double*)delta, k, ctx->V, Vp, ctx->d_gt, dn,
-                           (const int32_t*)nullptr);
+                           (const double*)b.Z, Y, (const double*)b.delta, k, ctx->V, Vp, ctx->d_gt,
+                           b.dn, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_ir_corrnorm, dim3(1), dim3(256), 0, ctx->stream, (const double*)dn,
+        hipLaunchKernelGGL(k_ir_corrnorm, dim3(1), dim3(256), 0, ctx->stream, (const double*)b.dn,
                            3 * ngx, d_out + 2, d_out + 1, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
         if (precision == RC_PRECISION_MIXED) {
@@ -641,6 +665,15 @@ rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double
         }
     }
     return RC_OK;
+}
+
+// The compensated residual of the refined solution (d_out[0]), a diagnostic
+// nothing on the step reads: launched on stream st after the refinement
+// (the caller orders st after it), beside the line search.
+rc_status rc_saddle_final_residual(rc_ctx* ctx, int k, hipStream_t st, double* d_out) {
+    rc_status s = ir_bufs(ctx);
+    if (s != RC_OK) return s;
+    return ir_residual(ctx, k, st, ir_layout(ctx), d_out);
 }
 
 // Refinement of the solve's columns X = A_hat^-1 B (ctx->d_sol, the 3 + 3k
