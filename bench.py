@@ -135,7 +135,7 @@ def accuracy(v, e, cs, pen):
     for name, prec, nref in (("fp64", "fp64", 1), ("mixed", "mixed", 1), ("fp64_unrefined", "fp64", 0)):
         r = core.sobolev_step_set(v, e, None, cs, core.SobolevStepOptions(
             penalties=pen, collect_field=True, precision=prec, projection_mode="frozen",
-            saddle_refine=nref))
+            saddle_refine=nref, report_ir_residual=True))
         g = r["descentField"]
         if g is None:
             out[name] = {"error": "singular saddle: " + str(r["stats"])}

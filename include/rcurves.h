@@ -98,6 +98,10 @@ typedef struct {
                                  * targets that grow, many pins) */
     double saddle_refine_tol;   /* mixed form: PCG relative residual of each refinement
                                  * correction, default 1e-6 */
+    int32_t report_ir_residual; /* 1: also form saddle_ir_residual, the compensated residual of
+                                 * the returned (refined) solution: a diagnostic, one more
+                                 * compensated pass over A beside the line search; default 0
+                                 * (saddle_ir_residual reports 0) */
 } rc_step_opts;
 
 /* SobolevStepStats (optimizer.ts:196-202) + line-search diagnostics. */
@@ -126,7 +130,7 @@ typedef struct {
     int32_t trials;
     int32_t refine_iters;
     double saddle_ir_residual;   /* compensated ||rhs - K z|| / max(1, ||rhs||) of the returned
-                                  * solution (0 when saddle_refine = 0) */
+                                  * solution (0 when saddle_refine = 0 or report_ir_residual = 0) */
     double saddle_ir_correction; /* relative size of the last refinement correction */
     int32_t saddle_ir_steps;     /* refinement steps applied (columns + saddle) */
     int32_t fp64_fallback;       /* mixed form: 1 when the PCG missed refine_tol within

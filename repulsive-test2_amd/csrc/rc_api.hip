@@ -97,6 +97,7 @@ extern "C" void rc_default_step_opts(rc_step_opts* o) {
     o->saddle_refine = 1;
     o->refine_columns = 0;
     o->saddle_refine_tol = 1e-6;
+    o->report_ir_residual = 0;
 }
 
 extern "C" const char* rc_last_error(const rc_ctx* ctx) {
@@ -827,8 +828,10 @@ static rc_status solve_phase(rc_ctx* ctx, const rc_params* p, const DevSet& ds, 
     }
     s = rc_launch_residual(ctx, rst, ctx->d_gt, ctx->d_dE, k, ctx->d_ssol, ctx->d_Ax, ctx->d_Cg,
                            ctx->d_scal + RC_SCAL_RESID);
-    // the refined solution's compensated residual: a diagnostic as well
-    if (s == RC_OK && nref > 0) s = rc_saddle_final_residual(ctx, k, rst, ctx->d_refine + 2);
+    // the refined solution's compensated residual: a diagnostic as well,
+    // formed only on request
+    if (s == RC_OK && nref > 0 && o && o->report_ir_residual)
+        s = rc_saddle_final_residual(ctx, k, rst, ctx->d_refine + 2);
     if (s == RC_OK && rst == ctx->stream2) RC_HIP(hipEventRecord(resid_done, ctx->stream2));
     return s;
 }

@@ -54,6 +54,7 @@ __global__ void k_trace_mu(const double* __restrict__ A, int V, int ldA, double*
                            double* __restrict__ diag) {
     __shared__ double sh[1024];
     double s = 0.0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < V; i += 1024) {
         const double a = A[(size_t)i * ldA + i];
         diag[i] = a;

@@ -18,6 +18,7 @@
 #define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
 #define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
 #define RC_NOALLOC_EMAX 16384  // largest E of rc_energy_noalloc (its static scratch)
+#define RED_T 1024          // threads of the one-workgroup-per-sum reductions (few long sums)
 
 // Structure-of-arrays per-edge geometry refreshed by k_edge_prep for the
 // current positions. Index = edge id; padded entries (id >= E) are unused.

@@ -208,6 +208,7 @@ __global__ __launch_bounds__(64) void k_ir_rho(const double* __restrict__ part_s
         cpair s = {0.0, 0.0};
         cp_fma(s, d[v], x[j]);
         const size_t base = ((size_t)I * nblk * 3 + c) * RC_NB + i;
+#pragma unroll 8
         for (int J = 0; J < nblk; ++J) {
             const size_t o = base + (size_t)J * 3 * RC_NB;
             s = cp_add(s, cpair{part_s[o], part_e[o]});
@@ -233,16 +234,17 @@ __global__ __launch_bounds__(64) void k_ir_rho(const double* __restrict__ part_s
 }
 
 // rho_lambda[p] = 0 - C_p . x (compensated), one workgroup per row p.
-__global__ __launch_bounds__(256) void k_ir_rhob(const double* __restrict__ C,
-                                                 const double* __restrict__ x, int V, int Vp,
-                                                 double* __restrict__ rhob,
-                                                 const int32_t* __restrict__ skip) {
+__global__ __launch_bounds__(RED_T) void k_ir_rhob(const double* __restrict__ C,
+                                                   const double* __restrict__ x, int V, int Vp,
+                                                   double* __restrict__ rhob,
+                                                   const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double sh[4][2];
+    __shared__ double sh[RED_T / 64][2];
     const int p = blockIdx.x;
     const double* row = C + (size_t)p * 3 * Vp;
     cpair s = {0.0, 0.0};
-    for (int q = threadIdx.x; q < 3 * V; q += 256) {
+#pragma unroll 8
+    for (int q = threadIdx.x; q < 3 * V; q += RED_T) {
         const int c = q / V, v = q - c * V;
         const size_t j = (size_t)c * Vp + v;
         cp_fma(s, row[j], x[j]);
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(256) void k_ir_rhob(const double* __restrict__ C,
     __syncthreads();
     if (threadIdx.x == 0) {
         cpair a = {sh[0][0], sh[0][1]};
-        for (int ww = 1; ww < 4; ++ww) a = cp_add(a, cpair{sh[ww][0], sh[ww][1]});
+        for (int ww = 1; ww < RED_T / 64; ++ww) a = cp_add(a, cpair{sh[ww][0], sh[ww][1]});
         rhob[p] = -(a.s + a.e);
     }
 }
@@ -335,6 +337,7 @@ __global__ __launch_bounds__(256) void k_ir_colsum(const double* __restrict__ X,
     __shared__ double sh[4];
     const int c = blockIdx.x;
     double s = 0.0;
+#pragma unroll 8
     for (int v = threadIdx.x; v < V; v += 256) s += X[(size_t)c * Vp + v];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -407,31 +410,37 @@ __global__ __launch_bounds__(256) void k_ir_resnorm(const double* __restrict__ r
 }
 
 // w[p] = <C_p, Z> for p < k; w[k + c] = sum_v rho_c[v]. One workgroup each.
-__global__ __launch_bounds__(256) void k_ir_dots(const double* __restrict__ C,
-                                                 const double* __restrict__ Z,
-                                                 const double* __restrict__ rho, int k, int V,
-                                                 int Vp, double* __restrict__ w,
-                                                 const int32_t* __restrict__ skip) {
+__global__ __launch_bounds__(RED_T) void k_ir_dots(const double* __restrict__ C,
+                                                   const double* __restrict__ Z,
+                                                   const double* __restrict__ rho, int k, int V,
+                                                   int Vp, double* __restrict__ w,
+                                                   const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double sh[4];
+    __shared__ double sh[RED_T / 64];
     const int p = blockIdx.x;
     double s = 0.0;
     if (p < k) {
         const double* row = C + (size_t)p * 3 * Vp;
-        for (int q = threadIdx.x; q < 3 * V; q += 256) {
+#pragma unroll 8
+        for (int q = threadIdx.x; q < 3 * V; q += RED_T) {
             const int c = q / V, v = q - c * V;
             const size_t j = (size_t)c * Vp + v;
             s = fma(row[j], Z[j], s);
         }
     } else {
         const double* col = rho + (size_t)(p - k) * Vp;
-        for (int v = threadIdx.x; v < V; v += 256) s += col[v];
+#pragma unroll 8
+        for (int v = threadIdx.x; v < V; v += RED_T) s += col[v];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) w[p] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int ww = 0; ww < RED_T / 64; ++ww) a += sh[ww];
+        w[p] = a;
+    }
 }
 
 // [dlambda; dt] = M^-1 [w_{<k} - rho_lambda; w_{>=k}], one wave per row i;
@@ -610,7 +619,7 @@ rc_status ir_residual(rc_ctx* ctx, int k, hipStream_t st, const IrBufs& b, doubl
                        b.rho, b.Z, b.rn, (const int32_t*)nullptr);
     RC_CHECK_LAUNCH();
     if (k > 0) {
-        hipLaunchKernelGGL(k_ir_rhob, dim3(k), dim3(256), 0, st, (const double*)ctx->d_C,
+        hipLaunchKernelGGL(k_ir_rhob, dim3(k), dim3(RED_T), 0, st, (const double*)ctx->d_C,
                            (const double*)ctx->d_gt, ctx->V, Vp, b.rhob,
                            (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();
@@ -644,7 +653,7 @@ rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double
             if (s == RC_OK) s = rc_solve_fp64_backward(ctx, b.Z, 3);
             if (s != RC_OK) return s;
         }
-        hipLaunchKernelGGL(k_ir_dots, dim3(k + 3), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_ir_dots, dim3(k + 3), dim3(RED_T), 0, ctx->stream,
                            (const double*)ctx->d_C, (const double*)b.Z, (const double*)b.rho, k,
                            ctx->V, Vp, b.wv, (const int32_t*)nullptr);
         RC_CHECK_LAUNCH();

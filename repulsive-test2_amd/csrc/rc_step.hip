@@ -175,6 +175,7 @@ __device__ double penalty_energy_block(const EdgeSoA& ed, int E, const int32_t* 
                                        const int32_t* vEdge, int V, const DevPen& pen,
                                        double* sh) {
     double sL = 0.0, sF = 0.0, sD = 0.0;
+#pragma unroll 4
     for (int K = threadIdx.x; K < E; K += blockDim.x) {
         const double l = ed.rlen[K];
         sL += l;
@@ -187,6 +188,7 @@ __device__ double penalty_energy_block(const EdgeSoA& ed, int E, const int32_t* 
         }
     }
     if (pen.wD != 0.0)
+#pragma unroll 8
         for (int w = threadIdx.x; w < V; w += blockDim.x)
             if (vOff[w + 1] - vOff[w] == 2) {
                 const double d = ed.rlen[vEdge[vOff[w]]] - ed.rlen[vEdge[vOff[w] + 1]];
@@ -229,15 +231,17 @@ __global__ void k_build_rhs(const double* __restrict__ dE, const double* __restr
 }
 
 // out[p * nr + q] = <L_p, R_q> over len entries (L stride ls, R stride rs).
-__global__ __launch_bounds__(256) void k_dots(const double* __restrict__ L, size_t ls,
-                                              const double* __restrict__ R, size_t rs, int nr,
-                                              size_t len, double* __restrict__ out) {
+// One workgroup (RED_T threads) per product: few products, each long.
+__global__ __launch_bounds__(RED_T) void k_dots(const double* __restrict__ L, size_t ls,
+                                                const double* __restrict__ R, size_t rs, int nr,
+                                                size_t len, double* __restrict__ out) {
     __shared__ double sh[32];
     const int p = blockIdx.x / nr, q = blockIdx.x % nr;
     const double* a = L + p * ls;
     const double* b = R + q * rs;
     double s = 0.0;
-    for (size_t j = threadIdx.x; j < len; j += 256) s = fma(a[j], b[j], s);
+#pragma unroll 8
+    for (size_t j = threadIdx.x; j < len; j += RED_T) s = fma(a[j], b[j], s);
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
@@ -296,14 +300,15 @@ __global__ __launch_bounds__(256) void k_dots_tile(const double* __restrict__ L,
 
 // Column sums of the coordinate blocks of C rows: CN[p][c] = sum_v C[p][c*Vp+v];
 // also Nr[c] = sum_v dE[c*Vp + v] at CN[k*3 + c].
-__global__ __launch_bounds__(256) void k_block_sums(const double* __restrict__ C,
-                                                    const double* __restrict__ dE, int k, int V,
-                                                    int Vp, double* __restrict__ out) {
+__global__ __launch_bounds__(RED_T) void k_block_sums(const double* __restrict__ C,
+                                                      const double* __restrict__ dE, int k, int V,
+                                                      int Vp, double* __restrict__ out) {
     __shared__ double sh[32];
     const int p = blockIdx.x / 3, c = blockIdx.x % 3;
     const double* src = (p < k) ? C + (size_t)p * 3 * Vp + (size_t)c * Vp : dE + (size_t)c * Vp;
     double s = 0.0;
-    for (int v = threadIdx.x; v < V; v += 256) s += src[v];
+#pragma unroll 8
+    for (int v = threadIdx.x; v < V; v += RED_T) s += src[v];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
@@ -685,6 +690,7 @@ __global__ __launch_bounds__(CT) void k_residual(const double* __restrict__ Ax,
                                                  int Vp, double* __restrict__ out) {
     __shared__ double sh[32];
     double r2 = 0.0, b2 = 0.0;
+#pragma unroll 4
     for (int q = threadIdx.x; q < 3 * V; q += CT) {
         const int c = q / V, v = q % V;
         const size_t j = (size_t)c * Vp + v;
@@ -718,6 +724,7 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     __shared__ double gn_s;
     const int t = threadIdx.x;
     double s = 0.0;
+#pragma unroll 4
     for (int K = t; K < E; K += CT) {
         const int a = ed.i1[K], b = ed.i2[K];
         const double l = ed.rlen[K];
@@ -733,6 +740,7 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     const double e0 = has_e0 ? e0_given : (e0tpe[0] + e0pen[0]);
     // finite check of g~ (a non-finite solve is the reference's "treating as singular")
     double nf = 0.0;
+#pragma unroll 4
     for (int q = t; q < 3 * V; q += CT) {
         const double x = gt[(size_t)(q / V) * Vp + q % V];
         if (!(x - x == 0.0)) nf = 1.0;
@@ -744,6 +752,7 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     const bool singular = (status[0] & (1 | 4)) != 0 || nf > 0.0;
     double sl = 0.0;
     if (!singular && gn >= 1e-4 && gn > 0.0 && gn - gn == 0.0) {
+#pragma unroll 4
         for (int q = t; q < 3 * V; q += CT) {
             const size_t j = (size_t)(q / V) * Vp + q % V;
             const double d = gt[j] / gn;
@@ -796,6 +805,7 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
 
 __device__ void edge_prep_block(EdgeSoA& ed, const double* pos, int V, int E, double alpha,
                                 double eps) {
+#pragma unroll 4
     for (int K = threadIdx.x; K < E; K += blockDim.x) {
         const int a = ed.i1[K], b = ed.i2[K];
         const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
@@ -832,6 +842,7 @@ __device__ __forceinline__ bool project_check(const LSCtl* ctl, const DevSet* __
                                               double& phin) {
     const int t = threadIdx.x;
     double L = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pe2 = 0.0;
+#pragma unroll 4
     for (int K = t; K < E; K += CT) {
         const int a = ed.i1[K], b = ed.i2[K];
         const double ax = cur[a], ay = cur[V + a], az = cur[2 * V + a];
@@ -857,6 +868,7 @@ __device__ __forceinline__ bool project_check(const LSCtl* ctl, const DevSet* __
     pe2 = block_sum(pe2, sh);
     double Rb = 0.0;
     if (set->has_bary)
+#pragma unroll 4
         for (int v = t; v < V; v += CT) {
             if (vOff[v + 1] == vOff[v]) continue;  // endpoints of edges only
             const double dx = cur[v] - set->x0[0], dy = cur[V + v] - set->x0[1],
@@ -867,6 +879,7 @@ __device__ __forceinline__ bool project_check(const LSCtl* ctl, const DevSet* __
     for (int pn = 0; pn < set->npin; ++pn) {
         double R = 0.0;
         const double* tg = set->pin_target[pn];
+#pragma unroll 4
         for (int v = t; v < V; v += CT) {
             const double dx = cur[v] - tg[0], dy = cur[V + v] - tg[1], dz = cur[2 * V + v] - tg[2];
             R = js_max(R, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
@@ -951,6 +964,7 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
     if (k > RC_KSMALL) {
         // a vertex's three components per thread, four partial sums over
         // the rows each: twelve independent loads in flight per thread
+#pragma unroll 4
         for (int v = t; v < V; v += CT) {
             double s[3][4] = {};
             int p = 0;
@@ -976,6 +990,7 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
         __syncthreads();
         return nf > 0.0;
     }
+#pragma unroll 4
     for (int q = t; q < 3 * V; q += CT) {
         const int c = q / V, v = q % V;
         double x = solv[k + c] / V;
@@ -992,6 +1007,7 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
 __device__ __forceinline__ void trial_candidate(const double* __restrict__ pos0,
                                                 const double* __restrict__ dir, double tau, int V,
                                                 int Vp, double* __restrict__ cur) {
+#pragma unroll 4
     for (int q = threadIdx.x; q < 3 * V; q += CT) {
         const int c = q / V, v = q % V;
         cur[(size_t)c * V + v] = pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v];
@@ -1011,6 +1027,7 @@ __device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, doubl
                                                double eps, double* sh) {
     const int t = threadIdx.x;
     double nfc = 0.0;
+#pragma unroll 4
     for (int q = t; q < 3 * V; q += CT)
         if (!(cur[q] - cur[q] == 0.0)) nfc = 1.0;
     nfc = block_sum(nfc, sh);
@@ -1370,12 +1387,12 @@ rc_status rc_launch_schur(rc_ctx* ctx, const double* X, const double* dE, int k,
             hipLaunchKernelGGL(k_dots_tile, dim3(tiles), dim3(256), 0, ctx->stream,
                                (const double*)ctx->d_C, len, k, X, len, k + 1, len, d_dots);
         } else {
-            hipLaunchKernelGGL(k_dots, dim3(k * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, len,
+            hipLaunchKernelGGL(k_dots, dim3(k * (k + 1)), dim3(RED_T), 0, ctx->stream, ctx->d_C, len,
                                X, len, k + 1, len, d_dots);
         }
         RC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_block_sums, dim3(3 * (k + 1)), dim3(256), 0, ctx->stream, ctx->d_C, dE, k,
+    hipLaunchKernelGGL(k_block_sums, dim3(3 * (k + 1)), dim3(RED_T), 0, ctx->stream, ctx->d_C, dE, k,
                        ctx->V, Vp, d_bsum);
     RC_CHECK_LAUNCH();
     const int n = k + 3;
@@ -1430,7 +1447,7 @@ rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, cons
                        d_Ax);
     RC_CHECK_LAUNCH();
     if (k > 0) {
-        hipLaunchKernelGGL(k_dots, dim3(k), dim3(256), 0, st, ctx->d_C, 3 * (size_t)Vp, gt,
+        hipLaunchKernelGGL(k_dots, dim3(k), dim3(RED_T), 0, st, ctx->d_C, 3 * (size_t)Vp, gt,
                            (size_t)0, 1, 3 * (size_t)Vp, d_Cg);
         RC_CHECK_LAUNCH();
     }

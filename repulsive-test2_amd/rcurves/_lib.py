@@ -88,6 +88,7 @@ class StepOpts(C.Structure):
         ("saddle_refine", C.c_int32),
         ("refine_columns", C.c_int32),
         ("saddle_refine_tol", C.c_double),
+        ("report_ir_residual", C.c_int32),
     ]
 
 

@@ -413,6 +413,7 @@ class SobolevStepOptions:
     saddle_refine: int = 1
     refine_columns: int = 0       # also refine A_hat^-1 C^T (exact frozen projection)
     saddle_refine_tol: float = 1e-6
+    report_ir_residual: bool = False  # also form the refined solution's compensated residual
 
     def to_c(self) -> L.StepOpts:
         o = L.StepOpts()
@@ -433,6 +434,7 @@ class SobolevStepOptions:
         o.saddle_refine = int(self.saddle_refine)
         o.refine_columns = int(self.refine_columns)
         o.saddle_refine_tol = float(self.saddle_refine_tol)
+        o.report_ir_residual = 1 if self.report_ir_residual else 0
         return o
 
 

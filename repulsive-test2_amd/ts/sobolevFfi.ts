@@ -29,7 +29,7 @@ type Edge = [number, number];
 const LIB = process.env.RCURVES_LIB ?? new URL('../librcurves.so', import.meta.url).pathname;
 
 // ---- C struct layouts (include/rcurves.h) ----------------------------------
-const OPTS_BYTES = 144; // rc_step_opts
+const OPTS_BYTES = 152; // rc_step_opts
 const STATS_BYTES = 112; // rc_step_stats
 const TIMINGS_BYTES = 128; // rc_timings: 8 x rc_phase {f64 ms, i32 calls, i32 pad}
 const PHASE_BYTES = 16; // rc_phase
@@ -158,6 +158,35 @@ export function gradientFiniteDiff(
     const st = rc.rc_gradient_fd(c, ptr(flatVerts(vertices)), ptr(params(alpha, beta, epsilon)), h, ptr(g));
     if (st !== 0) throw new Error(`rc_gradient_fd: ${rc.rc_last_error(c)}`);
     return unflat(g);
+}
+
+/**
+ * Function-style forward/backward pair over the device kernels: forward()
+ * evaluates the tangent-point energy (rc_energy), backward() the adjoint
+ * gradient of the same energy at the saved vertices (rc_gradient, which
+ * recomputes the pair terms instead of keeping an E^2 tape), scaled by the
+ * upstream gradient. The curve is kept between the two calls, as an autograd
+ * Function keeps its saved tensors.
+ */
+export class TangentPointEnergy {
+    private saved: Vec3[] | null = null;
+
+    constructor(
+        readonly edges: Edge[], readonly numVertices: number,
+        readonly alpha = 3, readonly beta = 6, readonly epsilon = 1e-10,
+    ) {}
+
+    forward(vertices: Vec3[]): number {
+        if (vertices.length !== this.numVertices) throw new Error('vertex count differs from the topology');
+        this.saved = vertices.map((p) => [p[0], p[1], p[2]] as Vec3);
+        return calculateEnergy(this.saved, this.edges, null, this.alpha, this.beta, this.epsilon);
+    }
+
+    backward(gradOutput = 1): Vec3[] {
+        if (!this.saved) throw new Error('backward() before forward()');
+        const g = gradientAnalytical(this.saved, this.edges, null, this.alpha, this.beta, this.epsilon);
+        return g.map((p) => [gradOutput * p[0], gradOutput * p[1], gradOutput * p[2]] as Vec3);
+    }
 }
 
 export interface StepOptions {

@@ -158,10 +158,10 @@ def run_two_forms(v, e, make_set, pen, steps, grow=1.0, refine_columns=0):
         cs = make_set(L0)
         rf = core.sobolev_step_set(vf, e, None, cs, core.SobolevStepOptions(
             penalties=pen, energy_before=ef, projection_mode="frozen", collect_field=True,
-            refine_columns=refine_columns))
+            refine_columns=refine_columns, report_ir_residual=True))
         rm = core.sobolev_step_set(vf, e, None, cs, core.SobolevStepOptions(
             penalties=pen, energy_before=ef, projection_mode="frozen", collect_field=True,
-            precision="mixed", refine_columns=refine_columns))
+            precision="mixed", refine_columns=refine_columns, report_ir_residual=True))
         de = abs(rm["energy"] - rf["energy"]) / abs(rf["energy"])
         dd = rel(rm["descentField"], rf["descentField"])
         dv = rel(rm["vertices"], rf["vertices"])
