@@ -40,7 +40,8 @@ rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const De
                                   const double* Minv);
 rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv,
                          const double* phi_g, double* solv, double* cur, LSCtl* d_ctl);
-rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
+                              const double* cur, double* d_epart_ls);
 rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
                            int pcg_max_iter, double* d_out);
 rc_status rc_refine_columns(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
@@ -961,7 +962,7 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
         }
         LED_CLOSE(led, id_proj, ctx->stream);
         LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, trial);
-        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
+        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, dp, ctx->d_work, ctx->d_epart_ls);
         if (s != RC_OK) return s;
         LED_CLOSE(led, id_e, ctx->stream);
         RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
@@ -1004,7 +1005,7 @@ static rc_status line_search_frozen_wide(rc_ctx* ctx, const rc_params* p, const 
         }
         LED_CLOSE(led, id_proj, ctx->stream);
         LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, trial);
-        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
+        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, dp, ctx->d_work, ctx->d_epart_ls);
         if (s != RC_OK) return s;
         LED_CLOSE(led, id_e, ctx->stream);
         RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
@@ -1132,7 +1133,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
             if (s != RC_OK) return s;
             LED_CLOSE(led, id_proj, ctx->stream);
             LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, done_trials + i);
-            s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, ctx->d_epart_ls);
+            s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, dp, ctx->d_work, ctx->d_epart_ls);
             if (s != RC_OK) return s;
             LED_CLOSE(led, id_e, ctx->stream);
         }

@@ -15,6 +15,7 @@
 // edge geometry of the projected curve) and the grid-wide energy kernel whose
 // last workgroup applies the Armijo test. Trials are enqueued in batches; a
 // finished line search turns the remaining kernels into immediate returns.
+#include <algorithm>
 #include <cmath>
 
 #include "rc_internal.h"
@@ -740,10 +741,12 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     const double e0 = has_e0 ? e0_given : (e0tpe[0] + e0pen[0]);
     // finite check of g~ (a non-finite solve is the reference's "treating as singular")
     double nf = 0.0;
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll 4
-    for (int q = t; q < 3 * V; q += CT) {
-        const double x = gt[(size_t)(q / V) * Vp + q % V];
-        if (!(x - x == 0.0)) nf = 1.0;
+        for (int v = t; v < V; v += CT) {
+            const double x = gt[(size_t)c * Vp + v];
+            if (!(x - x == 0.0)) nf = 1.0;
+        }
     }
     nf = block_sum(nf, sh);
     if (t == 0) gn_s = sqrt(s > 0.0 ? s : 0.0);
@@ -752,12 +755,14 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     const bool singular = (status[0] & (1 | 4)) != 0 || nf > 0.0;
     double sl = 0.0;
     if (!singular && gn >= 1e-4 && gn > 0.0 && gn - gn == 0.0) {
+        for (int c = 0; c < 3; ++c) {
 #pragma unroll 4
-        for (int q = t; q < 3 * V; q += CT) {
-            const size_t j = (size_t)(q / V) * Vp + q % V;
-            const double d = gt[j] / gn;
-            dir[j] = d;
-            sl += dE[j] * d;
+            for (int v = t; v < V; v += CT) {
+                const size_t j = (size_t)c * Vp + v;
+                const double d = gt[j] / gn;
+                dir[j] = d;
+                sl += dE[j] * d;
+            }
         }
     }
     sl = block_sum(sl, sh);
@@ -803,31 +808,43 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
     }
 }
 
-__device__ void edge_prep_block(EdgeSoA& ed, const double* pos, int V, int E, double alpha,
-                                double eps) {
-#pragma unroll 4
-    for (int K = threadIdx.x; K < E; K += blockDim.x) {
-        const int a = ed.i1[K], b = ed.i2[K];
-        const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
-        const double bx = pos[b], by = pos[V + b], bz = pos[2 * V + b];
-        const double ex = bx - ax, ey = by - ay, ez = bz - az;
-        const double r = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
-        const double ell = r + eps;
-        double ux = 0, uy = 0, uz = 0;
-        if (!(r < 1e-14)) {
-            const double inv = 1.0 / r;
-            ux = inv * ex;
-            uy = inv * ey;
-            uz = inv * ez;
-        }
-        ed.ax[K] = ax; ed.ay[K] = ay; ed.az[K] = az;
-        ed.bx[K] = bx; ed.by[K] = by; ed.bz[K] = bz;
-        ed.ell[K] = ell;
-        ed.rlen[K] = r;
-        ed.ux[K] = ux; ed.uy[K] = uy; ed.uz[K] = uz;
-        ed.lpow[K] = pow(ell, 1.0 - alpha);
-        ed.dpc[K] = (1.0 - alpha) * pow(ell, -alpha);
+// |e| of edge K at positions pos (SoA x[V], y[V], z[V]).
+__device__ __forceinline__ double edge_len(const EdgeSoA& ed, const double* __restrict__ pos,
+                                           int V, int K) {
+    const int a = ed.i1[K], b = ed.i2[K];
+    const double ex = pos[b] - pos[a], ey = pos[V + b] - pos[V + a];
+    const double ez = pos[2 * V + b] - pos[2 * V + a];
+    return __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+}
+
+// Row K of the edge table at positions pos, as k_edge_prep forms it;
+// returns |e| and the guarded unit tangent in u.
+__device__ __forceinline__ double edge_row(EdgeSoA& ed, const double* __restrict__ pos, int V,
+                                           int K, double alpha, double eps, double (&u)[3]) {
+    const int a = ed.i1[K], b = ed.i2[K];
+    const double ax = pos[a], ay = pos[V + a], az = pos[2 * V + a];
+    const double bx = pos[b], by = pos[V + b], bz = pos[2 * V + b];
+    const double ex = bx - ax, ey = by - ay, ez = bz - az;
+    const double r = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+    const double ell = r + eps;
+    double ux = 0, uy = 0, uz = 0;
+    if (!(r < 1e-14)) {
+        const double inv = 1.0 / r;
+        ux = inv * ex;
+        uy = inv * ey;
+        uz = inv * ez;
     }
+    ed.ax[K] = ax; ed.ay[K] = ay; ed.az[K] = az;
+    ed.bx[K] = bx; ed.by[K] = by; ed.bz[K] = bz;
+    ed.ell[K] = ell;
+    ed.rlen[K] = r;
+    ed.ux[K] = ux; ed.uy[K] = uy; ed.uz[K] = uz;
+    ed.lpow[K] = pow(ell, 1.0 - alpha);
+    ed.dpc[K] = (1.0 - alpha) * pow(ell, -alpha);
+    u[0] = ux;
+    u[1] = uy;
+    u[2] = uz;
+    return r;
 }
 
 // Constraint check of the projection iterate cur (lineSearch.ts:221-249):
@@ -990,13 +1007,14 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
         __syncthreads();
         return nf > 0.0;
     }
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll 4
-    for (int q = t; q < 3 * V; q += CT) {
-        const int c = q / V, v = q % V;
-        double x = solv[k + c] / V;
-        for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-        if (!(x - x == 0.0)) nf = 1.0;
-        cur[(size_t)c * V + v] += x;
+        for (int v = t; v < V; v += CT) {
+            double x = solv[k + c] / V;
+            for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+            if (!(x - x == 0.0)) nf = 1.0;
+            cur[(size_t)c * V + v] += x;
+        }
     }
     nf = block_sum(nf, sh);
     __syncthreads();
@@ -1007,24 +1025,22 @@ __device__ __forceinline__ bool project_correct(int k, const double* __restrict_
 __device__ __forceinline__ void trial_candidate(const double* __restrict__ pos0,
                                                 const double* __restrict__ dir, double tau, int V,
                                                 int Vp, double* __restrict__ cur) {
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll 4
-    for (int q = threadIdx.x; q < 3 * V; q += CT) {
-        const int c = q / V, v = q % V;
-        cur[(size_t)c * V + v] = pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v];
+        for (int v = threadIdx.x; v < V; v += CT)
+            cur[(size_t)c * V + v] = pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v];
     }
     __syncthreads();
 }
 
 // End of a trial's projection: on success (and a finite candidate,
-// lineSearch.ts:525-527) the edge table and the penalty energy at cur feed
-// the energy kernel; otherwise the trial fails with 'projection_failed' and
-// tau shrinks. Workgroup-collective.
+// lineSearch.ts:525-527) the trial goes on to its energy (k_ls_edges forms
+// the edge table and the penalty energy at cur over the device, then
+// k_energy_ls); otherwise it fails with 'projection_failed' and tau
+// shrinks. Workgroup-collective.
 __device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, double phin,
-                                               const double* __restrict__ cur, EdgeSoA& ed, int V,
-                                               int E, const DevPen& pen,
-                                               const int32_t* __restrict__ vOff,
-                                               const int32_t* __restrict__ vEdge, double alpha,
-                                               double eps, double* sh) {
+                                               const double* __restrict__ cur, int V,
+                                               double* sh) {
     const int t = threadIdx.x;
     double nfc = 0.0;
 #pragma unroll 4
@@ -1032,12 +1048,7 @@ __device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, doubl
         if (!(cur[q] - cur[q] == 0.0)) nfc = 1.0;
     nfc = block_sum(nfc, sh);
     if (ok && nfc == 0.0) {
-        edge_prep_block(ed, cur, V, E, alpha, eps);
-        __syncthreads();
-        __threadfence_block();
-        const double pe = pen.active ? penalty_energy_block(ed, E, vOff, vEdge, V, pen, sh) : 0.0;
         if (t == 0) {
-            ctl->penE = pe;
             ctl->proj_iters = it;
             ctl->phi_norm = phin;
             ctl->proj_ok = 1;
@@ -1090,7 +1101,7 @@ __global__ __launch_bounds__(CT) void k_ls_project(
     }
     if (it > maxit) it = maxit;
     if (threadIdx.x == 0) ctl->proj_solves += nsolve;
-    project_finish(ctl, ok, it, phin, cur, ed, V, E, pen, vOff, vEdge, alpha, eps, sh);
+    project_finish(ctl, ok, it, phin, cur, V, sh);
 }
 
 // Reassemble-mode projection (lineSearch.ts:263-276): every correction uses
@@ -1152,7 +1163,7 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
         }
         if (bad) {
             if (t == 0) ctl->need_solve = 0;
-            project_finish(ctl, 0, it, INFINITY, cur, ed, V, E, pen, vOff, vEdge, alpha, eps, sh);
+            project_finish(ctl, 0, it, INFINITY, cur, V, sh);
             return;
         }
         ++it;
@@ -1162,8 +1173,7 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
                                     phin);
     if (conv || it == maxit) {
         if (t == 0) ctl->need_solve = 0;
-        project_finish(ctl, conv ? 1 : 0, it, phin, cur, ed, V, E, pen, vOff, vEdge, alpha, eps,
-                       sh);
+        project_finish(ctl, conv ? 1 : 0, it, phin, cur, V, sh);
         return;
     }
     for (int j = t; j < k; j += CT) phi_g[j] = phi[j];
@@ -1220,6 +1230,74 @@ __global__ __launch_bounds__(1024) void k_corr_apply(int k, const double* __rest
         if (!(x - x == 0.0)) atomicOr(&ctl->corr_bad, 1);
         cur[(size_t)c * V + v] += x;
     }
+}
+
+// Edge table of the projected trial (the energy kernel's input) and its
+// penalty energy (penalties.ts:139-190), over the device: a thread per edge
+// (table row, length and field terms) and per vertex (length-difference
+// term, its two edge lengths recomputed from the positions). Per-workgroup
+// partials; the last workgroup to finish (integer ticket, no float atomics)
+// sums them in workgroup order into ctl->penE. Skipped when the trial's
+// projection failed or the search is done.
+__global__ __launch_bounds__(256) void k_ls_edges(LSCtl* ctl, const double* __restrict__ cur,
+                                                  EdgeSoA ed, int V, int E, DevPen pen,
+                                                  const int32_t* __restrict__ vOff,
+                                                  const int32_t* __restrict__ vEdge,
+                                                  double alpha, double eps,
+                                                  double* __restrict__ part) {
+    __shared__ double sh[32];
+    __shared__ int last;
+    if (ctl->skip_energy) return;
+    const int t = threadIdx.x;
+    const int stride = (int)gridDim.x * 256;
+    double sL = 0.0, sF = 0.0, sD = 0.0;
+    for (int K = (int)blockIdx.x * 256 + t; K < E; K += stride) {
+        double u[3];
+        const double l = edge_row(ed, cur, V, K, alpha, eps, u);
+        sL += l;
+        if (pen.wF != 0.0) {
+            const double cx = u[1] * pen.X[2] - u[2] * pen.X[1];
+            const double cy = u[2] * pen.X[0] - u[0] * pen.X[2];
+            const double cz = u[0] * pen.X[1] - u[1] * pen.X[0];
+            sF += l * (cx * cx + cy * cy + cz * cz);
+        }
+    }
+    if (!pen.active) {
+        if (blockIdx.x == 0 && t == 0) ctl->penE = 0.0;
+        return;
+    }
+    if (pen.wD != 0.0)
+        for (int w = (int)blockIdx.x * 256 + t; w < V; w += stride)
+            if (vOff[w + 1] - vOff[w] == 2) {
+                const double d = edge_len(ed, cur, V, vEdge[vOff[w]]) -
+                                 edge_len(ed, cur, V, vEdge[vOff[w] + 1]);
+                sD += d * d;
+            }
+    sL = block_sum(sL, sh);
+    sF = block_sum(sF, sh);
+    sD = block_sum(sD, sh);
+    if (t == 0) {
+        part[3 * blockIdx.x] = sL;
+        part[3 * blockIdx.x + 1] = sF;
+        part[3 * blockIdx.x + 2] = sD;
+        __threadfence();
+        last = atomicAdd(&ctl->counter, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || t != 0) return;
+    __threadfence();
+    double L = 0.0, F = 0.0, D = 0.0;
+    for (int i = 0; i < (int)gridDim.x; ++i) {
+        L += __hip_atomic_load(part + 3 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        F += __hip_atomic_load(part + 3 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        D += __hip_atomic_load(part + 3 * i + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    double tot = 0.0;
+    if (pen.wL != 0.0) tot += pen.wL * L;
+    if (pen.wD != 0.0) tot += pen.wD * D;
+    if (pen.wF != 0.0) tot += pen.wF * F;
+    ctl->penE = tot;
+    ctl->counter = 0;
 }
 
 // Energy of the projected trial (edge table) with the Armijo decision in the
@@ -1467,7 +1545,8 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
     return RC_OK;
 }
 
-rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, double* d_epart_ls);
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
+                              const double* cur, double* d_epart_ls);
 
 // One trial's candidate and frozen projection; rc_launch_ls_energy follows.
 rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
@@ -1507,10 +1586,17 @@ rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv
     return RC_OK;
 }
 
-// Energy of the projected trial and the Armijo decision (skipped on the
-// device when the trial's projection failed or the search is done).
-rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p,
-                              double* d_epart_ls) {
+// Energy of the projected trial cur: its edge table and penalty energy,
+// then the pair energy with the Armijo decision (both skipped on the device
+// when the trial's projection failed or the search is done). The penalty
+// partials use d_epart_ls before the pair energy's partials do.
+rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
+                              const double* cur, double* d_epart_ls) {
+    const int nwg = std::min((ctx->E + 255) / 256, 256);
+    hipLaunchKernelGGL(k_ls_edges, dim3(nwg), dim3(256), 0, ctx->stream, d_ctl, cur, ctx->ed,
+                       ctx->V, ctx->E, pen, (const int32_t*)ctx->d_vOff,
+                       (const int32_t*)ctx->d_vEdge, p->alpha, p->epsilon, d_epart_ls);
+    RC_CHECK_LAUNCH();
     int grid = (ctx->npairs + 3) / 4;
     if (grid > 1024) grid = 1024;
     if (p->alpha == 3.0 && p->beta == 6.0) {
