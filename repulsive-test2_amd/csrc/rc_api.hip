@@ -345,7 +345,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_set, 1);
     ALLOC(ctx->d_ctl, 1);
     ALLOC(ctx->d_Ax, 3 * (size_t)Vp);
-    ALLOC(ctx->d_stats, 16);
+    ALLOC(ctx->d_stats, 24);
     ALLOC(ctx->d_epart_ls, 1024);
     ALLOC(ctx->d_refine, 8);
     ALLOC(ctx->d_stage, 3 * (size_t)V);
@@ -1113,7 +1113,11 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     RC_CHECK_LAUNCH();
 
     const int ntr = trial_count(o->tau0, o->shrink, o->tau_min);
-    const int batches[3] = {4, 12, 1 << 20};
+    // trials are enqueued in batches, the host reading the done flag after
+    // each: the first trial alone (an accepted full step is the common case,
+    // and the trials launched past the decision still cost their launches),
+    // then 4, 12 and the rest
+    const int batches[4] = {1, 4, 12, 1 << 20};
     const bool wide = o->projection_frozen && ds.k > RC_KSMALL;
     int done_trials = (o->projection_frozen && !wide) ? 0 : ntr;
     if (!o->projection_frozen) {
@@ -1123,7 +1127,47 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         s = line_search_frozen_wide(ctx, p, dp, ds.k, ntr, o->max_iter, led);
         if (s != RC_OK) return s;
     }
-    for (int bi = 0; bi < 3 && done_trials < ntr; ++bi) {
+    // The step's tail (results, outputs, one staging copy) is enqueued after
+    // every batch and read back at the one synchronisation that follows: a
+    // search decided in the batch ends there; otherwise the next batch runs
+    // and the tail is enqueued again (the finalisation changes nothing until
+    // the search has accepted a trial).
+    bool waited_res = false;
+    auto tail = [&](bool& done) -> rc_status {
+        LED_CLOSE(led, id_ls, ctx->stream);
+        if (!waited_res) {
+            RC_HIP(hipStreamWaitEvent(ctx->stream, ctx->res_done, 0));
+            waited_res = true;
+        }
+        rc_status ts = rc_launch_ls_finalize(ctx, ctx->d_ctl, ctx->d_pos, ctx->d_work, update,
+                                             ctx->d_stats);
+        if (ts != RC_OK) return ts;
+        RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_stats, 24 * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        if (gtilde_out) {
+            hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0,
+                               ctx->stream, ctx->d_gt, ctx->V, Vp, ctx->d_tmp);
+            RC_CHECK_LAUNCH();
+            RC_HIP(hipMemcpyAsync(gtilde_out, ctx->d_tmp, sizeof(double) * 3 * ctx->V,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (xyz_out) {
+            // accepted -> the projected trial curve, otherwise the input echo
+            hipLaunchKernelGGL(k_pick_output, dim3((ctx->V + 255) / 256), dim3(256), 0,
+                               ctx->stream, (const double*)ctx->d_pos,
+                               (const double*)ctx->d_work, (const LSCtl*)ctx->d_ctl, ctx->V,
+                               ctx->d_stage);
+            RC_CHECK_LAUNCH();
+            RC_HIP(hipMemcpyAsync(xyz_out, ctx->d_stage, sizeof(double) * 3 * ctx->V,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        LED_CLOSE(led, id_step, ctx->stream);
+        RC_HIP(hipStreamSynchronize(ctx->stream));
+        done = *(const int32_t*)(ctx->d_host_stage + 14) != 0;
+        return RC_OK;
+    };
+    bool done = false, tail_read = false;
+    for (int bi = 0; bi < 4 && done_trials < ntr; ++bi) {
         const int nb = std::min(batches[bi], ntr - done_trials);
         for (int i = 0; i < nb; ++i) {
             int id_proj = -1, id_e = -1;
@@ -1138,42 +1182,12 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
             LED_CLOSE(led, id_e, ctx->stream);
         }
         done_trials += nb;
-        if (done_trials < ntr) {
-            RC_HIP(hipMemcpyAsync(ctx->d_host_stage, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-            RC_HIP(hipStreamSynchronize(ctx->stream));
-            if (*(int32_t*)ctx->d_host_stage) break;
-        }
+        if ((s = tail(done)) != RC_OK) return s;
+        tail_read = true;
+        if (done) break;
     }
-    LED_CLOSE(led, id_ls, ctx->stream);
-    RC_HIP(hipStreamWaitEvent(ctx->stream, ctx->res_done, 0));
-    s = rc_launch_ls_finalize(ctx, ctx->d_ctl, ctx->d_pos, ctx->d_work, update, ctx->d_stats);
-    if (s != RC_OK) return s;
-    RC_HIP(hipMemcpyAsync(ctx->d_host_stage, ctx->d_stats, 16 * sizeof(double),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    int32_t* hst = (int32_t*)(ctx->d_host_stage + 16);
-    RC_HIP(hipMemcpyAsync(hst, ctx->d_status, sizeof(int32_t) * 4, hipMemcpyDeviceToHost,
-                          ctx->stream));
-    RC_HIP(hipMemcpyAsync(ctx->d_host_stage + 24, ctx->d_refine + 2, sizeof(double) * 3,
-                          hipMemcpyDeviceToHost, ctx->stream));
-    if (gtilde_out) {
-        hipLaunchKernelGGL(k_blocks_to_aos, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
-                           ctx->d_gt, ctx->V, Vp, ctx->d_tmp);
-        RC_CHECK_LAUNCH();
-        RC_HIP(hipMemcpyAsync(gtilde_out, ctx->d_tmp, sizeof(double) * 3 * ctx->V,
-                              hipMemcpyDeviceToHost, ctx->stream));
-    }
-    if (xyz_out) {
-        // accepted -> the projected trial curve, otherwise the input echo
-        hipLaunchKernelGGL(k_pick_output, dim3((ctx->V + 255) / 256), dim3(256), 0, ctx->stream,
-                           (const double*)ctx->d_pos, (const double*)ctx->d_work,
-                           (const LSCtl*)ctx->d_ctl, ctx->V, ctx->d_stage);
-        RC_CHECK_LAUNCH();
-        RC_HIP(hipMemcpyAsync(xyz_out, ctx->d_stage, sizeof(double) * 3 * ctx->V,
-                              hipMemcpyDeviceToHost, ctx->stream));
-    }
-    LED_CLOSE(led, id_step, ctx->stream);
-    RC_HIP(hipStreamSynchronize(ctx->stream));
+    if (!tail_read && (s = tail(done)) != RC_OK) return s;
+    const int32_t* hst = (const int32_t*)(ctx->d_host_stage + 12);
     if (hst[0] & 2) {
         ctx->err = "device dependency wait timed out (triangular solve or factorisation)";
         return RC_HIP_ERROR;
@@ -1194,9 +1208,9 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     stats->reason = is[3];
     stats->trials = is[4];
     stats->refine_iters = is[5];
-    stats->saddle_ir_residual = d[24];
-    stats->saddle_ir_steps = (int32_t)d[25];
-    stats->saddle_ir_correction = d[26];
+    stats->saddle_ir_residual = d[16];
+    stats->saddle_ir_steps = (int32_t)d[17];
+    stats->saddle_ir_correction = d[18];
     stats->fp64_fallback = ctx->fp64_fallback;
     if (led) {
         // device-counted calls (trial energies, correction solves); the

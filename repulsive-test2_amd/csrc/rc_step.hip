@@ -1307,9 +1307,15 @@ __global__ __launch_bounds__(256) void k_energy_ls(EdgeSoA ed, int E, int npairs
                                                    double eps, double* __restrict__ part,
                                                    LSCtl* ctl);
 
+// Step results into one staging block the host reads with a single copy:
+// doubles 0-7 and ints at double 8 (rc_step_stats' order), the device status
+// words at double 12, the search's done flag at double 14, the refinement
+// diagnostics at doubles 16-18.
 __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restrict__ pos,
                                                     const double* __restrict__ cur, int V,
                                                     int update, const double* __restrict__ resid,
+                                                    const int32_t* __restrict__ status,
+                                                    const double* __restrict__ refine,
                                                     double* __restrict__ stats) {
     const int accepted = ctl->accepted;
     if (update && accepted)
@@ -1334,6 +1340,10 @@ __global__ __launch_bounds__(CT) void k_ls_finalize(LSCtl* ctl, double* __restri
         is[5] = ctl->refine_iters;
         is[6] = ctl->energy_evals;
         is[7] = ctl->proj_solves;
+        int32_t* st = (int32_t*)(stats + 12);
+        for (int q = 0; q < 4; ++q) st[q] = status[q];
+        *(int32_t*)(stats + 14) = ctl->done;
+        for (int q = 0; q < 3; ++q) stats[16 + q] = refine[q];
     }
 }
 
@@ -1615,7 +1625,8 @@ rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, con
 rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
                                 int update, double* d_stats) {
     hipLaunchKernelGGL(k_ls_finalize, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, pos, cur, ctx->V,
-                       update, (const double*)(ctx->d_scal + RC_SCAL_RESID), d_stats);
+                       update, (const double*)(ctx->d_scal + RC_SCAL_RESID),
+                       (const int32_t*)ctx->d_status, (const double*)(ctx->d_refine + 2), d_stats);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
