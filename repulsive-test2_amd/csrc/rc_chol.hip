@@ -1525,11 +1525,13 @@ rc_status rc_factor_fp64(rc_ctx* ctx, int nx) {
 // Factor S + delta I, S = D^-1/2 A_hat D^-1/2 (unit diagonal), into
 // ctx->d_F32 (FP32 storage, FP16x3 MFMA trailing updates); D^-1/2 in
 // ctx->d_dscale. kappa(A_hat) grows like V^2.4 and exceeds what an FP32
-// Cholesky survives (kappa u_f32 < ~1) from V ~ 2000 on, so the factored
-// matrix is shifted: delta = 2^-20 first, then 8x larger after each
-// breakdown (status bit 8, read back once per try), up to 4 tries. The shift
-// only weakens the preconditioner; the PCG in FP64 with the unshifted A_hat
-// fixes the solution.
+// Cholesky survives (kappa u_f32 < ~1) from a few thousand vertices on, so
+// the factored matrix is shifted: delta doubles after each breakdown
+// (status bit 8, read back once per try), up to 8 tries, from the last shift
+// that factored. The shift only weakens the preconditioner (PCG iterations
+// grow with it: 44 at 2^-23 against 109 at 2^-20 for V = 8192 in a CPU
+// model of this factor); the PCG in FP64 with the unshifted A_hat fixes the
+// solution.
 rc_status rc_factor_mixed(rc_ctx* ctx) {
     const int Vp = ctx->nblk * NB;
     if (!ctx->d_F32) {
@@ -1543,8 +1545,12 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
     hipLaunchKernelGGL(k_diag_scale, dim3((Vp + 255) / 256), dim3(256), 0, ctx->stream,
                        (const double*)ctx->d_F, ctx->V, Vp, Vp, ctx->d_scal + 1, ctx->d_dscale);
     RC_CHECK_LAUNCH();
-    double shift = 1.0 / 1048576.0;
-    for (int attempt = 0; attempt < 4; ++attempt, shift *= 8.0) {
+    // the smallest shift that factors (nearest the unshifted S) makes the
+    // best preconditioner: start from the last working one (2^-23 at first;
+    // the Gram matrix changes little from step to step) and double it after
+    // each breakdown
+    double shift = ctx->mixed_shift_start > 0.0 ? ctx->mixed_shift_start : 0x1p-23;
+    for (int attempt = 0; attempt < 8; ++attempt, shift *= 2.0) {
         RC_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int32_t) * 4, ctx->stream));
         hipLaunchKernelGGL(k_factor_prep<float>, dim3((Vp + 255) / 256, Vp), dim3(256), 0,
                            ctx->stream, (const double*)ctx->d_F, ctx->V, Vp, Vp, ctx->d_scal + 1,
@@ -1557,6 +1563,7 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
                               ctx->stream));
         RC_HIP(hipStreamSynchronize(ctx->stream));
         ctx->mixed_shift = shift;
+        ctx->mixed_shift_start = shift;
         if (!(st & 8)) break;
     }
     return RC_OK;

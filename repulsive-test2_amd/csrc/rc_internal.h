@@ -93,6 +93,7 @@ struct rc_ctx {
     float* d_Linv32 = nullptr;
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
     double mixed_shift = 0.0;     // diagonal shift of the last FP32 factorisation
+    double mixed_shift_start = 0.0; // first shift the next FP32 factorisation tries (0: 2^-23)
     int32_t fp64_fallback = 0;    // the last mixed solve was redone with the FP64 factor
     int32_t* d_flags = nullptr;   // triangular-solve block flags
     int32_t* d_fcnt = nullptr;    // factorisation completion counters [2][nblk]
