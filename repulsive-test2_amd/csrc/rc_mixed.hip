@@ -37,30 +37,36 @@ __global__ void k_scale_cols(const double* __restrict__ in, const double* __rest
     out[j] = (v < V) ? in[j] * d[v] : 0.0;
 }
 
-// out[c] = sum_v X[c][v] (v < V)
-__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ X, int V, int Vp,
-                                                double* __restrict__ out,
-                                                const int32_t* __restrict__ skip) {
+// out[c] = sum_v X[c][v] (v < V); one RED_T-thread workgroup per column
+__global__ __launch_bounds__(RED_T) void k_colsum(const double* __restrict__ X, int V, int Vp,
+                                                  double* __restrict__ out,
+                                                  const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double sh[4];
+    __shared__ double sh[RED_T / 64];
     const int c = blockIdx.x;
     double s = 0.0;
-    for (int v = threadIdx.x; v < V; v += 256) s += X[(size_t)c * Vp + v];
+#pragma unroll 8
+    for (int v = threadIdx.x; v < V; v += RED_T) s += X[(size_t)c * Vp + v];
     s = wsum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = 0; q < RED_T / 64; ++q) a += sh[q];
+        out[c] = a;
+    }
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Q = A_hat P for nc <= 16 columns of P (ld Vp): Q[c][u] = sum_j A[j][u] P[c][j]
-// + mu colsum[c] (A symmetric, so A's column strip u0 .. u0+15 serves as
-// rows). One workgroup per 16-wide strip; its four waves split j in four and
-// add their partials in LDS in a fixed order. v_mfma_f64_16x16x4f64 with A
-// operand P^T[c = lane % 16][j + lane / 16] and B operand
-// A[j + lane / 16][u0 + lane % 16]: every B fetch is four 128-byte row
-// segments, A is read once per product.
+// Q = A_hat P for nc <= 16 columns of P (ld Vp): Q[c][u] = sum_j A[u][j] P[c][j]
+// + mu colsum[c]. One workgroup per 16 rows u, read along the rows; its
+// four waves split j in four and add their partials in LDS in a fixed
+// order. v_mfma_f64_16x16x4f64 with A operand P[c = lane % 16][k] and B
+// operand A[u0 + lane % 16][k], where lane group g = lane / 16 takes
+// k = j + 4 g + m in the m-th of four products: each lane loads 32
+// consecutive bytes of its row of A (and of P) per four products, so a wave
+// reads 16 rows x 128 contiguous bytes per load, A once per product.
 __global__ __launch_bounds__(256) void k_ahat_mfma(const double* __restrict__ A, int V, int Vp,
                                                    const double* __restrict__ mu,
                                                    const double* __restrict__ colsum,
@@ -72,20 +78,39 @@ __global__ __launch_bounds__(256) void k_ahat_mfma(const double* __restrict__ A,
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int u0 = blockIdx.x * 16;
-    const int span = (((V + 3) / 4) + 3) & ~3;    // a multiple of 4
+    const int span = ((V + 63) / 64) * 16;      // a quarter of V, a multiple of 16
     const int j0 = w * span, j1 = min(V, j0 + span);
     const bool cval = lr < nc;
     const double* Pc = P + (size_t)(cval ? lr : 0) * Vp;
     const bool uval = u0 + lr < V;
-    const double* Au = A + (size_t)(uval ? u0 + lr : 0);
+    const double* Au = A + (size_t)(uval ? u0 + lr : 0) * Vp;
     d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 16
-    for (int j = j0; j < j1; j += 4) {
-        const int jj = j + lk;
-        const bool ok = jj < V;
-        const double a = (cval && ok) ? Pc[jj] : 0.0;
-        const double b = (uval && ok) ? Au[(size_t)jj * Vp] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    // 16 j per step, four steps' loads issued together (the last group's
+    // steps past j1 read the first step's addresses and add zeros)
+    const int nit = j1 > j0 ? (j1 - j0 + 15) / 16 : 0;
+    for (int it = 0; it < nit; it += 4) {
+        double2 a[4][2], p[4][2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int jj = j0 + 16 * (it + q < nit ? it + q : it) + 4 * lk;   // jj + 3 < Vp
+            a[q][0] = *reinterpret_cast<const double2*>(Au + jj);
+            a[q][1] = *reinterpret_cast<const double2*>(Au + jj + 2);
+            p[q][0] = *reinterpret_cast<const double2*>(Pc + jj);
+            p[q][1] = *reinterpret_cast<const double2*>(Pc + jj + 2);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int jj = j0 + 16 * (it + q) + 4 * lk;
+            const double av[4] = {a[q][0].x, a[q][0].y, a[q][1].x, a[q][1].y};
+            const double pv[4] = {p[q][0].x, p[q][0].y, p[q][1].x, p[q][1].y};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const bool ok = it + q < nit && jj + m < j1;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64((cval && ok) ? pv[m] : 0.0,
+                                                           (uval && ok) ? av[m] : 0.0, acc, 0,
+                                                           0, 0);
+            }
+        }
     }
     // acc[q] = partial Q[c = 4 q + lk][u = u0 + lr]
 #pragma unroll
@@ -97,20 +122,26 @@ __global__ __launch_bounds__(256) void k_ahat_mfma(const double* __restrict__ A,
             ((red[0][c][uu] + red[1][c][uu]) + red[2][c][uu]) + red[3][c][uu] + mu[0] * colsum[c];
 }
 
-// out[c] = <X[c], Y[c]> over v < V
-__global__ __launch_bounds__(256) void k_coldot(const double* __restrict__ X,
-                                                const double* __restrict__ Y, int V, int Vp,
-                                                double* __restrict__ out,
-                                                const int32_t* __restrict__ skip) {
+// out[c] = <X[c], Y[c]> over v < V; one RED_T-thread workgroup per column
+__global__ __launch_bounds__(RED_T) void k_coldot(const double* __restrict__ X,
+                                                  const double* __restrict__ Y, int V, int Vp,
+                                                  double* __restrict__ out,
+                                                  const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double sh[4];
+    __shared__ double sh[RED_T / 64];
     const int c = blockIdx.x;
     double s = 0.0;
-    for (int v = threadIdx.x; v < V; v += 256) s = fma(X[(size_t)c * Vp + v], Y[(size_t)c * Vp + v], s);
+#pragma unroll 8
+    for (int v = threadIdx.x; v < V; v += RED_T)
+        s = fma(X[(size_t)c * Vp + v], Y[(size_t)c * Vp + v], s);
     s = wsum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = 0; q < RED_T / 64; ++q) a += sh[q];
+        out[c] = a;
+    }
 }
 
 // R = B - Q
@@ -208,11 +239,12 @@ static rc_status pcg_bufs(rc_ctx* ctx, PcgBufs& b) {
     return RC_OK;
 }
 
-// M^-1 applied in place to nrhs columns of W (scale, FP32-factor solve, scale).
-static rc_status precond(rc_ctx* ctx, double* W, int nrhs, const int32_t* skip) {
+// W = M^-1 R for nrhs columns (scale, FP32-factor solve, scale); R may be W.
+static rc_status precond(rc_ctx* ctx, const double* R, double* W, int nrhs,
+                         const int32_t* skip) {
     const int Vp = ctx->nblk * RC_NB;
     dim3 g((Vp + 255) / 256, nrhs);
-    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, W, ctx->d_dscale, ctx->V, Vp,
+    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, R, ctx->d_dscale, ctx->V, Vp,
                        nrhs, W, skip);
     RC_CHECK_LAUNCH();
     rc_status s = rc_solve_f32_factor(ctx, W, nrhs, skip);
@@ -226,7 +258,7 @@ static rc_status precond(rc_ctx* ctx, double* W, int nrhs, const int32_t* skip) 
 static rc_status ahat_mult(rc_ctx* ctx, const double* X, double* Q, double* cs, int nrhs,
                            const int32_t* skip) {
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_colsum, dim3(nrhs), dim3(256), 0, ctx->stream, X, ctx->V, Vp, cs, skip);
+    hipLaunchKernelGGL(k_colsum, dim3(nrhs), dim3(RED_T), 0, ctx->stream, X, ctx->V, Vp, cs, skip);
     RC_CHECK_LAUNCH();
     for (int c0 = 0; c0 < nrhs; c0 += GEMM_CHUNK) {
         const int nc = (nrhs - c0) < GEMM_CHUNK ? (nrhs - c0) : GEMM_CHUNK;
@@ -251,24 +283,22 @@ rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, 
         ctx->err = "too many right-hand sides";
         return RC_BAD_ARG;
     }
-    const size_t bytes = sizeof(double) * (size_t)nrhs * Vp;
     const dim3 gv((ctx->V + 255) / 256, nrhs);
     RC_HIP(hipMemsetAsync(b.conv, 0, sizeof(int32_t) * (b.cap + 4), ctx->stream));
     RC_HIP(hipMemsetAsync(d_refine_out, 0, sizeof(double) * 2, ctx->stream));
     // X0 = M^-1 B
-    RC_HIP(hipMemcpyAsync(d_X, d_B, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((s = precond(ctx, d_X, nrhs, nullptr)) != RC_OK) return s;
+    if ((s = precond(ctx, d_B, d_X, nrhs, nullptr)) != RC_OK) return s;
     // R = B - A_hat X0, ||B||
     if ((s = ahat_mult(ctx, d_X, b.Q, b.cs, nrhs, nullptr)) != RC_OK) return s;
     hipLaunchKernelGGL(k_sub, dim3((unsigned)(((size_t)nrhs * Vp + 255) / 256)), dim3(256), 0,
                        ctx->stream, d_B, b.Q, Vp, nrhs, b.R);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, d_B, d_B, ctx->V, Vp, b.bn,
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, d_B, d_B, ctx->V, Vp, b.bn,
                        (const int32_t*)nullptr);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(256), 0, ctx->stream, b.bn, nrhs);
     RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.R, ctx->V, Vp, b.rr,
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, b.R, b.R, ctx->V, Vp, b.rr,
                        (const int32_t*)nullptr);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(64), 0, ctx->stream, b.rr, b.bn, nrhs, tol,
@@ -276,9 +306,8 @@ rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, 
     RC_CHECK_LAUNCH();
     const int32_t* skip = b.ctl;
     // Z = M^-1 R, rz = <R, Z>, P = Z
-    RC_HIP(hipMemcpyAsync(b.Z, b.R, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((s = precond(ctx, b.Z, nrhs, skip)) != RC_OK) return s;
-    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.Z, ctx->V, Vp, b.rzA,
+    if ((s = precond(ctx, b.R, b.Z, nrhs, skip)) != RC_OK) return s;
+    hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, b.R, b.Z, ctx->V, Vp, b.rzA,
                        skip);
     RC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pcg_dir, gv, dim3(256), 0, ctx->stream, b.Z, b.P, b.rzA, b.rzA, b.conv,
@@ -289,22 +318,21 @@ rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, 
     const int batch = 6;
     for (int it = 0; it < max_iter; ++it) {
         if ((s = ahat_mult(ctx, b.P, b.Q, b.cs, nrhs, skip)) != RC_OK) return s;
-        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.P, b.Q, ctx->V, Vp,
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, b.P, b.Q, ctx->V, Vp,
                            b.pq, skip);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_pcg_update, gv, dim3(256), 0, ctx->stream, d_X, b.R, b.P, b.Q, rz_old,
                            b.pq, b.conv, ctx->V, Vp, skip);
         RC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.R, ctx->V, Vp,
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, b.R, b.R, ctx->V, Vp,
                            b.rr, skip);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(64), 0, ctx->stream, b.rr, b.bn, nrhs, tol,
                            b.conv, b.ctl, d_refine_out, 1);
         RC_CHECK_LAUNCH();
         if (it + 1 == max_iter) break;
-        RC_HIP(hipMemcpyAsync(b.Z, b.R, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        if ((s = precond(ctx, b.Z, nrhs, skip)) != RC_OK) return s;
-        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(256), 0, ctx->stream, b.R, b.Z, ctx->V, Vp,
+        if ((s = precond(ctx, b.R, b.Z, nrhs, skip)) != RC_OK) return s;
+        hipLaunchKernelGGL(k_coldot, dim3(nrhs), dim3(RED_T), 0, ctx->stream, b.R, b.Z, ctx->V, Vp,
                            rz_new, skip);
         RC_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_pcg_dir, gv, dim3(256), 0, ctx->stream, b.Z, b.P, rz_new, rz_old,
