@@ -11,11 +11,9 @@
 #include "rc_step.h"
 
 rc_status rc_factor_fp64(rc_ctx* ctx, int nx);
-rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs);
 rc_status rc_factor_mixed(rc_ctx* ctx);
 rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, double tol,
                          int max_iter, double* d_refine_out);
-int rc_trsm_max_coresident(void);
 rc_status rc_launch_constraint_jacobian(rc_ctx* ctx, const DevSet* d_set, int k);
 rc_status rc_launch_penalty_grad(rc_ctx* ctx, const DevPen& pen, const double* gsrc, int Vsrc,
                                  double* gdst, int Vdst);
@@ -112,7 +110,7 @@ static void free_all(rc_ctx* ctx) {
                     ctx->d_work, ctx->d_edge_block, ctx->d_edge_iblock, ctx->d_epart,
                     ctx->d_gpart, ctx->d_grad, ctx->d_scal, ctx->d_rsp, ctx->d_rs, ctx->d_Adiag,
                     ctx->d_symv,
-                    ctx->d_F, ctx->d_F32, ctx->d_Linv128, ctx->d_Linv32, ctx->d_dscale,
+                    ctx->d_F, ctx->d_F32, ctx->d_Linv128, ctx->d_Linv32, ctx->d_X128m, ctx->d_dscale,
                     ctx->d_flags, ctx->d_ticket, ctx->d_fcnt, ctx->d_status, ctx->d_rhs, ctx->d_sol, ctx->d_tmp,
                     ctx->d_tmp2, ctx->d_tmp3, ctx->d_tmp32, ctx->d_C, ctx->d_phi,
                     ctx->d_blockdata, ctx->d_ell0, ctx->d_schur, ctx->d_ls, ctx->d_dir,
@@ -333,7 +331,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_Adiag, (size_t)Vp);
     ALLOC(ctx->d_symv, (size_t)ctx->nblk * ctx->nblk * 3 * RC_NB);
     ALLOC(ctx->d_Linv128, (size_t)ctx->nblk * RC_NB * RC_NB * 2);
-    ALLOC(ctx->d_flags, (size_t)ctx->nblk * RC_BWD_MAXCH);
+    ALLOC(ctx->d_flags, (size_t)ctx->nblk / 2 * RC_SWEEP_G * RC_BWD_MAXCH);
     ALLOC(ctx->d_ticket, 1);
     ALLOC(ctx->d_fcnt, 2 * (size_t)ctx->nblk);
     ALLOC(ctx->d_status, 4);
@@ -356,7 +354,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         return fail(RC_HIP_ERROR);
     }
     ctx->host_stage_bytes = 4096;
-    he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk * RC_BWD_MAXCH);
+    he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk / 2 * RC_SWEEP_G * RC_BWD_MAXCH);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
     if (he == hipSuccess) he = hipMemset(ctx->d_ticket, 0, sizeof(uint32_t));
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
@@ -377,8 +375,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
         ctx->err = std::string("hipMemcpy: ") + hipGetErrorString(he);
         return fail(RC_HIP_ERROR);
     }
-    const int cores = rc_trsm_max_coresident();
-    ctx->bwd_chunks = std::max(1, std::min(RC_BWD_MAXCH, cores / ctx->nblk));
+    ctx->bwd_chunks = rc_sweep_chunks(ctx->nblk);
     *out = ctx;
     return RC_OK;
 }

@@ -30,13 +30,12 @@
 //                 into fp16 hi + lo (x = hi + lo to ~2^-22) and
 //                 hi*hi^T + hi*lo^T + lo*hi^T runs on
 //                 v_mfma_f32_16x16x32_f16 with FP32 accumulation
-// Triangular solves (k_trsm_fwd / k_trsm_bwd, on either factor): one
-// workgroup per 64-row block, dependency flags in global memory (sync-free
-// substitution), FP64 MFMA products on the stored factor, one 64 x 64
-// product on the critical path per block; the diagonal solve is a product
-// with the stored L_kk^-1.
+// After either factorisation the blocks below the 128 x 128 block diagonal
+// are row-scaled by the diagonal blocks' inverses (rc_sweep.hip), which the
+// triangular solves read.
 #include "rc_internal.h"
 #include "rc_math.h"
+#include "rc_handoff.h"
 #include <utility>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -46,7 +45,6 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 namespace {
 
 constexpr int NB = RC_NB;
-constexpr int RHS_CHUNK = 32;
 
 // mu = trace(A) / V^2; the diagonal of A is kept in diag (the residual reads
 // A from the upper triangle of F and this diagonal once F holds the factor).
@@ -614,89 +612,6 @@ __device__ __forceinline__ void gemm_nt_f64(const double* __restrict__ A, int ld
     }
 }
 
-__device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t epoch, int32_t* status) {
-    if (threadIdx.x == 0) {
-        int spins = 0;
-        // poll with relaxed loads (an acquire load would invalidate the L2
-        // on every spin), then one acquire fence once the flag is seen
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1 << 24)) {
-                atomicOr(status, 2);
-                break;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-}
-
-// Write-through hand-off between the triangular sweeps' blocks. A block's
-// result tile is stored with agent-scope atomic stores (coherent at the
-// device level: never held dirty in one XCD's L2), every wave waits for its
-// stores to complete before the barrier, then one thread stores the flag;
-// consumers poll the flag and read such tiles only with agent-scope atomic
-// loads. A release fence would write back the XCD's whole L2 instead
-// (__threadfence() plus a release store: two writebacks and an invalidate),
-// 3.0 against 1.3 us per hop along a 128-block chain on this device.
-__device__ __forceinline__ void store_wt(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double load_wt(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void publish_wt(int32_t* flag, int32_t epoch) {
-    // the signal fences keep the compiler from moving the tile's stores
-    // across the wait for their completion
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void wait_flag_wt(const int32_t* flag, int32_t epoch, int32_t* status) {
-    if (threadIdx.x == 0) {
-        int spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1 << 24)) {
-                atomicOr(status, 2);
-                break;
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// A sweep block's position comes from a ticket counter, not from blockIdx:
-// tickets are taken in the order the workgroups start, and a block waits
-// only for blocks of smaller tickets, which are running or done, so the
-// sweep needs no co-residency of its grid. The counter only grows; each
-// launch passes the count issued before it (wraps modulo 2^32).
-__device__ __forceinline__ int take_ticket(uint32_t* ctr, uint32_t base) {
-    __shared__ int tk;
-    if (threadIdx.x == 0) tk = (int)(atomicAdd(ctr, 1u) - base);
-    __syncthreads();
-    return tk;
-}
-
-// Completion counter of a producer kernel whose workgroups wrote their
-// results with store_wt: every wave waits for its stores to complete, then
-// one thread adds 1 (consumers poll with wait_flag for the number of
-// producer workgroups; its acquire drops their stale L2 lines). A release
-// here would write back the XCD's L2, which the rest of the trailing update
-// keeps full of dirty lines.
-__device__ __forceinline__ void signal_done_wt(int32_t* cnt) {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Panel rows below the diagonal block: L_ik = A_ik L_kk^-T, 64 rows per
 // workgroup, in place. wcnt (nullable): the previous panel's column update
 // (k_syrk128 column blocks, other stream) must have counted `target` first.
@@ -1045,329 +960,6 @@ __global__ __launch_bounds__(256) void k_syrk_f16x3(float* __restrict__ F, int V
             }
 }
 
-// Inverses of the 64 x 64 diagonal blocks as the two factorisations store
-// them: the FP64 form keeps the 128 x 128 inverse of each panel (block r is
-// its diagonal 64 x 64 block r & 1, leading dimension 128), the FP32 form one
-// 64 x 64 inverse per block.
-template <typename T>
-struct InvLayout;
-template <>
-struct InvLayout<double> {
-    static constexpr int ld = PB;
-    __device__ static const double* block(const double* L, int r) {
-        return L + (size_t)(r >> 1) * PB * PB + (size_t)(r & 1) * (NB * PB + NB);
-    }
-};
-template <>
-struct InvLayout<float> {
-    static constexpr int ld = NB;
-    __device__ static const float* block(const float* L, int r) { return L + (size_t)r * NB * NB; }
-};
-
-// 8 consecutive factor entries as doubles
-__device__ __forceinline__ void load8(const double* p, double (&v)[8]) {
-#pragma unroll
-    for (int q = 0; q < 8; q += 2) {
-        const double2 x = *reinterpret_cast<const double2*>(p + q);
-        v[q] = x.x;
-        v[q + 1] = x.y;
-    }
-}
-__device__ __forceinline__ void load8(const float* p, double (&v)[8]) {
-#pragma unroll
-    for (int q = 0; q < 8; q += 4) {
-        const float4 x = *reinterpret_cast<const float4*>(p + q);
-        v[q] = x.x;
-        v[q + 1] = x.y;
-        v[q + 2] = x.z;
-        v[q + 3] = x.w;
-    }
-}
-
-// Backward substitution L^T X = Y (FP64 or FP32 factor, entries taken to
-// FP64 for the FP64 MFMA products) on nrhs columns of Y
-// (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
-// grid row y (flags row y: nblk flags); block (b, y) solves the 64-row block
-// r = nblk-1-b of its chunk. 8 waves, wave w owns the 16 x 16 output tile
-// (rows 16 (w & 3), columns 16 (w >> 2)).
-//   X_r = L_rr^-T (Y_r - sum_{c > r} L_cr^T X_c)
-//       = Z_r - P_r^T X_{r+1},  Z_r = L_rr^-T (Y_r - sum_{c >= r+2} L_cr^T X_c),
-//                               P_r = L_{r+1,r} L_rr^-1,
-// so only one 64 x 64 x nrhs product follows the wait for block r+1: Z_r is
-// formed while block r+1 is still solving, P_r at the start. P_r is computed
-// as MFMA results (P tile (nb, mb): P[16nb + 4q + lk][16mb + lr] in register
-// q), which is exactly the A-operand layout of -P_r^T for row block mb and
-// k-block nb, so it stays in registers.
-template <typename T>
-__global__ __launch_bounds__(512) void k_trsm_bwd(const T* __restrict__ F, int Vp,
-                                                  const T* __restrict__ Linv, double* B,
-                                                  int ldb, int nrhs, int nblk,
-                                                  int32_t* __restrict__ flags, int32_t epoch,
-                                                  int32_t* __restrict__ status,
-                                                  const int32_t* __restrict__ skip,
-                                                  uint32_t* __restrict__ ticket, uint32_t tbase) {
-    // ticket tk: step bx = tk / chunks of the sweep, column chunk by
-    const int tk = take_ticket(ticket, tbase);
-    if (skip && *skip) return;
-    const int nch = (int)gridDim.y;
-    if (tk < 0 || tk >= nblk * nch) {
-        if (threadIdx.x == 0) atomicOr(status, 2);
-        return;
-    }
-    const int bx = tk / nch, by = tk % nch;
-    __shared__ double Lt[NB][NB + 1];         // A operand L_cr^T (Lt[m][i]); staging at start
-    __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (X_c, the accumulator for Z)
-    // chunk by: columns 32 by .. of the launch's nrhs, its own flag row
-    B += (size_t)by * RHS_CHUNK * ldb;
-    nrhs = min(RHS_CHUNK, nrhs - by * RHS_CHUNK);
-    flags += (size_t)by * nblk;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int r = nblk - 1 - bx;
-    const size_t rb = (size_t)r * NB;
-    const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
-    // a wave whose 16 output columns are all padding (nrhs <= 16: waves 4-7)
-    // skips its MFMA products; it shares a SIMD with a wave that has work
-    const bool live = wc < nrhs;
-    // L_rr^-1 (row-major, leading dimension ldl)
-    const int ldl = InvLayout<T>::ld;
-    const T* Lrr = InvLayout<T>::block(Linv, r);
-    // factor tile rows tr, 8 consecutive columns from tc
-    const int tr = t >> 3, tc = (t & 7) * 8;
-    double v[8];
-    auto gload = [&](int cb) {                // L_{cb,r} rows tr, columns tc..tc+7 of block r
-        load8(F + ((size_t)cb * NB + tr) * Vp + rb + tc, v);
-    };
-    auto sstore_t = [&]() {                   // Lt[m][i] = L[i][m]
-#pragma unroll
-        for (int q = 0; q < 8; ++q) Lt[tc + q][tr] = v[q];
-    };
-    auto load_ys = [&](int cb) {              // Ys[m][c] = X_cb[m][c]
-#pragma unroll
-        for (int q = 0; q < NB * RHS_CHUNK / 512; ++q) {
-            const int e = t + 512 * q, m = e % NB, c = e / NB;
-            const int cc = c < nrhs ? c : nrhs - 1;
-            const double y = load_wt(B + (size_t)cc * ldb + (size_t)cb * NB + m);
-            Ys[m][c] = (c < nrhs) ? y : 0.0;
-        }
-    };
-    // this wave's rows of L_rr^-T as A operands: A[m = wr + lr][k = 4kk + lk]
-    double li[16];
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) li[kk] = (double)Lrr[(size_t)(4 * kk + lk) * ldl + wr + lr];
-    // -P_r^T rows wr.. as A operands for k-blocks nb: pt[nb][q]
-    double pt[4][4];
-    const int ndep = nblk - 1 - r;
-    if (ndep > 0) {
-        // stage L_{r+1,r} row-major in Lt (Lt[n][i]), then P = L_{r+1,r} L_rr^-1
-        gload(r + 1);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) Lt[tr][tc + q] = v[q];
-        __syncthreads();
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[16 * nb + lr][4 * kk + lk], li[kk],
-                                                           acc, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) pt[nb][q] = -acc[q];
-        }
-        __syncthreads();
-    }
-    d4 acc;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = wc + lr;
-        acc[q] = (c < nrhs) ? B[(size_t)c * ldb + rb + wr + 4 * q + lk] : 0.0;
-    }
-    // contributions of blocks c = nblk-1 .. r+2
-    if (ndep > 1) gload(nblk - 1);
-    for (int s = 0; s + 1 < ndep; ++s) {
-        const int cb = nblk - 1 - s;
-        wait_flag_wt(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
-        load_ys(cb);
-        sstore_t();
-        if (s + 2 < ndep) gload(nblk - 2 - s);
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
-                                                           Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // Z = L_rr^-T acc (acc through Ys as the B operand)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) Ys[wr + 4 * q + lk][wc + lr] = acc[q];
-    __syncthreads();
-    d4 x = {0.0, 0.0, 0.0, 0.0};
-    if (live) {
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk)
-            x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
-    }
-    if (ndep > 0) {
-        // X_r = Z - P^T X_{r+1}
-        wait_flag_wt(flags + r + 1, epoch, status);
-        load_ys(r + 1);
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    x = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                        pt[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr], x, 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = wc + lr;
-        if (c < nrhs) store_wt(B + (size_t)c * ldb + rb + wr + 4 * q + lk, x[q]);
-    }
-    publish_wt(flags + r, epoch);
-}
-
-// Forward substitution L Y = B (FP64 or FP32 factor) on nrhs columns of B
-// (column-major, leading dimension ldb), in place, RHS_CHUNK columns per
-// grid row y (flags row y); block (b, y) solves the 64-row block r = b.
-// The mirror of k_trsm_bwd:
-//   Y_r = L_rr^-1 (B_r - sum_{c < r} L_rc Y_c)
-//       = Z_r - Q_r Y_{r-1},  Z_r = L_rr^-1 (B_r - sum_{c <= r-2} L_rc Y_c),
-//                             Q_r = L_rr^-1 L_{r,r-1},
-// so one 64 x 64 x nrhs product follows the wait for block r-1. Q_r^T tiles
-// are formed at the start as MFMA results: tile (nb, mb) holds
-// Q[16 mb + lr][16 nb + 4 q + lk] in register q, the A-operand layout of
-// Q_r for row block mb and k-block nb.
-template <typename T>
-__global__ __launch_bounds__(512) void k_trsm_fwd(const T* __restrict__ F, int Vp,
-                                                  const T* __restrict__ Linv, double* B,
-                                                  int ldb, int nrhs, int nblk,
-                                                  int32_t* __restrict__ flags, int32_t epoch,
-                                                  int32_t* __restrict__ status,
-                                                  const int32_t* __restrict__ skip,
-                                                  uint32_t* __restrict__ ticket, uint32_t tbase) {
-    const int tk = take_ticket(ticket, tbase);
-    if (skip && *skip) return;
-    const int nch = (int)gridDim.y;
-    if (tk < 0 || tk >= nblk * nch) {
-        if (threadIdx.x == 0) atomicOr(status, 2);
-        return;
-    }
-    const int bx = tk / nch, by = tk % nch;
-    __shared__ double Lt[NB][NB + 1];         // A operand L_rc (Lt[i][m]); staging at start
-    __shared__ double Ys[NB][RHS_CHUNK + 1];  // B operand (Y_c, the accumulator for Z)
-    B += (size_t)by * RHS_CHUNK * ldb;
-    nrhs = min(RHS_CHUNK, nrhs - by * RHS_CHUNK);
-    flags += (size_t)by * nblk;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int r = bx;
-    const size_t rb = (size_t)r * NB;
-    const int mb = w & 3, wr = 16 * mb, wc = 16 * (w >> 2);
-    const bool live = wc < nrhs;
-    const int ldl = InvLayout<T>::ld;
-    const T* Lrr = InvLayout<T>::block(Linv, r);
-    const int tr = t >> 3, tc = (t & 7) * 8;
-    double v[8];
-    auto gload = [&](int cb) {                // L_{r,cb} rows tr, columns tc..tc+7
-        load8(F + (rb + tr) * Vp + (size_t)cb * NB + tc, v);
-    };
-    auto sstore = [&]() {                     // Lt[i][m] = L[i][m]
-#pragma unroll
-        for (int q = 0; q < 8; ++q) Lt[tr][tc + q] = v[q];
-    };
-    auto load_ys = [&](int cb) {              // Ys[m][c] = Y_cb[m][c]
-#pragma unroll
-        for (int q = 0; q < NB * RHS_CHUNK / 512; ++q) {
-            const int e = t + 512 * q, m = e % NB, c = e / NB;
-            const int cc = c < nrhs ? c : nrhs - 1;
-            const double y = load_wt(B + (size_t)cc * ldb + (size_t)cb * NB + m);
-            Ys[m][c] = (c < nrhs) ? y : 0.0;
-        }
-    };
-    // this wave's rows of L_rr^-1 as A operands: A[m = wr + lr][k = 4kk + lk]
-    double li[16];
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) li[kk] = (double)Lrr[(size_t)(wr + lr) * ldl + 4 * kk + lk];
-    double qa[4][4];
-    const int ndep = r;
-    if (ndep > 0) {
-        // stage L_{r,r-1} in Lt (Lt[i][m]); Q^T tile (nb, mb) =
-        // sum_k L_{r,r-1}[k][16 nb + i] L_rr^-1[16 mb + j][k]
-        gload(r - 1);
-        sstore();
-        __syncthreads();
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[4 * kk + lk][16 * nb + lr], li[kk],
-                                                           acc, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) qa[nb][q] = -acc[q];
-        }
-        __syncthreads();
-    }
-    d4 acc;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = wc + lr;
-        acc[q] = (c < nrhs) ? B[(size_t)c * ldb + rb + wr + 4 * q + lk] : 0.0;
-    }
-    // contributions of blocks c = 0 .. r-2
-    if (ndep > 1) gload(0);
-    for (int s = 0; s + 1 < ndep; ++s) {
-        const int cb = s;
-        wait_flag_wt(flags + cb, epoch, status);   // ends in a barrier: Lt / Ys are free again
-        load_ys(cb);
-        sstore();
-        if (s + 2 < ndep) gload(s + 1);
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-Lt[wr + lr][4 * kk + lk],
-                                                           Ys[4 * kk + lk][wc + lr], acc, 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // Z = L_rr^-1 acc (acc through Ys as the B operand)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) Ys[wr + 4 * q + lk][wc + lr] = acc[q];
-    __syncthreads();
-    d4 x = {0.0, 0.0, 0.0, 0.0};
-    if (live) {
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk)
-            x = __builtin_amdgcn_mfma_f64_16x16x4f64(li[kk], Ys[4 * kk + lk][wc + lr], x, 0, 0, 0);
-    }
-    if (ndep > 0) {
-        // Y_r = Z - Q Y_{r-1}
-        wait_flag_wt(flags + r - 1, epoch, status);
-        load_ys(r - 1);
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    x = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                        qa[nb][q], Ys[16 * nb + 4 * q + lk][wc + lr], x, 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = wc + lr;
-        if (c < nrhs) store_wt(B + (size_t)c * ldb + rb + wr + 4 * q + lk, x[q]);
-    }
-    publish_wt(flags + r, epoch);
-}
-
 // Mixed form: 64-column panels on the FP32 factor.
 rc_status factor_loop_mixed(rc_ctx* ctx, float* F, float* Linv) {
     const int Vp = ctx->nblk * NB;
@@ -1491,7 +1083,10 @@ rc_status factor_loop_fp64(rc_ctx* ctx, int nx) {
     // the main stream continues after all second-stream work
     RC_HIP(hipEventRecord(fin, ctx->stream2));
     RC_HIP(hipStreamWaitEvent(ctx->stream, fin, 0));
-    return RC_OK;
+    // the sweeps read the factor row-scaled by its diagonal blocks' inverses
+    // (scaling a row block while the factorisation still runs, on a third
+    // stream, delays the panel chain's launches by more than it saves)
+    return rc_ltilde_fp64(ctx);
 }
 
 }  // namespace
@@ -1566,92 +1161,5 @@ rc_status rc_factor_mixed(rc_ctx* ctx) {
         ctx->mixed_shift_start = shift;
         if (!(st & 8)) break;
     }
-    return RC_OK;
-}
-
-// L^T X = Y in place for nrhs columns of length Vp (column-major, ld = Vp)
-// with the FP64 factor; Y = L^-1 B comes out of the factorisation's
-// right-hand-side rows.
-rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs) {
-    const int Vp = ctx->nblk * NB;
-    // bwd_chunks chunks of RHS_CHUNK columns per launch: nblk x bwd_chunks
-    // workgroups (no more than the device holds at once, so that no
-    // workgroup of a later ticket waits for a slot while others spin)
-    const int per = RHS_CHUNK * ctx->bwd_chunks;
-    for (int c0 = 0; c0 < nrhs; c0 += per) {
-        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
-        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
-        const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_bwd<double>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
-                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
-                           d_Y + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
-                           ctx->d_status, (const int32_t*)nullptr, ctx->d_ticket, ctx->tickets);
-        RC_CHECK_LAUNCH();
-        ctx->tickets += (uint32_t)(ctx->nblk * nch);
-    }
-    return RC_OK;
-}
-
-// L Y = B in place for nrhs columns of length Vp (column-major, ld = Vp)
-// with the FP64 factor (the solves of the saddle refinement; the gradient
-// solve's forward substitution happens inside the factorisation).
-rc_status rc_solve_fp64_forward(rc_ctx* ctx, double* d_B, int nrhs) {
-    const int Vp = ctx->nblk * NB;
-    const int per = RHS_CHUNK * ctx->bwd_chunks;
-    for (int c0 = 0; c0 < nrhs; c0 += per) {
-        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
-        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
-        const int32_t ef = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_fwd<double>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
-                           (const double*)ctx->d_F, Vp, (const double*)ctx->d_Linv128,
-                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, ef,
-                           ctx->d_status, (const int32_t*)nullptr, ctx->d_ticket, ctx->tickets);
-        RC_CHECK_LAUNCH();
-        ctx->tickets += (uint32_t)(ctx->nblk * nch);
-    }
-    return RC_OK;
-}
-
-// Solve S Y = B in place with the FP32 factor (caller applies the scaling):
-// the forward and backward sweeps of the FP64 form on FP32 entries.
-rc_status rc_solve_f32_factor(rc_ctx* ctx, double* d_B, int nrhs, const int32_t* skip) {
-    const int Vp = ctx->nblk * NB;
-    const int per = RHS_CHUNK * ctx->bwd_chunks;
-    for (int c0 = 0; c0 < nrhs; c0 += per) {
-        const int nc = (nrhs - c0) < per ? (nrhs - c0) : per;
-        const int nch = (nc + RHS_CHUNK - 1) / RHS_CHUNK;
-        const int32_t ef = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_fwd<float>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
-                           (const float*)ctx->d_F32, Vp, (const float*)ctx->d_Linv32,
-                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, ef,
-                           ctx->d_status, skip, ctx->d_ticket, ctx->tickets);
-        RC_CHECK_LAUNCH();
-        ctx->tickets += (uint32_t)(ctx->nblk * nch);
-        const int32_t eb = ++ctx->epoch;
-        hipLaunchKernelGGL(k_trsm_bwd<float>, dim3(ctx->nblk, nch), dim3(512), 0, ctx->stream,
-                           (const float*)ctx->d_F32, Vp, (const float*)ctx->d_Linv32,
-                           d_B + (size_t)c0 * Vp, Vp, nc, ctx->nblk, ctx->d_flags, eb,
-                           ctx->d_status, skip, ctx->d_ticket, ctx->tickets);
-        RC_CHECK_LAUNCH();
-        ctx->tickets += (uint32_t)(ctx->nblk * nch);
-    }
-    return RC_OK;
-}
-
-int rc_trsm_max_coresident(void) {
-    // every sweep kernel has the same shape (512 threads, the same LDS);
-    // the smallest per-CU residency of the four instantiations bounds them
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return 0;
-    int per[4] = {0, 0, 0, 0};
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[0], k_trsm_bwd<double>, 512, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[1], k_trsm_fwd<double>, 512, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[2], k_trsm_bwd<float>, 512, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per[3], k_trsm_fwd<float>, 512, 0) != hipSuccess)
-        return 0;
-    int m = per[0];
-    for (int i = 1; i < 4; ++i) m = per[i] < m ? per[i] : m;
-    return m * ncu;
+    return rc_ltilde_f32(ctx);
 }

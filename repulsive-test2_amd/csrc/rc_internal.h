@@ -16,6 +16,7 @@
                             // (larger tiles are walked in chunks of this many)
 #define RC_NB 64            // Cholesky / triangular-solve block size
 #define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
+#define RC_SWEEP_G 4        // workgroups sharing one 128-row block of a triangular sweep
 #define RC_SCAL_RESID 8     // d_scal slot of the saddle residual
 #define RC_NOALLOC_EMAX 16384  // largest E of rc_energy_noalloc (its static scratch)
 #define RED_T 1024          // threads of the one-workgroup-per-sum reductions (few long sums)
@@ -84,13 +85,16 @@ struct rc_ctx {
     double* d_rs = nullptr;       // row sums W_K, Q_K (2E)
     double* d_F = nullptr;        // Vp x Vp FP64: A (Gram assembly output), then its
                                   // factor (lower) of A + mu 11^T with A kept in the
-                                  // strict upper triangle; below, the right-hand-side
-                                  // rows (B^T, then (L^-1 B)^T)
+                                  // strict upper triangle, the blocks below the 128 x 128
+                                  // block diagonal row-scaled (L~_RC = L_RR^-1 L_RC);
+                                  // below, the right-hand-side rows (B^T, then (L^-1 B)^T)
     double* d_Adiag = nullptr;    // diagonal of A (Vp), saved before the factorisation
     double* d_symv = nullptr;     // residual product partials [nblk][nblk][3][NB]
-    float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form)
+    float* d_F32 = nullptr;       // V x V FP32 scaled factor (mixed form), row-scaled below
+                                  // the 128 x 128 block diagonal like d_F
     double* d_Linv128 = nullptr;  // inverted 128-column diagonal blocks [nblk/2][128][128]
-    float* d_Linv32 = nullptr;
+    float* d_Linv32 = nullptr;    // inverted 64-column diagonal blocks of d_F32 [nblk][64][64]
+    double* d_X128m = nullptr;    // inverted 128-row diagonal blocks of d_F32 [nblk/2][128][128]
     double* d_dscale = nullptr;   // symmetric diagonal scaling (mixed form)
     double mixed_shift = 0.0;     // diagonal shift of the last FP32 factorisation
     double mixed_shift_start = 0.0; // first shift the next FP32 factorisation tries (0: 2^-23)
@@ -188,6 +192,15 @@ struct rc_ctx {
 
 // launchers (defined in the .hip files)
 rc_status rc_launch_edge_prep(rc_ctx* ctx, const double* pos, const rc_params* p);
+// triangular solves with the row-scaled factors (rc_sweep.hip)
+rc_status rc_ltilde_fp64(rc_ctx* ctx);
+rc_status rc_ltilde_f32(rc_ctx* ctx);
+rc_status rc_solve_fp64_forward(rc_ctx* ctx, double* d_B, int nrhs);
+rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs);
+rc_status rc_precond_f32(rc_ctx* ctx, const double* R, double* W, int nrhs,
+                         const int32_t* skip);
+int rc_trsm_max_coresident(void);
+int rc_sweep_chunks(int nblk);
 rc_status rc_launch_energy(rc_ctx* ctx, const rc_params* p, double* d_out);
 rc_status rc_launch_gradient(rc_ctx* ctx, const rc_params* p, double* d_grad_soa);
 rc_status rc_launch_gradient_fd(rc_ctx* ctx, const rc_params* p, double h, double* d_grad_soa);

@@ -13,8 +13,6 @@
 // return at once when every column has stopped.
 #include "rc_internal.h"
 
-rc_status rc_solve_f32_factor(rc_ctx* ctx, double* d_B, int nrhs, const int32_t* skip);
-
 namespace {
 
 constexpr int GEMM_CHUNK = 16;
@@ -23,18 +21,6 @@ __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
-}
-
-// out[c][v] = in[c][v] * d[v] (v < V), 0 on padding
-__global__ void k_scale_cols(const double* __restrict__ in, const double* __restrict__ d, int V,
-                             int Vp, int nrhs, double* __restrict__ out,
-                             const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (v >= Vp || c >= nrhs) return;
-    const size_t j = (size_t)c * Vp + v;
-    out[j] = (v < V) ? in[j] * d[v] : 0.0;
 }
 
 // out[c] = sum_v X[c][v] (v < V); one RED_T-thread workgroup per column
@@ -242,17 +228,7 @@ static rc_status pcg_bufs(rc_ctx* ctx, PcgBufs& b) {
 // W = M^-1 R for nrhs columns (scale, FP32-factor solve, scale); R may be W.
 static rc_status precond(rc_ctx* ctx, const double* R, double* W, int nrhs,
                          const int32_t* skip) {
-    const int Vp = ctx->nblk * RC_NB;
-    dim3 g((Vp + 255) / 256, nrhs);
-    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, R, ctx->d_dscale, ctx->V, Vp,
-                       nrhs, W, skip);
-    RC_CHECK_LAUNCH();
-    rc_status s = rc_solve_f32_factor(ctx, W, nrhs, skip);
-    if (s != RC_OK) return s;
-    hipLaunchKernelGGL(k_scale_cols, g, dim3(256), 0, ctx->stream, W, ctx->d_dscale, ctx->V, Vp,
-                       nrhs, W, skip);
-    RC_CHECK_LAUNCH();
-    return RC_OK;
+    return rc_precond_f32(ctx, R, W, nrhs, skip);
 }
 
 static rc_status ahat_mult(rc_ctx* ctx, const double* X, double* Q, double* cs, int nrhs,

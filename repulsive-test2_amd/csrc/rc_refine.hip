@@ -19,8 +19,6 @@
 #include "rc_internal.h"
 #include "rc_step.h"
 
-rc_status rc_solve_fp64_forward(rc_ctx* ctx, double* d_B, int nrhs);
-rc_status rc_solve_fp64_backward(rc_ctx* ctx, double* d_Y, int nrhs);
 rc_status rc_solve_mixed(rc_ctx* ctx, const double* d_B, double* d_X, int nrhs, double tol,
                          int max_iter, double* d_refine_out);
 
