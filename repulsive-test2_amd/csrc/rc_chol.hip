@@ -466,10 +466,9 @@ __global__ __launch_bounds__(512) void k_potrf128(double* __restrict__ F, int Vp
     // the later phases: block row r of X = L^-1 (D_r on the diagonal, the
     // transposed upper blocks of Am left of it; the zero blocks right of it
     // are never written, the buffer starts zeroed) after phase r, for the
-    // panel TRSM and the backward substitution; the block of L below the
-    // tile's first 64 x 64 diagonal block (L_{r+1,r} of the 64-row backward
-    // substitution) after phase 3. The diagonal blocks of L are used only
-    // through X, and F's upper triangle keeps A for the residual.
+    // panel TRSM, the row scaling of the factor and the triangular solves.
+    // The diagonal blocks of L are used only through X, and F's upper
+    // triangle keeps A for the residual.
     double* X = Linv128 + k0 * PB;
     auto store_xrow = [&](int r) __attribute__((always_inline)) {
         const int per_row = 8 * (r + 1);   // double2 per row
@@ -482,19 +481,11 @@ __global__ __launch_bounds__(512) void k_potrf128(double* __restrict__ F, int Vp
             *reinterpret_cast<double2*>(X + (size_t)(16 * r + a) * PB + m) = xv;
         }
     };
-    auto store_l10 = [&]() __attribute__((always_inline)) {
-#pragma unroll 4
-        for (int q = t; q < 64 * 32; q += 512) {
-            const int i = 64 + (q >> 5), m = 2 * (q & 31);
-            *reinterpret_cast<double2*>(F + (k0 + i) * Vp + k0 + m) = make_double2(Am[i][m], Am[i][m + 1]);
-        }
-    };
     if (w == 0) diag16(0);
     __syncthreads();
     for (int p = 0; p < PB / SB; ++p) {
         const int o = p * SB;
         if (p > 0) store_xrow(p - 1);
-        if (p == 4) store_l10();
         // (c) rows below the diagonal block: L_ip = A_ip D_p^T
         for (int i = p + 1 + w; i < PB / SB; i += 8) {
             d4 acc = {0.0, 0.0, 0.0, 0.0};
