@@ -118,7 +118,7 @@ static void free_all(rc_ctx* ctx) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     void* sp[] = {ctx->d_set, ctx->d_ctl, ctx->d_dots, ctx->d_bsum, ctx->d_ssol, ctx->d_Minv,
-                  ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_refine, ctx->d_ir,
+                  ctx->d_Ax, ctx->d_Cg, ctx->d_stats, ctx->d_epart_ls, ctx->d_lsp, ctx->d_gsync, ctx->d_refine, ctx->d_ir,
                   ctx->d_stage, ctx->d_bh_f, ctx->d_bh_i, ctx->d_bh_stats, ctx->d_bct_f,
                   ctx->d_bct_i, ctx->d_bh_front, ctx->d_bh_part};
     for (void* p : sp)
@@ -345,6 +345,8 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     ALLOC(ctx->d_Ax, 3 * (size_t)Vp);
     ALLOC(ctx->d_stats, 24);
     ALLOC(ctx->d_epart_ls, 1024);
+    ALLOC(ctx->d_lsp, (size_t)RC_LSG_NW * RC_LSG_PART);
+    ALLOC(ctx->d_gsync, 4);
     ALLOC(ctx->d_refine, 8);
     ALLOC(ctx->d_stage, 3 * (size_t)V);
 #undef ALLOC
@@ -357,6 +359,7 @@ extern "C" rc_status rc_ctx_create(const int32_t* edges, int32_t E, int32_t V, r
     he = hipMemset(ctx->d_flags, 0, sizeof(int32_t) * ctx->nblk / 2 * RC_SWEEP_G * RC_BWD_MAXCH);
     if (he == hipSuccess) he = hipMemset(ctx->d_status, 0, sizeof(int32_t) * 4);
     if (he == hipSuccess) he = hipMemset(ctx->d_ticket, 0, sizeof(uint32_t));
+    if (he == hipSuccess) he = hipMemset(ctx->d_gsync, 0, sizeof(uint32_t) * 4);
     if (he == hipSuccess) he = hipMemset(ctx->d_Adiag, 0, sizeof(double) * (size_t)Vp);
     // the inverse's zero blocks above the diagonal are never written
     if (he == hipSuccess)

@@ -143,6 +143,8 @@ struct rc_ctx {
     double* d_Cg = nullptr;
     double* d_stats = nullptr;
     double* d_epart_ls = nullptr;
+    double* d_lsp = nullptr;       // grid line-search partial records [RC_LSG_NW][RC_LSG_PART]
+    uint32_t* d_gsync = nullptr;   // grid line-search counters [4], zero between launches
     double* d_refine = nullptr;   // [0..1] PCG (residual, iterations); [2..7] saddle refinement
     double* d_ir = nullptr;       // saddle refinement workspace (rc_refine.hip), on first use
     size_t ir_doubles = 0;

@@ -5,6 +5,8 @@
 #include <cstdint>
 
 #define RC_MAXPIN 64
+#define RC_LSG_NW 64                 // workgroups of the grid-wide line-search kernels
+#define RC_LSG_PART (8 + RC_MAXPIN)  // doubles of one workgroup's partial record
 #define RC_KMAX 1024  // constraint rows supported on the device
 #define RC_KSMALL 60  // up to here the (k+3)^2 Schur system is factored in LDS; the
                       // context's k-sized buffers start at this capacity and grow

@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "rc_internal.h"
+#include "rc_handoff.h"
 #include "rc_math.h"
 #include "rc_pair_energy.h"
 #include "rc_step.h"
@@ -710,23 +711,57 @@ __global__ __launch_bounds__(CT) void k_residual(const double* __restrict__ Ax,
 // ---------------------------------------------------------------------------
 // line search
 
-// Initialise the control block after the solve: L2h norm, direction, slope,
-// converged / failure reasons (optimizer.ts:331-367, lineSearch.ts:459-497).
-__global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int Vp,
-                                                 const double* __restrict__ gt,
-                                                 const double* __restrict__ dE,
-                                                 double* __restrict__ dir,
-                                                 const double* __restrict__ e0tpe,
-                                                 const double* __restrict__ e0pen,
-                                                 const int32_t* __restrict__ status,
-                                                 int has_e0, double e0_given, double tau0,
-                                                 LSCtl* ctl) {
+// Grid-wide line-search kernels: up to RC_LSG_NW workgroups of LSG_T
+// threads. Workgroups hand their partial sums over as write-through records
+// (one per workgroup, RC_LSG_PART doubles), which are reduced in one fixed
+// order (lane w of one wave holds workgroup w), so every workgroup that
+// reduces them gets the same sums.
+constexpr int LSG_T = 256;
+
+// True in the workgroup that arrives last, after every workgroup's
+// write-through stores are complete; it resets the counter for the next
+// launch. Workgroup-collective.
+__device__ bool last_workgroup(uint32_t* cnt) {
+    __shared__ int last;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tk =
+            __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = tk == gridDim.x - 1;
+        if (last) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return last != 0;
+}
+
+// Sum of field f of the records part[w][RC_LSG_PART], w < gridDim.x, in
+// wave 0 (fixed order; every lane gets it). max: NaN-propagating maximum.
+__device__ __forceinline__ double record_reduce(const double* part, int f, bool max) {
+    const int lane = threadIdx.x & 63;
+    const double v =
+        lane < (int)gridDim.x ? load_wt(part + (size_t)lane * RC_LSG_PART + f) : 0.0;
+    return max ? wmax(v) : wsum(v);
+}
+
+// Control block after the solve, part 1 (optimizer.ts:331-367,
+// lineSearch.ts:459-497): the L2h norm of g~, its finite check and the
+// failure reasons that do not need the slope; part 2 (k_ls_setup_dir)
+// forms the direction and the slope.
+__global__ __launch_bounds__(LSG_T) void k_ls_setup_norm(EdgeSoA ed, int E, int V, int Vp,
+                                                         const double* __restrict__ gt,
+                                                         const double* __restrict__ e0tpe,
+                                                         const double* __restrict__ e0pen,
+                                                         const int32_t* __restrict__ status,
+                                                         int has_e0, double e0_given,
+                                                         double tau0, LSCtl* ctl,
+                                                         double* part, uint32_t* cnt) {
     __shared__ double sh[32];
-    __shared__ double gn_s;
-    const int t = threadIdx.x;
-    double s = 0.0;
-#pragma unroll 4
-    for (int K = t; K < E; K += CT) {
+    const int t = threadIdx.x, gid = blockIdx.x * LSG_T + t, gsz = gridDim.x * LSG_T;
+    double s = 0.0, nf = 0.0;
+    for (int K = gid; K < E; K += gsz) {
         const int a = ed.i1[K], b = ed.i2[K];
         const double l = ed.rlen[K];
         double n1 = 0.0, n2 = 0.0;
@@ -737,74 +772,90 @@ __global__ __launch_bounds__(CT) void k_ls_setup(EdgeSoA ed, int E, int V, int V
         }
         s += 0.5 * l * (n1 + n2);
     }
-    s = block_sum(s, sh);
-    const double e0 = has_e0 ? e0_given : (e0tpe[0] + e0pen[0]);
     // finite check of g~ (a non-finite solve is the reference's "treating as singular")
-    double nf = 0.0;
-    for (int c = 0; c < 3; ++c) {
-#pragma unroll 4
-        for (int v = t; v < V; v += CT) {
+    for (int c = 0; c < 3; ++c)
+        for (int v = gid; v < V; v += gsz) {
             const double x = gt[(size_t)c * Vp + v];
             if (!(x - x == 0.0)) nf = 1.0;
         }
-    }
+    s = block_sum(s, sh);
     nf = block_sum(nf, sh);
-    if (t == 0) gn_s = sqrt(s > 0.0 ? s : 0.0);
-    __syncthreads();
-    const double gn = gn_s;
-    const bool singular = (status[0] & (1 | 4)) != 0 || nf > 0.0;
-    double sl = 0.0;
-    if (!singular && gn >= 1e-4 && gn > 0.0 && gn - gn == 0.0) {
-        for (int c = 0; c < 3; ++c) {
-#pragma unroll 4
-            for (int v = t; v < V; v += CT) {
-                const size_t j = (size_t)c * Vp + v;
-                const double d = gt[j] / gn;
-                dir[j] = d;
-                sl += dE[j] * d;
-            }
-        }
-    }
-    sl = block_sum(sl, sh);
     if (t == 0) {
-        ctl->e0 = e0;
-        ctl->e1 = e0;
-        ctl->tau = tau0;
-        ctl->gnorm = gn;
-        ctl->slope = sl;
-        ctl->accepted = 0;
-        ctl->converged = 0;
-        ctl->trials = 0;
-        ctl->energy_evals = 0;
-        ctl->proj_solves = 0;
-        ctl->proj_iters = 0;
-        ctl->proj_iters_acc = -1;
-        ctl->tau_acc = 0.0;
-        ctl->phi_norm = NAN;
-        ctl->skip_energy = 1;
-        ctl->counter = 0;
-        ctl->last_reason = RC_REASON_TAU_BELOW_MIN;
-        ctl->reason = RC_REASON_NONE;
-        ctl->done = 0;
-        if (singular) {
-            ctl->done = 1;
-            ctl->reason = RC_REASON_SINGULAR_SYSTEM;
-            ctl->residual = NAN;
-            ctl->gnorm = NAN;
-        } else if (gn < 1e-4) {
-            // optimizer.ts:355-367 (a NaN norm fails this test and goes on)
-            ctl->done = 1;
-            ctl->converged = 1;
-            ctl->proj_iters_acc = 0;
-        } else if (!(gn - gn == 0.0) || gn <= 0.0) {
-            ctl->done = 1;
-            ctl->reason = RC_REASON_ZERO_OR_NONFINITE_GRADIENT_NORM;
-            ctl->proj_iters_acc = 0;
-        } else if (!(sl - sl == 0.0) || sl <= 0.0) {
-            ctl->done = 1;
-            ctl->reason = RC_REASON_NOT_A_DESCENT_DIRECTION;
-            ctl->proj_iters_acc = 0;
+        store_wt(part + (size_t)blockIdx.x * RC_LSG_PART, s);
+        store_wt(part + (size_t)blockIdx.x * RC_LSG_PART + 1, nf);
+    }
+    if (!last_workgroup(cnt) || t >= 64) return;
+    s = record_reduce(part, 0, false);
+    nf = record_reduce(part, 1, false);
+    if (t != 0) return;
+    const double e0 = has_e0 ? e0_given : (e0tpe[0] + e0pen[0]);
+    const double gn = sqrt(s > 0.0 ? s : 0.0);
+    const bool singular = (status[0] & (1 | 4)) != 0 || nf > 0.0;
+    ctl->e0 = e0;
+    ctl->e1 = e0;
+    ctl->tau = tau0;
+    ctl->gnorm = gn;
+    ctl->slope = 0.0;
+    ctl->accepted = 0;
+    ctl->converged = 0;
+    ctl->trials = 0;
+    ctl->energy_evals = 0;
+    ctl->proj_solves = 0;
+    ctl->proj_iters = 0;
+    ctl->proj_iters_acc = -1;
+    ctl->tau_acc = 0.0;
+    ctl->phi_norm = NAN;
+    ctl->skip_energy = 1;
+    ctl->counter = 0;
+    ctl->last_reason = RC_REASON_TAU_BELOW_MIN;
+    ctl->reason = RC_REASON_NONE;
+    ctl->done = 0;
+    if (singular) {
+        ctl->done = 1;
+        ctl->reason = RC_REASON_SINGULAR_SYSTEM;
+        ctl->residual = NAN;
+        ctl->gnorm = NAN;
+    } else if (gn < 1e-4) {
+        // optimizer.ts:355-367 (a NaN norm fails this test and goes on)
+        ctl->done = 1;
+        ctl->converged = 1;
+        ctl->proj_iters_acc = 0;
+    } else if (!(gn - gn == 0.0) || gn <= 0.0) {
+        ctl->done = 1;
+        ctl->reason = RC_REASON_ZERO_OR_NONFINITE_GRADIENT_NORM;
+        ctl->proj_iters_acc = 0;
+    }
+}
+
+// Control block, part 2: direction g~ / |g~|_L2h and slope dE . dir; not a
+// descent direction ends the search. Nothing to do once part 1 ended it.
+__global__ __launch_bounds__(LSG_T) void k_ls_setup_dir(int V, int Vp,
+                                                        const double* __restrict__ gt,
+                                                        const double* __restrict__ dE,
+                                                        double* __restrict__ dir, LSCtl* ctl,
+                                                        double* part, uint32_t* cnt) {
+    __shared__ double sh[32];
+    if (ctl->done) return;
+    const int t = threadIdx.x, gid = blockIdx.x * LSG_T + t, gsz = gridDim.x * LSG_T;
+    const double gn = ctl->gnorm;
+    double sl = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int v = gid; v < V; v += gsz) {
+            const size_t j = (size_t)c * Vp + v;
+            const double d = gt[j] / gn;
+            dir[j] = d;
+            sl += dE[j] * d;
         }
+    sl = block_sum(sl, sh);
+    if (t == 0) store_wt(part + (size_t)blockIdx.x * RC_LSG_PART, sl);
+    if (!last_workgroup(cnt) || t >= 64) return;
+    sl = record_reduce(part, 0, false);
+    if (t != 0) return;
+    ctl->slope = sl;
+    if (!(sl - sl == 0.0) || sl <= 0.0) {
+        ctl->done = 1;
+        ctl->reason = RC_REASON_NOT_A_DESCENT_DIRECTION;
+        ctl->proj_iters_acc = 0;
     }
 }
 
@@ -845,6 +896,55 @@ __device__ __forceinline__ double edge_row(EdgeSoA& ed, const double* __restrict
     u[1] = uy;
     u[2] = uz;
     return r;
+}
+
+// Block tests of the projection iterate (lineSearch.ts:221-249) from its
+// sums over the edges and vertices (L, the barycenter moment m, the
+// per-edge residual norm pe2, the radii Rb and Rpin): stacks Phi in set
+// order into phi (a per-edge length block's rows are already there) and
+// tests every block's scaled tolerance; returns whether all pass, |Phi| in
+// phin. One thread. pin(pn, c): coordinate c of the vertex of pin pn (its
+// index checked here).
+template <class PinPos>
+__device__ bool check_blocks(const LSCtl* ctl, const DevSet* __restrict__ set, double L,
+                             double m0, double m1, double m2, double pe2, double Rb,
+                             const double* Rpin, PinPos pin, int V, double* phi, double& phin) {
+    bool conv = true;
+    double tot = 0.0;
+    for (int b = 0; b < set->nblocks; ++b) {
+        const int kind = set->kind[b], r0 = set->row0[b];
+        double bn2 = 0.0, scale = 1.0;
+        if (kind == RC_BLOCK_BARYCENTER) {
+            phi[r0] = m0; phi[r0 + 1] = m1; phi[r0 + 2] = m2;
+            bn2 = m0 * m0 + m1 * m1 + m2 * m2;
+            scale = js_max(1.0, L * js_max(1.0, Rb));
+        } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
+            phi[r0] = set->L0 - L;
+            bn2 = phi[r0] * phi[r0];
+            scale = js_max(1.0, L);
+        } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
+            bn2 = pe2;
+            scale = js_max(1.0, L);
+        } else {
+            const int pn = set->pin_of_block[b];
+            const int pv = set->pin_vertex[pn];
+            const double* tg = set->pin_target[pn];
+            if (pv >= 0 && pv < V) {
+                phi[r0] = pin(pn, 0) - tg[0];
+                phi[r0 + 1] = pin(pn, 1) - tg[1];
+                phi[r0 + 2] = pin(pn, 2) - tg[2];
+            } else {
+                phi[r0] = phi[r0 + 1] = phi[r0 + 2] = NAN;
+            }
+            bn2 = phi[r0] * phi[r0] + phi[r0 + 1] * phi[r0 + 1] + phi[r0 + 2] * phi[r0 + 2];
+            scale = js_max(1.0, Rpin[pn]);
+        }
+        tot += bn2;
+        const double bn = sqrt(bn2);
+        if (conv && bn > js_max(ctl->tol_abs, ctl->tol_rel * scale)) conv = false;
+    }
+    phin = sqrt(tot);
+    return conv;
 }
 
 // Constraint check of the projection iterate cur (lineSearch.ts:221-249):
@@ -905,43 +1005,10 @@ __device__ __forceinline__ bool project_check(const LSCtl* ctl, const DevSet* __
         if (t == 0) Rpin[pn] = R;
     }
     if (t == 0) {
-        // stack Phi in set order and test every block (lineSearch.ts:221-249)
-        bool conv = true;
-        double tot = 0.0;
-        for (int b = 0; b < set->nblocks; ++b) {
-            const int kind = set->kind[b], r0 = set->row0[b];
-            double bn2 = 0.0, scale = 1.0;
-            if (kind == RC_BLOCK_BARYCENTER) {
-                phi[r0] = m0; phi[r0 + 1] = m1; phi[r0 + 2] = m2;
-                bn2 = m0 * m0 + m1 * m1 + m2 * m2;
-                scale = js_max(1.0, L * js_max(1.0, Rb));
-            } else if (kind == RC_BLOCK_TOTAL_LENGTH) {
-                phi[r0] = set->L0 - L;
-                bn2 = phi[r0] * phi[r0];
-                scale = js_max(1.0, L);
-            } else if (kind == RC_BLOCK_EDGE_LENGTHS) {
-                bn2 = pe2;
-                scale = js_max(1.0, L);
-            } else {
-                const int pn = set->pin_of_block[b];
-                const int pv = set->pin_vertex[pn];
-                const double* tg = set->pin_target[pn];
-                if (pv >= 0 && pv < V) {
-                    phi[r0] = cur[pv] - tg[0];
-                    phi[r0 + 1] = cur[V + pv] - tg[1];
-                    phi[r0 + 2] = cur[2 * V + pv] - tg[2];
-                } else {
-                    phi[r0] = phi[r0 + 1] = phi[r0 + 2] = NAN;
-                }
-                bn2 = phi[r0] * phi[r0] + phi[r0 + 1] * phi[r0 + 1] + phi[r0 + 2] * phi[r0 + 2];
-                scale = js_max(1.0, Rpin[pn]);
-            }
-            tot += bn2;
-            const double bn = sqrt(bn2);
-            if (conv && bn > js_max(ctl->tol_abs, ctl->tol_rel * scale)) conv = false;
-        }
-        *flag_s = conv ? 1 : 0;
-        sh[31] = sqrt(tot);
+        double pn2 = 0.0;
+        auto pin = [&](int pn, int c) { return cur[(size_t)c * V + set->pin_vertex[pn]]; };
+        *flag_s = check_blocks(ctl, set, L, m0, m1, m2, pe2, Rb, Rpin, pin, V, phi, pn2) ? 1 : 0;
+        sh[31] = pn2;
     }
     __syncthreads();
     phin = sh[31];
@@ -1067,41 +1134,218 @@ __device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, doubl
     }
 }
 
-// One trial's candidate + frozen projection (lineSearch.ts:196-295,501-548).
-// Single workgroup. Y = X columns 3.. (A_hat^{-1} C^T blocks), Minv = M^{-1}.
-__global__ __launch_bounds__(CT) void k_ls_project(
-    LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0, DevPen pen,
-    const double* __restrict__ pos0, const double* __restrict__ dir, double* __restrict__ cur,
-    EdgeSoA ed, int V, int E, int Vp, const double* __restrict__ Y,
-    const double* __restrict__ Minv, const int32_t* __restrict__ vOff,
-    const int32_t* __restrict__ vEdge, double alpha, double eps) {
+// Grid barrier of the launch's gridDim.x workgroups (all co-resident: at
+// most RC_LSG_NW of LSG_T threads): each workgroup's write-through stores
+// complete, then one thread adds 1 and waits for target. Workgroup-collective.
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int32_t* status) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1 << 24)) {
+                atomicOr(status, 2);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One trial's candidate and frozen projection (lineSearch.ts:196-295,
+// 501-548) over the grid, for k <= RC_KSMALL. Each check: every workgroup
+// sums its edges and vertices (cur read write-through) into its record
+// (L, m0..m2, pe2, Rb, non-finite cur, non-finite previous correction,
+// Rpin) and the per-edge rows of Phi into phig; after a grid barrier every
+// workgroup reduces the records in the same order and runs the same block
+// tests, so all take the same branch; the correction [lambda; t] =
+// M^-1 [Phi; 0], cur += -Y lambda + N t / V is applied by each workgroup to
+// its own vertices, and a grid barrier precedes the next check. A
+// non-finite correction (lineSearch.ts:283-288) is seen in the next check's
+// records and fails the trial, as the reference's break does. Counters:
+// gcnt[0] the barrier, gcnt[1] the exits (the last workgroup to exit resets
+// both).
+__global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
+    LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
+    const double* __restrict__ pos0, const double* __restrict__ dir, double* cur, EdgeSoA ed,
+    int V, int E, int Vp, const double* __restrict__ Y, const double* __restrict__ Minv,
+    const int32_t* __restrict__ vOff, double* part, double* phig, uint32_t* gcnt,
+    int32_t* status) {
     __shared__ double sh[32];
-    __shared__ double phi[RC_KMAX + 8];
-    __shared__ double solv[RC_KMAX + 8];
-    __shared__ double Rpin[RC_MAXPIN];
-    __shared__ int flag_s;
+    __shared__ double phi[RC_KSMALL + 8];
+    __shared__ double solv[RC_KSMALL + 8];
+    __shared__ double rec[RC_LSG_PART];
+    __shared__ double pinx[RC_MAXPIN][3];
+    __shared__ int conv_s;
+    __shared__ double phin_s;
     if (ctl->done) {
-        if (threadIdx.x == 0) ctl->skip_energy = 1;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->skip_energy = 1;
         return;
     }
-    trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
-    int ok = 0, it = 0, nsolve = 0;
-    double phin = INFINITY;
+    const int t = threadIdx.x, nwg = (int)gridDim.x;
+    const int gid = blockIdx.x * LSG_T + t, gsz = nwg * LSG_T;
+    const int k = set->k, n = k + 3, npin = set->npin;
     const int maxit = ctl->max_iter;
+    double* myrec = part + (size_t)blockIdx.x * RC_LSG_PART;
+    // candidate cur = gamma0 - tau dir (lineSearch.ts:501-503)
+    {
+        const double tau = ctl->tau;
+        for (int c = 0; c < 3; ++c)
+            for (int v = gid; v < V; v += gsz)
+                store_wt(cur + (size_t)c * V + v,
+                         pos0[(size_t)c * V + v] - tau * dir[(size_t)c * Vp + v]);
+    }
+    uint32_t gen = 0;
+    double nf_corr = 0.0;
+    int ok = 0, it = 0, nsolve = 0;
+    double phin = INFINITY, nfc = 0.0;
     for (it = 0; it <= maxit; ++it) {
-        if (project_check(ctl, set, ell0, cur, ed, V, E, vOff, sh, phi, Rpin, &flag_s, phin)) {
+        grid_barrier(gcnt, (++gen) * (uint32_t)nwg, status);   // cur is complete
+        double L = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0, pe2 = 0.0, Rb = 0.0, nfw = 0.0;
+        for (int K = gid; K < E; K += gsz) {
+            const int a = ed.i1[K], b = ed.i2[K];
+            const double ax = load_wt(cur + a), ay = load_wt(cur + V + a),
+                         az = load_wt(cur + 2 * V + a);
+            const double bx = load_wt(cur + b), by = load_wt(cur + V + b),
+                         bz = load_wt(cur + 2 * V + b);
+            const double ex = bx - ax, ey = by - ay, ez = bz - az;
+            const double l = __builtin_sqrt(ex * ex + ey * ey + ez * ez);
+            L += l;
+            if (set->has_bary) {
+                m0 += l * (0.5 * (ax + bx) - set->x0[0]);
+                m1 += l * (0.5 * (ay + by) - set->x0[1]);
+                m2 += l * (0.5 * (az + bz) - set->x0[2]);
+            }
+            if (set->len_per_edge) {
+                const double ph = ell0[K] - l;
+                store_wt(phig + set->len_row0 + K, ph);
+                pe2 += ph * ph;
+            }
+        }
+        for (int v = gid; v < V; v += gsz) {
+            const double x = load_wt(cur + v), y = load_wt(cur + V + v),
+                         z = load_wt(cur + 2 * V + v);
+            if (!(x - x == 0.0) || !(y - y == 0.0) || !(z - z == 0.0)) nfw = 1.0;
+            if (set->has_bary && vOff[v + 1] != vOff[v]) {
+                const double dx = x - set->x0[0], dy = y - set->x0[1], dz = z - set->x0[2];
+                Rb = js_max(Rb, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        L = block_sum(L, sh);
+        m0 = block_sum(m0, sh);
+        m1 = block_sum(m1, sh);
+        m2 = block_sum(m2, sh);
+        pe2 = block_sum(pe2, sh);
+        nfw = block_sum(nfw, sh);
+        Rb = block_max(Rb, sh);
+        if (t == 0) {
+            store_wt(myrec + 0, L);
+            store_wt(myrec + 1, m0);
+            store_wt(myrec + 2, m1);
+            store_wt(myrec + 3, m2);
+            store_wt(myrec + 4, pe2);
+            store_wt(myrec + 5, Rb);
+            store_wt(myrec + 6, nfw);
+            store_wt(myrec + 7, nf_corr);
+        }
+        for (int pn = 0; pn < npin; ++pn) {
+            double R = 0.0;
+            const double* tg = set->pin_target[pn];
+            for (int v = gid; v < V; v += gsz) {
+                const double dx = load_wt(cur + v) - tg[0], dy = load_wt(cur + V + v) - tg[1],
+                             dz = load_wt(cur + 2 * V + v) - tg[2];
+                R = js_max(R, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+            }
+            R = block_max(R, sh);
+            if (t == 0) store_wt(myrec + 8 + pn, R);
+        }
+        // the pins' positions, read before any workgroup moves on to its
+        // correction
+        for (int q = t; q < 3 * npin; q += LSG_T) {
+            const int pn = q / 3, c = q - 3 * pn, pv = set->pin_vertex[pn];
+            pinx[pn][c] = (pv >= 0 && pv < V) ? load_wt(cur + (size_t)c * V + pv) : 0.0;
+        }
+        grid_barrier(gcnt, (++gen) * (uint32_t)nwg, status);   // the records are complete
+        if (t < 64) {
+            for (int f = 0; f < 8 + npin; ++f) {
+                const double r = record_reduce(part, f, f == 5 || f >= 8);
+                if (t == 0) rec[f] = r;
+            }
+        }
+        if (set->len_per_edge)
+            for (int K = t; K < E; K += LSG_T)
+                phi[set->len_row0 + K] = load_wt(phig + set->len_row0 + K);
+        __syncthreads();
+        if (t == 0) {
+            double pn2 = 0.0;
+            auto pin = [&](int pn, int c) { return pinx[pn][c]; };
+            conv_s = check_blocks(ctl, set, rec[0], rec[1], rec[2], rec[3], rec[4], rec[5],
+                                  rec + 8, pin, V, phi, pn2) ? 1 : 0;
+            phin_s = pn2;
+        }
+        __syncthreads();
+        nfc = rec[6];
+        if (rec[7] > 0.0) break;   // the last correction was not finite: ok = false
+        phin = phin_s;
+        if (conv_s) {
             ok = 1;
             break;
         }
         if (it == maxit) break;
-        // frozen correction with Y and M^{-1} of gamma0
+        // frozen correction with Y and M^-1 of gamma0
         ++nsolve;
-        if (project_correct(set->k, Y, Minv, V, Vp, cur, phi, solv, sh))
-            break;  // non-finite step -> ok = false (lineSearch.ts:283-288)
+        for (int i = t; i < n; i += LSG_T) {
+            double sv = 0.0;
+            for (int j = 0; j < k; ++j) sv += Minv[i * n + j] * phi[j];
+            solv[i] = sv;
+        }
+        __syncthreads();
+        double nf = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int v = gid; v < V; v += gsz) {
+                double x = solv[k + c] / V;
+                for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+                if (!(x - x == 0.0)) nf = 1.0;
+                double* q = cur + (size_t)c * V + v;
+                store_wt(q, load_wt(q) + x);
+            }
+        nf_corr = block_sum(nf, sh);
     }
     if (it > maxit) it = maxit;
-    if (threadIdx.x == 0) ctl->proj_solves += nsolve;
-    project_finish(ctl, ok, it, phin, cur, V, sh);
+    // end of the trial (project_finish): workgroup 0 writes the control block
+    if (blockIdx.x == 0 && t == 0) {
+        ctl->proj_solves += nsolve;
+        if (ok && nfc == 0.0) {
+            ctl->proj_iters = it;
+            ctl->phi_norm = phin;
+            ctl->proj_ok = 1;
+            ctl->skip_energy = 0;
+        } else {
+            ctl->proj_ok = 0;
+            ctl->skip_energy = 1;
+            ctl->trials += 1;
+            ctl->last_reason = RC_REASON_PROJECTION_FAILED;
+            ctl->tau = ctl->tau * ctl->shrink;
+            if (!(ctl->tau >= ctl->tau_min)) {
+                ctl->done = 1;
+                ctl->reason = ctl->last_reason;
+            }
+        }
+    }
+    // every workgroup is past its last barrier once it gets here: the last
+    // one to arrive resets the counters for the next launch
+    if (t == 0) {
+        const uint32_t tk =
+            __hip_atomic_fetch_add(gcnt + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tk == (uint32_t)nwg - 1) {
+            __hip_atomic_store(gcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gcnt + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // Reassemble-mode projection (lineSearch.ts:263-276): every correction uses
@@ -1545,12 +1789,23 @@ rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, cons
     return RC_OK;
 }
 
+// workgroups of the grid-wide line-search kernels (co-resident: <= RC_LSG_NW)
+static int lsg_grid(const rc_ctx* ctx) {
+    const int n = std::max(ctx->E, ctx->V);
+    return std::max(1, std::min(RC_LSG_NW, (n + LSG_T - 1) / LSG_T));
+}
+
 rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
                              const double* e0tpe, const double* e0pen, int has_e0,
                              double e0_given, double tau0, LSCtl* d_ctl) {
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_ls_setup, dim3(1), dim3(CT), 0, ctx->stream, ctx->ed, ctx->E, ctx->V, Vp,
-                       gt, dE, dir, e0tpe, e0pen, ctx->d_status, has_e0, e0_given, tau0, d_ctl);
+    const dim3 grid(lsg_grid(ctx));
+    hipLaunchKernelGGL(k_ls_setup_norm, grid, dim3(LSG_T), 0, ctx->stream, ctx->ed, ctx->E,
+                       ctx->V, Vp, gt, e0tpe, e0pen, (const int32_t*)ctx->d_status, has_e0,
+                       e0_given, tau0, d_ctl, ctx->d_lsp, ctx->d_gsync + 2);
+    RC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ls_setup_dir, grid, dim3(LSG_T), 0, ctx->stream, ctx->V, Vp, gt, dE, dir,
+                       d_ctl, ctx->d_lsp, ctx->d_gsync + 3);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
@@ -1558,14 +1813,17 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
                               const double* cur, double* d_epart_ls);
 
-// One trial's candidate and frozen projection; rc_launch_ls_energy follows.
+// One trial's candidate and frozen projection (k <= RC_KSMALL) over the
+// grid; rc_launch_ls_energy follows.
 rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
                                const rc_params* p, const double* pos0, const double* dir,
                                double* cur, const double* Y, const double* Minv) {
+    (void)pen;
+    (void)p;
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_ls_project, dim3(1), dim3(CT), 0, ctx->stream, d_ctl, d_set, ctx->d_ell0,
-                       pen, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv, ctx->d_vOff,
-                       ctx->d_vEdge, p->alpha, p->epsilon);
+    hipLaunchKernelGGL(k_ls_project_grid, dim3(lsg_grid(ctx)), dim3(LSG_T), 0, ctx->stream, d_ctl,
+                       d_set, ctx->d_ell0, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
+                       ctx->d_vOff, ctx->d_lsp, ctx->d_phi, ctx->d_gsync, ctx->d_status);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
