@@ -27,17 +27,15 @@ rc_status rc_launch_residual(rc_ctx* ctx, hipStream_t st, const double* gt, cons
 rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, double* dir,
                              const double* e0tpe, const double* e0pen, int has_e0,
                              double e0_given, double tau0, LSCtl* d_ctl);
-rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
-                               const rc_params* p, const double* pos0, const double* dir,
-                               double* cur, const double* Y, const double* Minv);
+rc_status rc_launch_ls_project(rc_ctx* ctx, int k, LSCtl* d_ctl, const DevSet* d_set,
+                               const double* pos0, const double* dir, double* cur,
+                               const double* Y, const double* Minv);
 rc_status rc_launch_ls_finalize(rc_ctx* ctx, LSCtl* d_ctl, double* pos, const double* cur,
                                 int update, double* d_stats);
 rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const DevSet* d_set,
                                   const DevPen& pen, const rc_params* p, const double* pos0,
                                   const double* dir, double* cur, const double* Y,
                                   const double* Minv);
-rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv,
-                         const double* phi_g, double* solv, double* cur, LSCtl* d_ctl);
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
                               const double* cur, double* d_epart_ls);
 rc_status rc_saddle_refine(rc_ctx* ctx, int k, int precision, int nsteps, double pcg_tol,
@@ -973,49 +971,6 @@ static rc_status line_search_reassemble(rc_ctx* ctx, const rc_params* p, const D
     return RC_OK;
 }
 
-// Frozen projection of a set beyond the LDS Schur bound (k > RC_KSMALL): the
-// trial's phases as in the reassembled search, the correction with Y and
-// M^{-1} of gamma0 applied by grid-wide launches between them (one CU
-// streaming M^{-1} and Y per correction was the single-workgroup kernel's
-// bound). Same checks, iteration count and failure rules as k_ls_project.
-static rc_status line_search_frozen_wide(rc_ctx* ctx, const rc_params* p, const DevPen& dp,
-                                         int k, int ntr, int max_iter, Ledger* led) {
-    const int Vp = ctx->nblk * RC_NB;
-    const double* Y = ctx->d_sol + 3 * (size_t)Vp;
-    // solv (k + 3 doubles): the Schur workspace, idle once M^{-1} is formed
-    double* solv = ctx->d_schur;
-    int32_t* flag = (int32_t*)ctx->d_host_stage;
-    rc_status s = RC_OK;
-    for (int trial = 0; trial < ntr; ++trial) {
-        int id_proj = -1, id_e = -1;
-        LED_OPEN(led, LK_PROJ, ctx->stream, id_proj, trial);
-        s = rc_launch_ls_project_re(ctx, 0, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
-                                    ctx->d_work, Y, ctx->d_Minv);
-        if (s != RC_OK) return s;
-        for (int corr = 0; corr <= max_iter; ++corr) {
-            RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, need_solve),
-                                  sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-            RC_HIP(hipStreamSynchronize(ctx->stream));
-            if (!*flag) break;
-            s = rc_launch_corr(ctx, k, Y, ctx->d_Minv, ctx->d_phi, solv, ctx->d_work, ctx->d_ctl);
-            if (s == RC_OK)
-                s = rc_launch_ls_project_re(ctx, 2, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos,
-                                            ctx->d_dir, ctx->d_work, Y, ctx->d_Minv);
-            if (s != RC_OK) return s;
-        }
-        LED_CLOSE(led, id_proj, ctx->stream);
-        LED_OPEN(led, LK_ENERGY, ctx->stream, id_e, trial);
-        s = rc_launch_ls_energy(ctx, ctx->d_ctl, p, dp, ctx->d_work, ctx->d_epart_ls);
-        if (s != RC_OK) return s;
-        LED_CLOSE(led, id_e, ctx->stream);
-        RC_HIP(hipMemcpyAsync(flag, (char*)ctx->d_ctl + offsetof(LSCtl, done), sizeof(int32_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-        RC_HIP(hipStreamSynchronize(ctx->stream));
-        if (*flag) break;
-    }
-    return RC_OK;
-}
-
 // Device-resident step on ctx->d_pos. update: replace d_pos with the accepted
 // curve. Stats (and g~ / positions when requested) are the single D2H copy.
 static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
@@ -1118,13 +1073,9 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
     // and the trials launched past the decision still cost their launches),
     // then 4, 12 and the rest
     const int batches[4] = {1, 4, 12, 1 << 20};
-    const bool wide = o->projection_frozen && ds.k > RC_KSMALL;
-    int done_trials = (o->projection_frozen && !wide) ? 0 : ntr;
+    int done_trials = o->projection_frozen ? 0 : ntr;
     if (!o->projection_frozen) {
         s = line_search_reassemble(ctx, p, ds, dp, ntr, o->max_iter, led);
-        if (s != RC_OK) return s;
-    } else if (wide) {
-        s = line_search_frozen_wide(ctx, p, dp, ds.k, ntr, o->max_iter, led);
         if (s != RC_OK) return s;
     }
     // The step's tail (results, outputs, one staging copy) is enqueued after
@@ -1172,7 +1123,7 @@ static rc_status step_impl(rc_ctx* ctx, const rc_block* blocks, int32_t nblocks,
         for (int i = 0; i < nb; ++i) {
             int id_proj = -1, id_e = -1;
             LED_OPEN(led, LK_PROJ, ctx->stream, id_proj, done_trials + i);
-            s = rc_launch_ls_project(ctx, ctx->d_ctl, ctx->d_set, dp, p, ctx->d_pos, ctx->d_dir,
+            s = rc_launch_ls_project(ctx, ds.k, ctx->d_ctl, ctx->d_set, ctx->d_pos, ctx->d_dir,
                                      ctx->d_work, ctx->d_sol + 3 * (size_t)Vp, ctx->d_Minv);
             if (s != RC_OK) return s;
             LED_CLOSE(led, id_proj, ctx->stream);

@@ -63,7 +63,6 @@ struct LSCtl {
     int32_t proj_iters_acc;
     int32_t need_solve;   // reassemble mode: the trial waits for a solve at cur
     int32_t proj_it;      // reassemble mode: corrections applied to the trial so far
-    int32_t corr_bad;     // a correction applied by k_corr_apply was not finite
     int32_t energy_evals; // trial energies evaluated (phase ledger)
     int32_t proj_solves;  // projection correction solves over all trials (phase ledger)
 };

@@ -1157,14 +1157,16 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int
 }
 
 // One trial's candidate and frozen projection (lineSearch.ts:196-295,
-// 501-548) over the grid, for k <= RC_KSMALL. Each check: every workgroup
+// 501-548) over the grid. Each check: every workgroup
 // sums its edges and vertices (cur read write-through) into its record
 // (L, m0..m2, pe2, Rb, non-finite cur, non-finite previous correction,
 // Rpin) and the per-edge rows of Phi into phig; after a grid barrier every
 // workgroup reduces the records in the same order and runs the same block
 // tests, so all take the same branch; the correction [lambda; t] =
 // M^-1 [Phi; 0], cur += -Y lambda + N t / V is applied by each workgroup to
-// its own vertices, and a grid barrier precedes the next check. A
+// its own vertices, and a grid barrier precedes the next check (beyond
+// RC_KSMALL rows, [lambda; t] is formed a wave per row over the grid and
+// handed over through solvg behind one more barrier). A
 // non-finite correction (lineSearch.ts:283-288) is seen in the next check's
 // records and fails the trial, as the reference's break does. Counters:
 // gcnt[0] the barrier, gcnt[1] the exits (the last workgroup to exit resets
@@ -1173,11 +1175,11 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
     LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
     const double* __restrict__ pos0, const double* __restrict__ dir, double* cur, EdgeSoA ed,
     int V, int E, int Vp, const double* __restrict__ Y, const double* __restrict__ Minv,
-    const int32_t* __restrict__ vOff, double* part, double* phig, uint32_t* gcnt,
+    const int32_t* __restrict__ vOff, double* part, double* phig, double* solvg, uint32_t* gcnt,
     int32_t* status) {
     __shared__ double sh[32];
-    __shared__ double phi[RC_KSMALL + 8];
-    __shared__ double solv[RC_KSMALL + 8];
+    __shared__ double phi[RC_KMAX + 8];
+    __shared__ double solv[RC_KMAX + 8];
     __shared__ double rec[RC_LSG_PART];
     __shared__ double pinx[RC_MAXPIN][3];
     __shared__ int conv_s;
@@ -1298,17 +1300,42 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
         if (it == maxit) break;
         // frozen correction with Y and M^-1 of gamma0
         ++nsolve;
-        for (int i = t; i < n; i += LSG_T) {
-            double sv = 0.0;
-            for (int j = 0; j < k; ++j) sv += Minv[i * n + j] * phi[j];
-            solv[i] = sv;
+        if (k <= RC_KSMALL) {
+            for (int i = t; i < n; i += LSG_T) {
+                double sv = 0.0;
+                for (int j = 0; j < k; ++j) sv += Minv[i * n + j] * phi[j];
+                solv[i] = sv;
+            }
+        } else {
+            // a wave per row of M^-1, its lanes over the columns (coalesced)
+            const int lane = t & 63, nwv = nwg * (LSG_T / 64);
+            for (int i = blockIdx.x * (LSG_T / 64) + (t >> 6); i < n; i += nwv) {
+                double sv = 0.0;
+                for (int j = lane; j < k; j += 64) sv += Minv[(size_t)i * n + j] * phi[j];
+                sv = wsum(sv);
+                if (lane == 0) store_wt(solvg + i, sv);
+            }
+            grid_barrier(gcnt, (++gen) * (uint32_t)nwg, status);   // [lambda; t] is complete
+            for (int i = t; i < n; i += LSG_T) solv[i] = load_wt(solvg + i);
         }
         __syncthreads();
         double nf = 0.0;
         for (int c = 0; c < 3; ++c)
             for (int v = gid; v < V; v += gsz) {
                 double x = solv[k + c] / V;
-                for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+                if (k <= RC_KSMALL) {
+                    for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+                } else {
+                    // four partial sums: independent loads in flight
+                    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+                    int p = 0;
+                    for (; p + 4 <= k; p += 4)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            s4[u] += solv[p + u] * Y[(size_t)(3 * (p + u) + c) * Vp + v];
+                    for (; p < k; ++p) s4[0] += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
+                    x -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                }
                 if (!(x - x == 0.0)) nf = 1.0;
                 double* q = cur + (size_t)c * V + v;
                 store_wt(q, load_wt(q) + x);
@@ -1354,10 +1381,7 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
 // phase 1 applies the correction with the Y and M^{-1} just factored at cur
 // (a singular system there fails the trial, as the reference's solve throws)
 // and checks again. Not converged with corrections left: need_solve = 1 and
-// Phi is kept in phi_g for the next launch. Single workgroup. The frozen
-// projection of a set beyond the LDS Schur bound runs the same phases with
-// the correction (Y and M^{-1} of gamma0) applied over the device by
-// k_corr_solv / k_corr_apply between them; phase 2 only checks.
+// Phi is kept in phi_g for the next launch. Single workgroup.
 __global__ __launch_bounds__(CT) void k_ls_project_re(
     int phase, LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
     DevPen pen, const double* __restrict__ pos0, const double* __restrict__ dir,
@@ -1382,28 +1406,16 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
     const int maxit = ctl->max_iter;
     int it = 0;
     if (phase == 0) {
-        if (t == 0) ctl->corr_bad = 0;
         trial_candidate(pos0, dir, ctl->tau, V, Vp, cur);
     } else {
         it = ctl->proj_it;
-        bool bad;
-        if (phase == 2) {
-            // frozen correction already applied to cur by k_corr_solv / k_corr_apply
-            bad = ctl->corr_bad != 0;
+        bool bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
+        if (!bad) {
+            // a solve that completed (the reference's throws on a singular system)
+            if (t == 0) ctl->proj_solves += 1;
+            for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
             __syncthreads();
-            if (t == 0) {
-                ctl->corr_bad = 0;
-                ctl->proj_solves += 1;
-            }
-        } else {
-            bad = (status[0] & ~2) != 0;  // breakdown or zero Schur pivot at cur
-            if (!bad) {
-                // a solve that completed (the reference's throws on a singular system)
-                if (t == 0) ctl->proj_solves += 1;
-                for (int j = t; j < k; j += CT) phi[j] = phi_g[j];
-                __syncthreads();
-                bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
-            }
+            bad = project_correct(k, Y, Minv, V, Vp, cur, phi, solv, sh);
         }
         if (bad) {
             if (t == 0) ctl->need_solve = 0;
@@ -1428,53 +1440,6 @@ __global__ __launch_bounds__(CT) void k_ls_project_re(
     }
 }
 
-// Frozen correction spread over the device (sets beyond the LDS Schur bound):
-// solv = M^{-1} [Phi; 0] with a wave per row, then x = -Y lambda + N t / V
-// added to cur, a workgroup per (64 vertices, component) whose 16 waves take
-// the rows p = w mod 16 (four partial sums each, combined in a fixed order).
-__global__ __launch_bounds__(256) void k_corr_solv(int k, const double* __restrict__ Minv,
-                                                   const double* __restrict__ phi_g,
-                                                   double* __restrict__ solv) {
-    const int n = k + 3;
-    const int lane = threadIdx.x & 63;
-    const int i = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    if (i >= n) return;  // whole waves
-    double s = 0.0;
-    for (int j = lane; j < k; j += 64) s += Minv[(size_t)i * n + j] * phi_g[j];
-    s = wsum(s);
-    if (lane == 0) solv[i] = s;
-}
-
-__global__ __launch_bounds__(1024) void k_corr_apply(int k, const double* __restrict__ Y,
-                                                     const double* __restrict__ solv, int V,
-                                                     int Vp, double* __restrict__ cur,
-                                                     LSCtl* ctl) {
-    __shared__ double red[16][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int c = blockIdx.y;
-    const int v = (int)blockIdx.x * 64 + lane;
-    const bool live = v < V;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    if (live) {
-        int p = w;
-        for (; p + 48 < k; p += 64) {
-            s0 += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-            s1 += solv[p + 16] * Y[(size_t)(3 * (p + 16) + c) * Vp + v];
-            s2 += solv[p + 32] * Y[(size_t)(3 * (p + 32) + c) * Vp + v];
-            s3 += solv[p + 48] * Y[(size_t)(3 * (p + 48) + c) * Vp + v];
-        }
-        for (; p < k; p += 16) s0 += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-    }
-    red[w][lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (w == 0 && live) {
-        double s = 0.0;
-        for (int q = 0; q < 16; ++q) s += red[q][lane];
-        const double x = solv[k + c] / V - s;
-        if (!(x - x == 0.0)) atomicOr(&ctl->corr_bad, 1);
-        cur[(size_t)c * V + v] += x;
-    }
-}
 
 // Edge table of the projected trial (the energy kernel's input) and its
 // penalty energy (penalties.ts:139-190), over the device: a thread per edge
@@ -1813,17 +1778,19 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
                               const double* cur, double* d_epart_ls);
 
-// One trial's candidate and frozen projection (k <= RC_KSMALL) over the
-// grid; rc_launch_ls_energy follows.
-rc_status rc_launch_ls_project(rc_ctx* ctx, LSCtl* d_ctl, const DevSet* d_set, const DevPen& pen,
-                               const rc_params* p, const double* pos0, const double* dir,
-                               double* cur, const double* Y, const double* Minv) {
-    (void)pen;
-    (void)p;
+// One trial's candidate and frozen projection over the grid (all
+// RC_LSG_NW workgroups beyond RC_KSMALL rows, for the rows of M^-1 Phi);
+// rc_launch_ls_energy follows. solvg (k + 3 doubles): the Schur workspace,
+// idle once M^-1 is formed.
+rc_status rc_launch_ls_project(rc_ctx* ctx, int k, LSCtl* d_ctl, const DevSet* d_set,
+                               const double* pos0, const double* dir, double* cur,
+                               const double* Y, const double* Minv) {
     const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_ls_project_grid, dim3(lsg_grid(ctx)), dim3(LSG_T), 0, ctx->stream, d_ctl,
-                       d_set, ctx->d_ell0, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
-                       ctx->d_vOff, ctx->d_lsp, ctx->d_phi, ctx->d_gsync, ctx->d_status);
+    const int nwg = k > RC_KSMALL ? RC_LSG_NW : lsg_grid(ctx);
+    hipLaunchKernelGGL(k_ls_project_grid, dim3(nwg), dim3(LSG_T), 0, ctx->stream, d_ctl, d_set,
+                       ctx->d_ell0, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
+                       ctx->d_vOff, ctx->d_lsp, ctx->d_phi, ctx->d_schur, ctx->d_gsync,
+                       ctx->d_status);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
@@ -1841,18 +1808,6 @@ rc_status rc_launch_ls_project_re(rc_ctx* ctx, int phase, LSCtl* d_ctl, const De
     return RC_OK;
 }
 
-rc_status rc_launch_corr(rc_ctx* ctx, int k, const double* Y, const double* Minv,
-                         const double* phi_g, double* solv, double* cur, LSCtl* d_ctl) {
-    const int n = k + 3;
-    const int Vp = ctx->nblk * RC_NB;
-    hipLaunchKernelGGL(k_corr_solv, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, k, Minv, phi_g,
-                       solv);
-    RC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_corr_apply, dim3((ctx->V + 63) / 64, 3), dim3(1024), 0, ctx->stream, k, Y,
-                       (const double*)solv, ctx->V, Vp, cur, d_ctl);
-    RC_CHECK_LAUNCH();
-    return RC_OK;
-}
 
 // Energy of the projected trial cur: its edge table and penalty energy,
 // then the pair energy with the Armijo decision (both skipped on the device
