@@ -1134,8 +1134,10 @@ __device__ __forceinline__ void project_finish(LSCtl* ctl, int ok, int it, doubl
     }
 }
 
+constexpr int WVL = 16;   // vertex lanes of the grid projection's wide correction
+
 // Grid barrier of the launch's gridDim.x workgroups (all co-resident: at
-// most RC_LSG_NW of LSG_T threads): each workgroup's write-through stores
+// most RC_LSG_NW): each workgroup's write-through stores
 // complete, then one thread adds 1 and waits for target. Workgroup-collective.
 __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int32_t* status) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1171,7 +1173,8 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int
 // records and fails the trial, as the reference's break does. Counters:
 // gcnt[0] the barrier, gcnt[1] the exits (the last workgroup to exit resets
 // both).
-__global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
+template <int T>
+__global__ __launch_bounds__(T) void k_ls_project_grid(
     LSCtl* ctl, const DevSet* __restrict__ set, const double* __restrict__ ell0,
     const double* __restrict__ pos0, const double* __restrict__ dir, double* cur, EdgeSoA ed,
     int V, int E, int Vp, const double* __restrict__ Y, const double* __restrict__ Minv,
@@ -1182,6 +1185,7 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
     __shared__ double solv[RC_KMAX + 8];
     __shared__ double rec[RC_LSG_PART];
     __shared__ double pinx[RC_MAXPIN][3];
+    __shared__ double wred[3][T / WVL][WVL];   // correction partial sums (k > RC_KSMALL)
     __shared__ int conv_s;
     __shared__ double phin_s;
     if (ctl->done) {
@@ -1189,7 +1193,7 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
         return;
     }
     const int t = threadIdx.x, nwg = (int)gridDim.x;
-    const int gid = blockIdx.x * LSG_T + t, gsz = nwg * LSG_T;
+    const int gid = blockIdx.x * T + t, gsz = nwg * T;
     const int k = set->k, n = k + 3, npin = set->npin;
     const int maxit = ctl->max_iter;
     double* myrec = part + (size_t)blockIdx.x * RC_LSG_PART;
@@ -1267,7 +1271,7 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
         }
         // the pins' positions, read before any workgroup moves on to its
         // correction
-        for (int q = t; q < 3 * npin; q += LSG_T) {
+        for (int q = t; q < 3 * npin; q += T) {
             const int pn = q / 3, c = q - 3 * pn, pv = set->pin_vertex[pn];
             pinx[pn][c] = (pv >= 0 && pv < V) ? load_wt(cur + (size_t)c * V + pv) : 0.0;
         }
@@ -1279,7 +1283,7 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
             }
         }
         if (set->len_per_edge)
-            for (int K = t; K < E; K += LSG_T)
+            for (int K = t; K < E; K += T)
                 phi[set->len_row0 + K] = load_wt(phig + set->len_row0 + K);
         __syncthreads();
         if (t == 0) {
@@ -1301,45 +1305,67 @@ __global__ __launch_bounds__(LSG_T) void k_ls_project_grid(
         // frozen correction with Y and M^-1 of gamma0
         ++nsolve;
         if (k <= RC_KSMALL) {
-            for (int i = t; i < n; i += LSG_T) {
+            for (int i = t; i < n; i += T) {
                 double sv = 0.0;
                 for (int j = 0; j < k; ++j) sv += Minv[i * n + j] * phi[j];
                 solv[i] = sv;
             }
         } else {
             // a wave per row of M^-1, its lanes over the columns (coalesced)
-            const int lane = t & 63, nwv = nwg * (LSG_T / 64);
-            for (int i = blockIdx.x * (LSG_T / 64) + (t >> 6); i < n; i += nwv) {
+            const int lane = t & 63, nwv = nwg * (T / 64);
+            for (int i = blockIdx.x * (T / 64) + (t >> 6); i < n; i += nwv) {
                 double sv = 0.0;
                 for (int j = lane; j < k; j += 64) sv += Minv[(size_t)i * n + j] * phi[j];
                 sv = wsum(sv);
                 if (lane == 0) store_wt(solvg + i, sv);
             }
             grid_barrier(gcnt, (++gen) * (uint32_t)nwg, status);   // [lambda; t] is complete
-            for (int i = t; i < n; i += LSG_T) solv[i] = load_wt(solvg + i);
+            for (int i = t; i < n; i += T) solv[i] = load_wt(solvg + i);
         }
         __syncthreads();
         double nf = 0.0;
-        for (int c = 0; c < 3; ++c)
-            for (int v = gid; v < V; v += gsz) {
-                double x = solv[k + c] / V;
-                if (k <= RC_KSMALL) {
+        if (k <= RC_KSMALL) {
+            for (int c = 0; c < 3; ++c)
+                for (int v = gid; v < V; v += gsz) {
+                    double x = solv[k + c] / V;
                     for (int p = 0; p < k; ++p) x -= solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-                } else {
-                    // four partial sums: independent loads in flight
-                    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-                    int p = 0;
-                    for (; p + 4 <= k; p += 4)
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            s4[u] += solv[p + u] * Y[(size_t)(3 * (p + u) + c) * Vp + v];
-                    for (; p < k; ++p) s4[0] += solv[p] * Y[(size_t)(3 * p + c) * Vp + v];
-                    x -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                    if (!(x - x == 0.0)) nf = 1.0;
+                    double* q = cur + (size_t)c * V + v;
+                    store_wt(q, load_wt(q) + x);
                 }
-                if (!(x - x == 0.0)) nf = 1.0;
-                double* q = cur + (size_t)c * V + v;
-                store_wt(q, load_wt(q) + x);
+        } else {
+            // this workgroup's vertices, VL at a time; lanes (vertex vl, row
+            // slice pl) sum rows p = pl mod PL, combined in LDS in that order
+            const int per = (V + nwg - 1) / nwg;
+            const int vb = blockIdx.x * per, ve = min(V, vb + per);
+            const int vl = t % WVL, pl = t / WVL;
+            for (int v0 = vb; v0 < ve; v0 += WVL) {
+                const int v = v0 + vl;
+                double sc[3] = {0.0, 0.0, 0.0};
+                if (v < ve) {
+#pragma unroll 4
+                    for (int p = pl; p < k; p += T / WVL) {
+                        const double lam = solv[p];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            sc[c] += lam * Y[(size_t)(3 * p + c) * Vp + v];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) wred[c][pl][vl] = sc[c];
+                __syncthreads();
+                if (t < 3 * WVL && v0 + t % WVL < ve) {
+                    const int c = t / WVL, vv = v0 + t % WVL;
+                    double sum = 0.0;
+                    for (int q = 0; q < T / WVL; ++q) sum += wred[c][q][t % WVL];
+                    const double x = solv[k + c] / V - sum;
+                    if (!(x - x == 0.0)) nf = 1.0;
+                    double* q = cur + (size_t)c * V + vv;
+                    store_wt(q, load_wt(q) + x);
+                }
+                __syncthreads();
             }
+        }
         nf_corr = block_sum(nf, sh);
     }
     if (it > maxit) it = maxit;
@@ -1778,19 +1804,25 @@ rc_status rc_launch_ls_setup(rc_ctx* ctx, const double* gt, const double* dE, do
 rc_status rc_launch_ls_energy(rc_ctx* ctx, LSCtl* d_ctl, const rc_params* p, const DevPen& pen,
                               const double* cur, double* d_epart_ls);
 
-// One trial's candidate and frozen projection over the grid (all
-// RC_LSG_NW workgroups beyond RC_KSMALL rows, for the rows of M^-1 Phi);
+// One trial's candidate and frozen projection over the grid (beyond
+// RC_KSMALL rows: RC_LSG_NW workgroups of 1024 threads, for the rows of
+// M^-1 Phi and the O(Vk) correction);
 // rc_launch_ls_energy follows. solvg (k + 3 doubles): the Schur workspace,
 // idle once M^-1 is formed.
 rc_status rc_launch_ls_project(rc_ctx* ctx, int k, LSCtl* d_ctl, const DevSet* d_set,
                                const double* pos0, const double* dir, double* cur,
                                const double* Y, const double* Minv) {
     const int Vp = ctx->nblk * RC_NB;
-    const int nwg = k > RC_KSMALL ? RC_LSG_NW : lsg_grid(ctx);
-    hipLaunchKernelGGL(k_ls_project_grid, dim3(nwg), dim3(LSG_T), 0, ctx->stream, d_ctl, d_set,
-                       ctx->d_ell0, pos0, dir, cur, ctx->ed, ctx->V, ctx->E, Vp, Y, Minv,
-                       ctx->d_vOff, ctx->d_lsp, ctx->d_phi, ctx->d_schur, ctx->d_gsync,
-                       ctx->d_status);
+    if (k <= RC_KSMALL)
+        hipLaunchKernelGGL(k_ls_project_grid<LSG_T>, dim3(lsg_grid(ctx)), dim3(LSG_T), 0,
+                           ctx->stream, d_ctl, d_set, ctx->d_ell0, pos0, dir, cur, ctx->ed,
+                           ctx->V, ctx->E, Vp, Y, Minv, ctx->d_vOff, ctx->d_lsp, ctx->d_phi,
+                           ctx->d_schur, ctx->d_gsync, ctx->d_status);
+    else
+        hipLaunchKernelGGL(k_ls_project_grid<1024>, dim3(RC_LSG_NW), dim3(1024), 0, ctx->stream,
+                           d_ctl, d_set, ctx->d_ell0, pos0, dir, cur, ctx->ed, ctx->V, ctx->E,
+                           Vp, Y, Minv, ctx->d_vOff, ctx->d_lsp, ctx->d_phi, ctx->d_schur,
+                           ctx->d_gsync, ctx->d_status);
     RC_CHECK_LAUNCH();
     return RC_OK;
 }
