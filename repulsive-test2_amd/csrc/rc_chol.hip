@@ -683,6 +683,37 @@ __device__ __forceinline__ void decode_rest(int q, int& ti, int& hj) {
     hj = q - (t - 1) * (t + 2);
 }
 
+// The same blocks of rows 1 <= ti <= nrow in 2-D order: bands of RB_ROWS
+// row tiles, each band by chunks of RB_COLS column halves (then row, then
+// column half), so that the blocks of a chunk, which an XCD runs at the same
+// time, share RB_ROWS row tiles of the panel (A operands) and RB_COLS column
+// halves (B operands) in that XCD's L2; in row order one row's blocks span
+// every column half, and the B operands miss L2 row after row.
+constexpr int RB_ROWS = 4, RB_COLS = 8;
+__device__ __forceinline__ void decode_rest_2d(int q, int nrow, int& ti, int& hj) {
+    int t0, h0;
+    decode_rest(q, t0, h0);
+    const int b0 = 1 + ((t0 - 1) / RB_ROWS) * RB_ROWS;
+    const int b1 = min(b0 + RB_ROWS - 1, nrow);
+    int r = q - (b0 - 1) * (b0 + 2);   // position in the band
+    ti = t0;
+    hj = h0;
+    for (int c0 = 0; c0 <= 2 * b1 + 1; c0 += RB_COLS) {
+        const int c1 = c0 + RB_COLS - 1;
+        for (int t = b0; t <= b1; ++t) {
+            const int hi = min(2 * t + 1, c1);
+            if (hi < c0) continue;
+            const int n = hi - c0 + 1;
+            if (r < n) {
+                ti = t;
+                hj = c0 + r;
+                return;
+            }
+            r -= n;
+        }
+    }
+}
+
 // C_rows(ri, rj) (64 x 128) -= P_i P_j^T with the panel columns k0 ..
 // (WT: write-through stores, for blocks another stream's kernel waits for)
 template <bool WT>
@@ -829,7 +860,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
     const int q = (r & 7) * per + (r >> 3);
     if (q >= sh.nrest) return;
     int ti, hj;
-    decode_rest(q, ti, hj);
+    decode_rest_2d(q, nt - 2, ti, hj);
     const size_t ri = c2 + (size_t)ti * PB, rj = c2 + (size_t)hj * 64;
     update_128x64<false>(F, Vp, k0, ri, rj, (hj >> 1) == ti, (hj & 1) * 64, mu0,
                   kb == 0 ? V - (int)ri : 0, kb == 0 ? V - (int)rj : 0, As, Bs);
