@@ -211,7 +211,10 @@ __device__ __forceinline__ void decode_lower(int p, int& ti, int& tj) {
 // ---- FP64 form: 128-column panels --------------------------------------
 constexpr int PB = 128;       // panel width of the FP64 factor
 constexpr int KC_TRSM = 16;   // K chunk staged through LDS: panel TRSM and trailing update
-constexpr int KC_SYRK = 16;   // (2 workgroups/CU; a 32 chunk at 1 workgroup/CU measured slower)
+// K chunk of the trailing update (2 workgroups/CU): 8 leaves its registers
+// unspilled (16 spilled 14 VGPRs: 76.1 against 77.3 us per launch); a 4
+// chunk takes 81 us, a 32 chunk at 1 workgroup/CU measured slower
+constexpr int KC_SYRK = 8;
 constexpr int LDT = PB + 4;   // LDS row length of the k-major operand tiles
 
 // Factor the 128x128 diagonal block of panel kb and invert it, in one
@@ -716,11 +719,11 @@ __device__ __forceinline__ void decode_rest_2d(int q, int nrow, int& ti, int& hj
 
 // C_rows(ri, rj) (64 x 128) -= P_i P_j^T with the panel columns k0 ..
 // (WT: write-through stores, for blocks another stream's kernel waits for)
-template <bool WT>
+template <bool WT, int KC = KC_TRSM>
 __device__ __forceinline__ void update_64x128(double* __restrict__ F, int Vp, size_t k0,
                                               size_t ri, size_t rj, double mu0, int vi, int vj,
-                                              double (*As)[KC_TRSM][LDT],
-                                              double (*Bs)[KC_TRSM][LDT]) {
+                                              double (*As)[KC][LDT],
+                                              double (*Bs)[KC][LDT]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 64;
     double* C = F + (ri + wr + (lane >> 4)) * Vp + rj + wc + (lane & 15);
@@ -730,7 +733,7 @@ __device__ __forceinline__ void update_64x128(double* __restrict__ F, int Vp, si
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
     double cv[2][4][4];
-    gemm_nt_f64<64, KC_TRSM>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc, [&](int ch) {
+    gemm_nt_f64<64, KC>(F + ri * Vp + k0, Vp, F + rj * Vp + k0, Vp, As, Bs, acc, [&](int ch) {
         if (ch == 0) {
 #pragma unroll
             for (int a = 0; a < 2; ++a)
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
         const size_t ri = wg < nreg ? c1 + PB + (size_t)(wg >> 1) * PB + 64 * (wg & 1)
                                     : (size_t)Vp + 64 * (size_t)(wg - nreg);
         const bool vv = kb == 0 && wg < nreg;
-        update_64x128<true>(F, Vp, k0, ri, c1, vv ? mu0 : 0.0, vv ? V - (int)ri : 0,
+        update_64x128<true, KC_SYRK>(F, Vp, k0, ri, c1, vv ? mu0 : 0.0, vv ? V - (int)ri : 0,
                             vv ? V - (int)c1 : 0, As, Bs);
         signal_done_wt(col_done);
         return;
@@ -850,7 +853,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk128(double* __restrict__ F, int 
         } else {
             // B: right-hand-side rows of columns kb+2 ..
             const int r = q - 2, nc = nt - 1;
-            update_64x128<false>(F, Vp, k0, (size_t)Vp + 64 * (size_t)(r / nc),
+            update_64x128<false, KC_SYRK>(F, Vp, k0, (size_t)Vp + 64 * (size_t)(r / nc),
                           c2 + (size_t)(r % nc) * PB, 0.0, 0, 0, As, Bs);
         }
         return;
