@@ -3,7 +3,7 @@ E = 8192 benchmark workload (closed Fourier curve V = E = 8192, seed 1,
 [barycenter, total length] frozen at the initial curve, field penalty w = 10,
 X = (0.5, 1, 0), frozen projection): tests/golden/c4_exact_step1.npz.
 
-Run on a GPU box (it reads the device-assembled A through the C ABI):
+Run on a machine with an MI355X (it reads the device-assembled A through the C ABI):
     python tests/golden/make_c4_exact.py [out.npz]
 
 Contents:

@@ -16,7 +16,7 @@
   with compensated residuals).
 - C5: 20 steps the same way, Grow-L 1.01 after each accepted step, and the
   constraint invariants at the end.
-Gates: energy 1e-10, direction 1e-8 (BASELINE), tau and projection
+Gates: energy 1e-10, direction 1e-8 (the benchmark's accuracy gates), tau and projection
 iterations exact."""
 import hashlib
 import os

@@ -10,7 +10,7 @@ LAPACK Cholesky plus refinement with the oracle's double-double residual
 (rc_step_opts.saddle_refine) and must land on it; the committed fixture
 tests/golden/c4_exact_step1.npz holds that reference for the first step of
 the E = 8192 benchmark workload (tests/golden/make_c4_exact.py).
-Gates (BASELINE): energy <= 1e-10 and direction <= 1e-8, relative; the
+Gates of the benchmark: energy <= 1e-10 and direction <= 1e-8, relative; the
 refined forms are held to 1e-13 in the direction here."""
 import hashlib
 import os
