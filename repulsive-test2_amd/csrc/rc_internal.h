@@ -12,8 +12,9 @@
 
 #define RC_TILE 64          // edges per tile of the pair kernels (one wave64 lane per edge)
 #define RC_VTILE 32         // vertices per ownership tile of the Gram assembly
-#define RC_RMAX 40          // incident edges of one vertex tile staged in LDS by the assembly
-                            // (larger tiles are walked in chunks of this many)
+#define RC_RMAX 36          // incident edges of one vertex tile staged in LDS by the assembly
+                            // (larger tiles are walked in chunks of this many; 36 keeps
+                            // the assembly at 39.7 KiB of LDS, four workgroups per CU)
 #define RC_NB 64            // Cholesky / triangular-solve block size
 #define RC_BWD_MAXCH 32     // most right-hand-side chunks in one backward-sweep launch
 #define RC_SWEEP_G 4        // workgroups sharing one 128-row block of a triangular sweep

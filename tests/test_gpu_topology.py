@@ -2,7 +2,7 @@
 the 3-D 'stress' grid (200 vertices of a jittered 6x6x6 lattice, degree up
 to 6) and 'random' graphs (random points, random edges, isolated vertices
 joined to a neighbour). Their 32-vertex tiles touch up to ~140 edges, beyond
-the assembly's LDS staging of 40, so the Gram matrix of these tiles comes
+the assembly's LDS staging of 36, so the Gram matrix of these tiles comes
 from the chunked path. Energy, gradient and A against the oracle at 1e-12,
 A exactly symmetric, and one Sobolev step against the oracle's (the
 reference's full saddle LDL^T): tau and projection iterations exact,
